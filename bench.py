@@ -1,0 +1,257 @@
+"""Benchmark of the SuperCountingBloom hot path on one MI355X.
+
+Workload (default geometry k=31 m=21 s=25 h=4, 16-bit counters, 2^30 slots,
+2^9 slots/block): seeded 5 Mbp uniform genome; insert 150 bp reads at 50x
+(1,666,667 reads, 1% substitutions, 0.1% N), then query the same reads plus a
+fresh 10x set (333,333 reads). One step = new builder (table zeroed in HBM) +
+insert of the whole read set + into_index + query of the whole query set with
+per-k-mer abundances and validity bits written to HBM. Inputs are resident in
+HBM before the timed region; outputs stay in HBM.
+
+Prints one JSON line. `value` is inserted+queried k-mers per second of the
+batched pipeline; the plain one-launch-per-batch method is timed on the same
+inputs and checked bit-exact against it; `cpu_baseline` is the oracle's
+restatement of the reference's multithreaded CPU path (1 producer, bounded
+queue, T workers, 1024 mutex shards) on a bounded sample of the same reads.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "G k-mers/s insert+query, batched vs plain GPU (bit-exact); served p50/p99 ms"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def gen_workload(seed=1):
+    from scbf_amd import synth
+    g = synth.genome(5_000_000, seed)
+    ins_b, ins_o = synth.coverage_reads(g, 50, 150, seed + 1, sub_rate=0.01, n_rate=0.001)
+    new_b, new_o = synth.coverage_reads(g, 10, 150, seed + 2, sub_rate=0.01, n_rate=0.001)
+    q_b = np.concatenate([ins_b, new_b])
+    q_o = np.concatenate([ins_o, new_o[1:] + ins_o[-1]]).astype(np.uint64)
+    return ins_b, ins_o, q_b, q_o
+
+
+def cfg_c2(scb):
+    return scb.SuperCountingBloomConfig(k=31, m=21, s=25, n_hashes=4, counter_bits=16,
+                                        counter_slots_exponent=30, block_slots_exponent=9,
+                                        threads=max(1, os.cpu_count() or 1), queue=4096)
+
+
+def stage_times(scb):
+    ms = (C.c_double * 4)()
+    scb.api.lib().scb_last_stage_times(ms)
+    return list(ms)
+
+
+def run_step(scb, cfg, method, dev, out_vals, out_bits):
+    """One insert + query pass; returns (ins_report, q_report, ins_stage, q_stage, index)."""
+    b = scb.SuperCountingBloomBuilder(cfg)
+    b.set_method(method)
+    ir = b.add_sequences_device(dev["ib"], dev["io"], dev["in"], dev["iN"])
+    ist = stage_times(scb)
+    idx = b.into_index()
+    idx.set_method(method)
+    qr = idx.estimate_sequences_device(dev["qb"], dev["qo"], dev["qn"], dev["qN"], out_vals, out_bits)
+    qst = stage_times(scb)
+    return ir, qr, ist, qst, idx
+
+
+def lines_touched(counters, line_bytes=64):
+    per = line_bytes // counters.itemsize
+    return int(np.count_nonzero(counters.reshape(-1, per).any(axis=1)))
+
+
+def cpu_baseline(ins_b, ins_o, q_b, q_o, target_s=15.0):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py
+    threads = max(1, min(16, os.cpu_count() or 1))
+    cfg = oracle_py.make_cfg(31, 21, 25, 4, 16, 30, 9, threads=threads, queue=4096)
+
+    def sample(n_ins, n_q):
+        ib, io = ins_b[:int(ins_o[n_ins])], ins_o[:n_ins + 1]
+        # query sample: the same inserted reads + an equal share of fresh reads
+        fresh0 = ins_o.size - 1
+        qi = q_o[fresh0:fresh0 + n_q + 1]
+        qb = np.concatenate([ib, q_b[int(qi[0]):int(qi[-1])]])
+        qo = np.concatenate([io, qi[1:] - qi[0] + io[-1]]).astype(np.uint64)
+        return ib, io, qb, qo
+
+    n = 20_000
+    while True:
+        ib, io, qb, qo = sample(n, n // 5)
+        r = oracle_py.cpu_baseline(cfg, ib, io, qb, qo)
+        t = r.build_s + r.query_s
+        if t >= target_s / 4 or n >= ins_o.size - 1:
+            break
+        n = min(ins_o.size - 1, int(n * max(2.0, target_s / max(t, 1e-3) * 0.8)))
+    kps = (r.indexed_kmers + r.queried_kmers) / t
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": kps / 1e9, "unit": "G k-mers/s", "cores": threads, "kind": "port",
+            "sample": f"{n} inserted + {n + n // 5} queried 150 bp reads of the same workload "
+                      f"({r.indexed_kmers} + {r.queried_kmers} k-mers; build {r.build_s:.2f} s, "
+                      f"query {r.query_s:.2f} s), default geometry, 2^30 16-bit counters",
+            "host_cpus": os.cpu_count(), "cpu_model": model}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import scbf_amd as scb
+
+    t0 = time.time()
+    ins_b, ins_o, q_b, q_o = gen_workload()
+    log(f"workload: {ins_o.size - 1} insert reads, {q_o.size - 1} query reads "
+        f"({time.time() - t0:.1f} s to generate)")
+    dev_t = {
+        "ib": torch.from_numpy(ins_b).cuda(), "io": torch.from_numpy(ins_o.view(np.int64)).cuda(),
+        "qb": torch.from_numpy(q_b).cuda(), "qo": torch.from_numpy(q_o.view(np.int64)).cuda(),
+    }
+    dev = {"ib": dev_t["ib"].data_ptr(), "io": dev_t["io"].data_ptr(), "in": ins_o.size - 1,
+           "iN": int(ins_o[-1]), "qb": dev_t["qb"].data_ptr(), "qo": dev_t["qo"].data_ptr(),
+           "qn": q_o.size - 1, "qN": int(q_o[-1])}
+    qN = dev["qN"]
+    out_vals = torch.zeros(qN, dtype=torch.int16, device="cuda")
+    out_bits = torch.zeros((qN + 31) // 32, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    cfg = cfg_c2(scb)
+
+    for _ in range(args.warmup):
+        run_step(scb, cfg, scb.METHOD_BATCHED, dev, out_vals.data_ptr(), out_bits.data_ptr())
+    torch.cuda.synchronize()
+    ins_ms, q_ms, ist_all, qst_all = [], [], [], []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        ir, qr, ist, qst, idx = run_step(scb, cfg, scb.METHOD_BATCHED, dev, out_vals.data_ptr(),
+                                         out_bits.data_ptr())
+        ist_all.append(ist)
+        qst_all.append(qst)
+        del idx
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t_start
+    ms_per_step = t_total * 1e3 / args.steps
+    kmers = ir.inserted_kmers + qr.valid_kmers
+    value = kmers / (ms_per_step / 1e3) / 1e9
+    ist_m = np.median(np.array(ist_all), axis=0).tolist()
+    qst_m = np.median(np.array(qst_all), axis=0).tolist()
+    log(f"batched: {ms_per_step:.2f} ms/step, {value:.2f} G k-mers/s; insert stages {ist_m}, "
+        f"query stages {qst_m}")
+
+    # keep one batched result for the checks
+    ir_b, qr_b, _, _, idx_b = run_step(scb, cfg, scb.METHOD_BATCHED, dev, out_vals.data_ptr(),
+                                       out_bits.data_ptr())
+    counters_b = idx_b.counters()
+    vals_b, bits_b = out_vals.clone(), out_bits.clone()
+    del idx_b
+
+    plain = None
+    bit_exact = None
+    if not args.no_plain:
+        out_vals.zero_()
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        ir_p, qr_p, ist_p, qst_p, idx_p = run_step(scb, cfg, scb.METHOD_PLAIN, dev, out_vals.data_ptr(),
+                                                   out_bits.data_ptr())
+        torch.cuda.synchronize()
+        plain_ms = (time.perf_counter() - tp) * 1e3
+        plain = {"value": kmers / (plain_ms / 1e3) / 1e9, "ms_per_step": plain_ms,
+                 "insert_ms": ist_p[3], "query_ms": qst_p[3]}
+        log(f"plain: {plain_ms:.2f} ms/step, {plain['value']:.2f} G k-mers/s")
+        if not args.no_check:
+            counters_p = idx_p.counters()
+            bit_exact = bool(np.array_equal(counters_b, counters_p)
+                             and torch.equal(vals_b, out_vals) and torch.equal(bits_b, out_bits)
+                             and (ir_b.inserted_kmers, ir_b.inserted_smers, qr_b.valid_kmers,
+                                  qr_b.positive_estimates, qr_b.estimate_checksum)
+                             == (ir_p.inserted_kmers, ir_p.inserted_smers, qr_p.valid_kmers,
+                                 qr_p.positive_estimates, qr_p.estimate_checksum))
+            log(f"batched vs plain bit-exact: {bit_exact}")
+        del idx_p
+
+    # roofline: algorithmic bytes (SURVEY 8(d) read-modify-write model)
+    W = 2
+    L_touch = lines_touched(counters_b)
+    n_sk_i = (ir.inserted_smers - ir.inserted_kmers) // (31 - 25)
+    iN, qNb = dev["iN"], dev["qN"]
+    b_insert_pipe = iN + 128 * L_touch
+    b_query_pipe = qNb + 64 * L_touch + qr.valid_kmers * W + qr.valid_kmers / 8
+    pipe_ms = ist_m[3] + qst_m[3]
+    stages = {"insert_scan": (ist_m[0], iN * 1.25 + 8 * n_sk_i),
+              "insert_bin": (ist_m[2], 128 * L_touch + 8 * n_sk_i + iN / 4),
+              "query_scan": (qst_m[0], qNb * 1.25 + qNb / 8 + 8 * (qr.valid_kmers / 7.2)),
+              "query_bin": (qst_m[2], 64 * L_touch + qr.valid_kmers * W + 8 * (qr.valid_kmers / 7.2) + qNb / 4)}
+    dom = max(stages, key=lambda s: stages[s][0])
+    dom_ms, dom_bytes = stages[dom]
+    ach = dom_bytes / (dom_ms / 1e3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+            "kernel_ms": round(dom_ms, 3), "kernel_bytes": int(dom_bytes),
+            "pipeline_bytes": int(b_insert_pipe + b_query_pipe),
+            "pipeline_device_ms": round(pipe_ms, 3),
+            "pipeline_achieved_GBs": round((b_insert_pipe + b_query_pipe) / (pipe_ms / 1e3) / 1e9, 1),
+            "pipeline_frac": round((b_insert_pipe + b_query_pipe) / (pipe_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+            "bytes_per_kmer": round((b_insert_pipe + b_query_pipe) / kmers, 2),
+            "lines_touched": L_touch}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(ins_b, ins_o, q_b, q_o)
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "G k-mers/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u16",
+        "data": "synthetic (seeded uniform genome, simulated reads)",
+        "config": {"workload": "default geometry k31 m21 s25 h4, 16-bit counters, 2^30 slots, "
+                               "2^9 slots/block; 5 Mbp genome; insert 50x 150 bp reads (1% subst, "
+                               "0.1% N), query those + fresh 10x", "parallelism": "single GPU",
+                   "insert_reads": dev["in"], "query_reads": dev["qn"], "inserted_kmers": ir.inserted_kmers,
+                   "queried_kmers": qr.valid_kmers},
+        "stage_ms": {"insert": {"scan": ist_m[0], "plan_scatter": ist_m[1], "bin": ist_m[2], "total": ist_m[3]},
+                     "query": {"scan": qst_m[0], "plan_scatter": qst_m[1], "bin": qst_m[2], "total": qst_m[3]}},
+        "plain": plain,
+        "batched_over_plain": round(value / plain["value"], 2) if plain else None,
+        "bit_exact_batched_vs_plain": bit_exact,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "host": platform.node(),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

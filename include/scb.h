@@ -1,0 +1,254 @@
+/*
+ * scb.h: C ABI of the MI355X SuperCountingBloom engine (libscb.so).
+ *
+ * Drop-in boundary for the hot path of the Rust crate super-counting-bloom
+ * 0.1.0 (insert + query + spectrum). Each entry point names the reference
+ * interface it replaces (file:line in the reference crate). A Rust crate would
+ * bind these symbols through `extern "C"`; the C++ wrapper
+ * in scb.hpp mirrors the crate's types and method names on top of them.
+ *
+ * Conventions
+ *  - Every fallible call returns int: SCB_OK (0), or 1..7 mapping one-to-one to
+ *    the variants of SuperCountingBloomError (error.rs:3-11), or SCB_ERR_DEVICE
+ *    (8, extension) for a HIP runtime failure. scb_last_error() returns the
+ *    message of the last failure on the calling thread, formatted like the
+ *    reference's Display impl (error.rs:13-27).
+ *  - Handles are owned by the caller and freed with *_free. Output buffers are
+ *    caller-allocated.
+ *  - Builder calls must be serialized by the caller (the Rust API takes
+ *    &mut self, api.rs:123,140). Index calls are safe to call concurrently
+ *    (the Rust API takes &self, api.rs:250,273): each call uses its own
+ *    device workspace.
+ *  - Calls block until their results are in the caller's memory. The *_device
+ *    variants take device pointers (HBM-resident inputs and outputs).
+ *  - Batched inputs are concatenated bytes plus n+1 offsets (offsets[0] = 0,
+ *    offsets[n] = total bytes). Per-window outputs of the batched calls are
+ *    indexed by position: window j of record i is at offsets[i] + j.
+ */
+#ifndef SCB_H
+#define SCB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error.rs:3-11 (+ SCB_ERR_DEVICE, an extension for HIP runtime failures) */
+enum {
+    SCB_OK = 0,
+    SCB_ERR_INVALID_CONFIG = 1,
+    SCB_ERR_FASTX_OPEN = 2,
+    SCB_ERR_INVALID_INDEX_FORMAT = 3,
+    SCB_ERR_CHANNEL_CLOSED = 4,
+    SCB_ERR_WORKER_PANIC = 5,
+    SCB_ERR_INTERNAL_STATE = 6,
+    SCB_ERR_IO = 7,
+    SCB_ERR_DEVICE = 8
+};
+
+/* Insert/query execution method (extension). BATCHED: super-k-mers binned by
+ * counter block and applied block-by-block in LDS. PLAIN: one fused launch per
+ * batch, one thread per super-k-mer, global atomics / loads. Both are
+ * bit-identical. */
+enum { SCB_METHOD_BATCHED = 0, SCB_METHOD_PLAIN = 1 };
+
+/* SuperCountingBloomConfig (config.rs:20-31); counter_bits is 8, 16 or 32. */
+typedef struct scb_config {
+    uint64_t k, m, s, n_hashes;
+    uint8_t counter_bits;
+    uint8_t counter_slots_exponent;
+    uint8_t block_slots_exponent;
+    uint8_t _pad[5];
+    uint64_t threads, queue;
+} scb_config;
+
+/* AddReport (api.rs:66-73) */
+typedef struct scb_add_report {
+    uint64_t records_processed, records_indexed, inserted_kmers, inserted_smers;
+    double add_seconds;
+} scb_add_report;
+
+/* QueryFastaReport (api.rs:75-82) */
+typedef struct scb_query_report {
+    uint64_t records_processed, total_windows, valid_kmers, positive_estimates, estimate_checksum;
+} scb_query_report;
+
+/* BuildIndexReport without the index (api.rs:56-64) */
+typedef struct scb_build_report {
+    uint64_t chunks, indexed_kmers, inserted_smers, counter_slots, blocks;
+    double build_seconds;
+} scb_build_report;
+
+/* SuperCountingBloomStats (fastx.rs:21-34) */
+typedef struct scb_spectrum_stats {
+    uint64_t chunks, indexed_kmers, inserted_smers, queried_kmers, distinct_keys, counter_slots,
+        blocks;
+    double build_seconds, query_seconds, spectrum_seconds, total_seconds;
+} scb_spectrum_stats;
+
+/* StreamEstimateStats (fastx.rs:42-55) */
+typedef struct scb_stream_stats {
+    uint64_t chunks, indexed_kmers, inserted_smers, queried_kmers, positive_estimates,
+        estimate_checksum, counter_slots, blocks;
+    double build_seconds, query_seconds, total_seconds;
+} scb_stream_stats;
+
+/* SequenceAbundanceSummary (api.rs:84-92); mean/median are NaN for None */
+typedef struct scb_summary {
+    uint64_t sequence_len, total_windows, valid_kmers;
+    double mean_abundance, median_abundance;
+} scb_summary;
+
+typedef struct scb_builder scb_builder; /* SuperCountingBloomBuilder (api.rs:43-48) */
+typedef struct scb_index scb_index;     /* SuperCountingBloomIndex (api.rs:50-54) */
+
+/* ---- errors and config --------------------------------------------------- */
+/* Copies the calling thread's last error message (NUL-terminated, truncated to
+ * cap); returns its full length. */
+size_t scb_last_error(char *buf, size_t cap);
+/* SuperCountingBloomConfig::default (config.rs:33-47); threads = online CPUs */
+void scb_config_default(scb_config *cfg);
+/* SuperCountingBloomConfig::validate (config.rs:50-92) */
+int scb_config_validate(const scb_config *cfg);
+/* number of HIP devices visible (0 without a GPU; never fails) */
+int scb_device_count(void);
+
+/* extension: device milliseconds of the calling thread's last insert or query
+ * call, measured with HIP events on the engine's stream: ms[0] scan kernel,
+ * ms[1] bin planning + scatter, ms[2] per-bin kernel, ms[3] total (the plain
+ * method reports its single fused kernel in ms[0] and ms[3]). */
+void scb_last_stage_times(double *ms);
+
+/* ---- builder ------------------------------------------------------------- */
+/* SuperCountingBloomBuilder::new (api.rs:102-121): allocates and zeroes the
+ * 2^counter_slots_exponent-counter table in HBM. */
+int scb_builder_new(const scb_config *cfg, scb_builder **out);
+void scb_builder_free(scb_builder *b);
+/* SuperCountingBloomBuilder::add_sequence (api.rs:123-138) */
+int scb_builder_add_sequence(scb_builder *b, const uint8_t *seq, size_t len,
+                             uint64_t *inserted_kmers);
+/* Batched add_sequence over n host records (api.rs:123-138 per record;
+ * AddReport fields as add_fasta_typed sums them, api.rs:633-662) */
+int scb_builder_add_sequences(scb_builder *b, const uint8_t *bases, const uint64_t *offsets,
+                              size_t n, scb_add_report *report);
+/* Same with HBM-resident inputs (d_offsets: n+1 device u64; total bases must
+ * be < 2^32). records_* fields of the report are not computed (left 0). */
+int scb_builder_add_sequences_device(scb_builder *b, const uint8_t *d_bases,
+                                     const uint64_t *d_offsets, size_t n, uint64_t total_bases,
+                                     scb_add_report *report);
+/* SuperCountingBloomBuilder::add_fasta (api.rs:140-158) */
+int scb_builder_add_fasta(scb_builder *b, const char *path, scb_add_report *report);
+/* SuperCountingBloomBuilder::into_index (api.rs:160-177): consumes the builder
+ * (it is freed whatever the result). */
+int scb_builder_into_index(scb_builder *b, scb_index **out);
+/* api.rs:179-207 */
+uint64_t scb_builder_inserted_kmers(const scb_builder *b);
+uint64_t scb_builder_inserted_smers(const scb_builder *b);
+int scb_builder_set_threads(scb_builder *b, size_t threads);
+void scb_builder_clear_threads(scb_builder *b);
+size_t scb_builder_threads(const scb_builder *b);
+void scb_builder_config(const scb_builder *b, scb_config *out);
+/* extension: choose SCB_METHOD_BATCHED (default) or SCB_METHOD_PLAIN */
+int scb_builder_set_method(scb_builder *b, int method);
+/* extension: copy the counter table (physical = .scb payload order) to host */
+int scb_builder_counters(const scb_builder *b, void *out, size_t bytes);
+
+/* ---- index --------------------------------------------------------------- */
+/* SuperCountingBloomIndex::build_from_fastx (api.rs:211-223; fastx.rs:206-253,
+ * 378-390); report may be NULL */
+int scb_index_build_from_fastx(const char *path, const scb_config *cfg, scb_index **out,
+                               scb_build_report *report);
+void scb_index_free(scb_index *idx);
+void scb_index_config(const scb_index *idx, scb_config *out);   /* api.rs:225-227 */
+uint64_t scb_index_inserted_kmers(const scb_index *idx);         /* api.rs:229-231 */
+int scb_index_set_method(scb_index *idx, int method);            /* extension */
+int scb_index_counters(const scb_index *idx, void *out, size_t bytes); /* extension */
+/* SuperCountingBloomIndex::save / load (api.rs:233-248, 388-608): byte-exact
+ * .scb files (139-byte header, then shard-major little-endian counters) */
+int scb_index_save(const scb_index *idx, const char *path);
+int scb_index_load(const char *path, scb_index **out);
+/* SuperCountingBloomIndex::estimate_sequence_abundances (api.rs:250-262,
+ * 799-853): writes len-k+1 entries (none if len<k) to values/valid (valid=0
+ * is None); *n_windows receives the count. */
+int scb_index_estimate_sequence_abundances(const scb_index *idx, const uint8_t *seq, size_t len,
+                                           uint64_t *values, uint8_t *valid, size_t *n_windows);
+/* Batched estimate_sequence_abundances over n host records: values/valid are
+ * indexed by position (offsets[i] + j, j < len_i-k+1) and hold
+ * offsets[n] entries; report (may be NULL) is the QueryFastaReport of the batch
+ * (api.rs:283-296). */
+int scb_index_estimate_sequences(const scb_index *idx, const uint8_t *bases,
+                                 const uint64_t *offsets, size_t n, uint64_t *values,
+                                 uint8_t *valid, scb_query_report *report);
+/* Same with HBM-resident inputs and outputs: d_values holds total_bases
+ * counters of counter_bits/8 bytes each, d_valid_bits total_bases bits
+ * (rounded up to 32-bit words); either may be NULL to skip it (then only the
+ * report is produced: the stream path of fastx.rs:310-369). report's
+ * records_processed/total_windows are not computed (left 0). */
+int scb_index_estimate_sequences_device(const scb_index *idx, const uint8_t *d_bases,
+                                        const uint64_t *d_offsets, size_t n,
+                                        uint64_t total_bases, void *d_values,
+                                        uint32_t *d_valid_bits, scb_query_report *report);
+/* SuperCountingBloomIndex::summarize_sequence (api.rs:264-271, 855-891) */
+int scb_index_summarize_sequence(const scb_index *idx, const uint8_t *seq, size_t len,
+                                 scb_summary *out);
+/* SuperCountingBloomIndex::query_fasta (api.rs:273-327) */
+int scb_index_query_fasta(const scb_index *idx, const char *path, scb_query_report *report);
+/* SuperCountingBloomIndex::counter_histogram (api.rs:329-335; filter.rs:267-275):
+ * rows sorted by value; *n_rows receives the row count (rows beyond cap are not
+ * written). */
+int scb_index_counter_histogram(const scb_index *idx, uint64_t *values, uint64_t *counts,
+                                size_t cap, size_t *n_rows);
+/* Distinct-k-mer spectrum of a batch of query records against the index
+ * (fastx.rs:255-308, 392-410 with the DnaChunk split of fastx.rs:14-19):
+ * rows (estimate, #distinct canonical k-mers) sorted by estimate. *queried
+ * receives the number of queried k-mers, *distinct the number of keys. */
+int scb_index_spectrum(const scb_index *idx, const uint8_t *bases, const uint64_t *offsets,
+                       size_t n, uint64_t *values, uint64_t *counts, size_t cap, size_t *n_rows,
+                       uint64_t *queried, uint64_t *distinct);
+int scb_index_spectrum_device(const scb_index *idx, const uint8_t *d_bases,
+                              const uint64_t *d_offsets, size_t n, uint64_t total_bases,
+                              uint64_t *values, uint64_t *counts, size_t cap, size_t *n_rows,
+                              uint64_t *queried, uint64_t *distinct);
+/* estimate_filter_only_spectrum(_from_counter_histogram) (api.rs:337-368,
+ * 933-1033); writes rows 1..=max_abundance to counts (counts[a-1]). When
+ * hist_values is NULL the index's counter_histogram is used. */
+int scb_index_filter_only_spectrum(const scb_index *idx, const uint64_t *hist_values,
+                                   const uint64_t *hist_counts, size_t hist_rows,
+                                   uint64_t max_abundance, uint64_t kmer_count_hint,
+                                   uint64_t *counts);
+/* filter_only_count_hint(_from_counter_histogram) (api.rs:370-385); max_abundance
+ * is 255 for filter_only_count_hint(). */
+int scb_index_filter_only_count_hint(const scb_index *idx, const uint64_t *hist_values,
+                                     const uint64_t *hist_counts, size_t hist_rows,
+                                     uint64_t max_abundance, uint64_t *hint);
+
+/* ---- FASTX pipelines (fastx.rs:81-204) ------------------------------------ */
+/* estimate_spectrum_from_fastx_pair (fastx.rs:88-105, 133-170); query_path may
+ * equal index_path (estimate_spectrum_from_fastx, fastx.rs:81-86) */
+int scb_estimate_spectrum_from_fastx_pair(const char *index_path, const char *query_path,
+                                          const scb_config *cfg, uint64_t *values,
+                                          uint64_t *counts, size_t cap, size_t *n_rows,
+                                          scb_spectrum_stats *stats);
+/* stream_estimates_from_fastx_pair (fastx.rs:114-131, 172-204) */
+int scb_stream_estimates_from_fastx_pair(const char *index_path, const char *query_path,
+                                         const scb_config *cfg, scb_stream_stats *stats);
+/* write_spectrum (spectrum.rs:11-31): TSV with header "estimated_count\tkmers";
+ * path NULL writes to stdout */
+int scb_write_spectrum(const uint64_t *values, const uint64_t *counts, size_t n_rows,
+                       const char *path);
+/* for_each_fastx_record (api.rs:893-927): calls visit(user, name, seq, len) per
+ * record (keep_non_actg); a nonzero return stops the walk and is returned. */
+typedef int (*scb_record_visitor)(void *user, const char *name, const uint8_t *seq, size_t len);
+int scb_for_each_fastx_record(const char *path, scb_record_visitor visit, void *user);
+
+/* ---- low-level primitives (filter.rs:622-635) ----------------------------- */
+uint64_t scb_xorshift_u64(uint64_t x);
+/* canonical_kmer over an ASCII ACGT sequence (2-bit packed as packed-seq) */
+uint64_t scb_canonical_kmer(const uint8_t *seq, size_t len, size_t klen, size_t pos);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

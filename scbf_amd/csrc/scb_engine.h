@@ -1,0 +1,133 @@
+// Internal interface between the C ABI (scb_api.cpp) and the HIP engine
+// (scb_kernels.hip). Not installed; no torch types anywhere.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace scb {
+
+// Filter geometry derived from SuperCountingBloomConfig and FrozenFilterLayout
+// (config.rs:20-107, filter.rs:90-116).
+struct Geometry {
+    uint32_t k = 0, m = 0, s = 0, nh = 0;  // k-mer, minimizer, s-mer lengths, n_hashes
+    uint32_t w = 0;                         // minimizer window in m-mers: k - m + 1
+    uint32_t win = 0;                       // fimpera window in s-mers: k - s + 1
+    uint32_t bits = 16, bytes = 2;          // counter width
+    uint32_t cse = 0, bse = 0;              // log2 counter slots, log2 block slots
+    uint64_t slots = 0, nb = 0, block_mask = 0;
+    uint64_t shard_count = 0, subblocks = 0;
+    uint32_t shard_mask = 0, shard_shift = 0, sub_log = 0;
+    uint64_t table_bytes() const { return slots * bytes; }
+};
+
+// Status of an engine call: code 0 = ok, otherwise an SCB_ERR_* value.
+struct Status {
+    int code = 0;
+    std::string msg;
+    static Status ok() { return {}; }
+    bool good() const { return code == 0; }
+};
+
+// A batch of records already in device memory: d_bases holds n_bases bytes,
+// d_offsets n_reads + 1 u64 (d_offsets[0] = 0, d_offsets[n_reads] = n_bases).
+struct DeviceBatch {
+    const uint8_t *d_bases = nullptr;
+    const uint64_t *d_offsets = nullptr;
+    uint64_t n_reads = 0, n_bases = 0;
+};
+
+struct InsertResult {
+    uint64_t inserted_kmers = 0;   // valid windows (api.rs:723)
+    uint64_t inserted_smers = 0;   // filter.rs:232
+    uint64_t super_kmers = 0;
+};
+
+struct QueryResult {
+    uint64_t queried = 0, positive = 0, checksum = 0;  // fastx.rs:325-334 / api.rs:284-296
+    uint64_t super_kmers = 0;
+};
+
+struct SpectrumResult {
+    uint64_t queried = 0, distinct = 0;
+    std::vector<std::pair<uint64_t, uint64_t>> rows;  // (estimate, #distinct keys), sorted
+};
+
+enum Method { kBatched = 0, kPlain = 1 };
+
+// Tunables of the batched pipeline (chosen per geometry by the autotuner or the
+// defaults below).
+struct Tuning {
+    uint32_t insert_bin_log = 15;   // slots per insert bin (u32 LDS counts)
+    uint32_t query_bin_log = 15;    // slots per query bin (native-width LDS copy)
+    uint32_t max_records_per_item = 4096;  // hot-bin split size
+    uint32_t insert_threads = 1024, query_threads = 512;
+};
+
+class Workspace;  // device scratch buffers + stream, one per concurrent call
+
+// Counter table resident in HBM in physical (.scb payload) block order
+// P(b) = (b & (shard_count-1)) * subblocks + (b >> log2(shard_count)).
+class Engine {
+   public:
+    static Status create(const Geometry &g, std::unique_ptr<Engine> &out);
+    ~Engine();
+
+    const Geometry &geometry() const { return geo_; }
+    Tuning &tuning() { return tune_; }
+
+    Status insert(const DeviceBatch &b, Method method, InsertResult &r);
+    // d_values: n_bases counters of geo.bytes each (or null); d_valid_bits:
+    // n_bases bits rounded up to u32 words (or null).
+    Status query(const DeviceBatch &b, Method method, void *d_values, uint32_t *d_valid_bits,
+                 QueryResult &r) const;
+    Status spectrum(const DeviceBatch &b, SpectrumResult &r) const;
+    Status counter_histogram(std::vector<std::pair<uint64_t, uint64_t>> &rows) const;
+
+    // Host <-> HBM copies of the whole table (physical order), and of batches.
+    Status download(void *host, uint64_t bytes) const;
+    Status upload(const void *host, uint64_t bytes);
+
+    // Host-memory convenience: stages records into device memory and runs the
+    // device path in chunks of at most max_chunk_bases.
+    Status insert_host(const uint8_t *bases, const uint64_t *offsets, uint64_t n, Method method,
+                       InsertResult &r);
+    // values/valid indexed by position; either may be null.
+    Status query_host(const uint8_t *bases, const uint64_t *offsets, uint64_t n, Method method,
+                      uint64_t *values, uint8_t *valid, QueryResult &r) const;
+    Status spectrum_host(const uint8_t *bases, const uint64_t *offsets, uint64_t n,
+                         SpectrumResult &r) const;
+
+   private:
+    Engine() = default;
+    Geometry geo_;
+    Tuning tune_;
+    void *d_table_ = nullptr;
+    int device_ = 0;
+    mutable std::mutex pool_mu_;
+    mutable std::vector<std::unique_ptr<Workspace>> pool_;
+    std::unique_ptr<Workspace> acquire() const;
+    void release(std::unique_ptr<Workspace> ws) const;
+};
+
+// Device time of the stages of the calling thread's last insert/query call,
+// measured with HIP events on the engine's stream.
+struct StageTimes {
+    float scan_ms = 0, plan_scatter_ms = 0, bin_ms = 0, total_ms = 0;
+    bool batched = false;
+};
+StageTimes last_stage_times();
+
+Status make_geometry(uint64_t k, uint64_t m, uint64_t s, uint64_t nh, uint32_t bits, uint32_t cse,
+                     uint32_t bse, Geometry &g);
+int device_count();
+
+// Maximum bases per device batch (positions are 32-bit inside the kernels).
+constexpr uint64_t kMaxBatchBases = (uint64_t(1) << 32) - 4096;
+
+}  // namespace scb

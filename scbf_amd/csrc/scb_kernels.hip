@@ -1,0 +1,1424 @@
+// gfx950 kernels of the SuperCountingBloom hot path and the Engine that drives
+// them. All arithmetic is integer and bit-exact with the reference CPU path
+// (super-counting-bloom 0.1.0, src/filter.rs); each kernel cites the loop it
+// replaces.
+//
+// Device data layout (HBM):
+//   table   : 2^cse counters of 1/2/4 bytes, physical block order
+//             P(b) = ((b & shard_mask) << sub_log) | (b >> shard_shift), which is
+//             the .scb payload order (filter.rs:575-580), so save/load are copies.
+//   pk      : 2-bit codes of the batch, 16 bases per u32, base i at bits 2*(i%16)
+//   rec     : one u64 per super-k-mer: window index (32 bits) | (n_kmers-1) (5) |
+//             physical block (27)
+//   sorted  : rec grouped by bin (a run of consecutive physical blocks whose
+//             counters fit in LDS)
+//
+// Batched pipeline (one stream, no host sync inside):
+//   k_scan      per 2048-window tile: ASCII -> codes, ACGT-run and read-boundary
+//               masks, canonical ntHash of every m-mer, minimizer of every k-mer
+//               window, super-k-mer cut, block hash -> rec + bin histogram
+//   k_bin_plan  one workgroup: bin offsets + work list (hot bins split)
+//   k_scatter   rec -> sorted (counting sort by bin)
+//   k_bin_insert / k_bin_query  persistent work queue, one bin per item, counters
+//               staged in LDS, written back once (insert) or read once (query)
+// Plain pipeline: k_scan alone, one thread per super-k-mer, global atomics/loads.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+
+#include "scb_engine.h"
+
+namespace scb {
+namespace {
+
+// ----------------------------------------------------------------------------
+// Constants and small device helpers
+// ----------------------------------------------------------------------------
+constexpr int kTile = 2048;            // windows owned by one scan tile
+constexpr int kTileThreads = 256;
+constexpr int kLeft = 32;              // LDS holds bases [t0 - 32, t0 + kTile + 96)
+constexpr int kSpan = kTile + 128;
+constexpr int kSpanWords = kSpan / 32 + 2;
+constexpr int kPkWords = kSpan / 16 + 8;
+constexpr int kHashN = kTile + 64;
+constexpr int kWinN = kTile + 32;
+constexpr uint16_t kNoWin = 0xFFFF;
+constexpr int kPkPadWords = 16;        // zero words after the packed batch
+
+enum StatSlot {
+    kStRecords = 0,
+    kStValid = 1,
+    kStSmers = 2,
+    kStQueried = 3,
+    kStPositive = 4,
+    kStChecksum = 5,
+    kStNWork = 6,
+    kStWorkHead = 7,
+    kStOverflow = 8,
+    kStDistinct = 9,
+    kStCount = 16
+};
+
+struct DevGeo {
+    uint32_t k, m, s, nh, w, win, bse, shard_mask, shard_shift, sub_log;
+    uint64_t block_mask;
+};
+
+DevGeo dev_geo(const Geometry &g) {
+    DevGeo d;
+    d.k = g.k, d.m = g.m, d.s = g.s, d.nh = g.nh, d.w = g.w, d.win = g.win, d.bse = g.bse;
+    d.shard_mask = g.shard_mask, d.shard_shift = g.shard_shift, d.sub_log = g.sub_log;
+    d.block_mask = g.block_mask;
+    return d;
+}
+
+// filter.rs:629-635 (SplitMix64 output function)
+__device__ __forceinline__ uint64_t xs64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// Reverse complement of a len-base 2-bit word (packed-seq read_revcomp_kmer).
+__device__ __forceinline__ uint64_t revcomp(uint64_t x, uint32_t len) {
+    uint32_t lo = uint32_t(x), hi = uint32_t(x >> 32);
+    uint64_t r = (uint64_t(__builtin_bitreverse32(lo)) << 32) | __builtin_bitreverse32(hi);
+    r = ((r >> 1) & 0x5555555555555555ull) | ((r & 0x5555555555555555ull) << 1);
+    r ^= 0xAAAAAAAAAAAAAAAAull;
+    return r >> (64 - 2 * len);
+}
+
+__device__ __forceinline__ uint64_t kmask(uint32_t len) {
+    return len >= 32 ? ~0ull : ((1ull << (2 * len)) - 1);
+}
+
+// Canonical k-mer of len bases starting at base index b of a packed array.
+__device__ __forceinline__ uint64_t canonical_at(const uint32_t *words, uint64_t b, uint32_t len) {
+    uint64_t wi = b >> 4;
+    uint32_t sh = 2 * uint32_t(b & 15);
+    uint32_t w0 = words[wi], w1 = words[wi + 1], w2 = words[wi + 2];
+    uint32_t lo = __builtin_amdgcn_alignbit(w1, w0, sh);
+    uint32_t hi = __builtin_amdgcn_alignbit(w2, w1, sh);
+    uint64_t fw = ((uint64_t(hi) << 32) | lo) & kmask(len);
+    uint64_t rc = revcomp(fw, len);
+    return fw < rc ? fw : rc;
+}
+
+__device__ __forceinline__ uint32_t rotl32(uint32_t x, uint32_t r) {
+    return __builtin_amdgcn_alignbit(x, x, (32 - (r & 31)) & 31);
+}
+__device__ __forceinline__ uint32_t rotr32(uint32_t x, uint32_t r) {
+    return __builtin_amdgcn_alignbit(x, x, r & 31);
+}
+
+// ntHash seeds (low 32 bits of the original ntHash seeds), by 2-bit code
+// A=0 C=1 T=2 G=3. Third-party arithmetic (seq-hash 0.1.2), restated.
+__device__ __forceinline__ uint32_t nt_seed(uint32_t c) {
+    uint32_t a = (c & 1) ? 0x62a02b4cu : 0x95c60474u;  // C : A
+    uint32_t b = (c & 1) ? 0x82572324u : 0x4be24456u;  // G : T
+    return (c & 2) ? b : a;
+}
+
+__device__ __forceinline__ bool is_acgt(uint32_t ch) {
+    uint32_t u = ch & 0xDF;
+    return u == 'A' || u == 'C' || u == 'G' || u == 'T';
+}
+
+__device__ __forceinline__ uint32_t get_bits(const uint32_t *bm, int li, int n) {
+    if (n <= 0) return 0;
+    uint32_t w0 = bm[li >> 5], w1 = bm[(li >> 5) + 1];
+    uint32_t v = __builtin_amdgcn_alignbit(w1, w0, uint32_t(li & 31));
+    return n >= 32 ? v : (v & ((1u << n) - 1));
+}
+
+template <int BITS>
+struct Ctr;
+template <>
+struct Ctr<8> {
+    using T = uint8_t;
+    static constexpr uint32_t kMax = 0xFFu;
+};
+template <>
+struct Ctr<16> {
+    using T = uint16_t;
+    static constexpr uint32_t kMax = 0xFFFFu;
+};
+template <>
+struct Ctr<32> {
+    using T = uint32_t;
+    static constexpr uint32_t kMax = 0xFFFFFFFFu;
+};
+
+// counter.rs:20-23 applied through a 32-bit CAS (plain path and hot-bin merge):
+// adds `add` saturating to the sub-word counter at element index idx.
+template <int BITS>
+__device__ __forceinline__ void sat_add_global(void *table, uint64_t idx, uint32_t add) {
+    if (BITS == 32) {
+        uint32_t *p = reinterpret_cast<uint32_t *>(table) + idx;
+        uint32_t old = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while (old != 0xFFFFFFFFu) {
+            uint64_t sum = uint64_t(old) + add;
+            uint32_t nv = sum > 0xFFFFFFFFull ? 0xFFFFFFFFu : uint32_t(sum);
+            uint32_t prev = atomicCAS(p, old, nv);
+            if (prev == old) break;
+            old = prev;
+        }
+    } else {
+        constexpr uint32_t per = 32 / BITS;
+        constexpr uint32_t mx = Ctr<BITS>::kMax;
+        uint32_t *p = reinterpret_cast<uint32_t *>(table) + idx / per;
+        uint32_t sh = uint32_t(idx % per) * BITS;
+        uint32_t old = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (;;) {
+            uint32_t cur = (old >> sh) & mx;
+            if (cur == mx) break;
+            uint32_t nc = cur + add > mx ? mx : cur + add;
+            uint32_t nv = (old & ~(mx << sh)) | (nc << sh);
+            uint32_t prev = atomicCAS(p, old, nv);
+            if (prev == old) break;
+            old = prev;
+        }
+    }
+}
+
+template <int BITS>
+__device__ __forceinline__ uint32_t load_ctr(const void *base, uint64_t idx) {
+    return reinterpret_cast<const typename Ctr<BITS>::T *>(base)[idx];
+}
+
+// Streams the bases of one super-k-mer (from 4 pre-aligned words holding its
+// first 64 bases) and calls f(j, canonical s-mer) for each of its s-mers in
+// position order (filter.rs:224-226, 423-426 with canonical_kmer 622-627).
+template <class F>
+__device__ __forceinline__ void for_each_smer(uint32_t A0, uint32_t A1, uint32_t A2, uint32_t A3,
+                                              uint32_t nbases, uint32_t s, F &&f) {
+    const uint64_t mask = kmask(s);
+    const uint32_t top = 2 * (s - 1);
+    uint64_t fw = 0, rc = 0;
+    uint32_t cur = A0;
+    for (uint32_t i = 0; i < nbases; ++i) {
+        uint32_t c = cur & 3;
+        cur >>= 2;
+        if ((i & 15) == 15) {
+            cur = A1;
+            A1 = A2;
+            A2 = A3;
+        }
+        fw = (fw >> 2) | (uint64_t(c) << top);
+        rc = ((rc << 2) | (c ^ 2)) & mask;
+        if (i + 1 >= s) f(i + 1 - s, fw < rc ? fw : rc);
+    }
+}
+
+// Loads 5 packed words starting at base b and aligns them so that A0 holds
+// bases b..b+15.
+__device__ __forceinline__ void load_aligned(const uint32_t *words, uint64_t b, uint32_t &A0,
+                                             uint32_t &A1, uint32_t &A2, uint32_t &A3) {
+    uint64_t wi = b >> 4;
+    uint32_t sh = 2 * uint32_t(b & 15);
+    uint32_t W0 = words[wi], W1 = words[wi + 1], W2 = words[wi + 2], W3 = words[wi + 3],
+             W4 = words[wi + 4];
+    A0 = __builtin_amdgcn_alignbit(W1, W0, sh);
+    A1 = __builtin_amdgcn_alignbit(W2, W1, sh);
+    A2 = __builtin_amdgcn_alignbit(W3, W2, sh);
+    A3 = __builtin_amdgcn_alignbit(W4, W3, sh);
+}
+
+// Sliding minimum over the last `win` s-mer estimates (filter.rs:485-507; min
+// is associative, so a shift register gives the deque's result).
+template <int WINC>
+struct WinMin {
+    uint32_t e[WINC];
+    __device__ __forceinline__ uint32_t push(uint32_t est, uint32_t win) {
+#pragma unroll
+        for (int i = 0; i < WINC - 1; ++i) e[i] = e[i + 1];
+        e[WINC - 1] = est;
+        uint32_t m = est;
+#pragma unroll
+        for (int i = 1; i < WINC; ++i)
+            if (uint32_t(i) < win) m = e[WINC - 1 - i] < m ? e[WINC - 1 - i] : m;
+        return m;
+    }
+};
+
+// ----------------------------------------------------------------------------
+// Fill kernel (used instead of hipMemsetAsync so that every step can be
+// captured in a graph)
+// ----------------------------------------------------------------------------
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += uint64_t(gridDim.x) * blockDim.x)
+        p[i] = v;
+}
+
+// ----------------------------------------------------------------------------
+// Scan tile (front end shared by both pipelines)
+// ----------------------------------------------------------------------------
+struct ScanArgs {
+    const uint8_t *bases;
+    const uint64_t *offsets;
+    uint64_t n_reads, n_bases;
+    uint32_t *pk;          // packed codes out (batched) or null
+    uint32_t *valid_bits;  // per-window validity out or null
+    uint64_t *rec;         // records out (batched)
+    uint32_t *bin_count;   // records per bin (batched)
+    uint32_t bin_shift;    // bin = pblk >> bin_shift
+    uint64_t *stats;
+    void *table;           // plain paths
+    void *out;             // plain query abundances (position-indexed) or null
+};
+
+enum ScanMode { kModeRecords = 0, kModePlainInsert = 1, kModePlainQuery = 2 };
+
+// One tile owns windows [t0, t0 + kTile). Window j is valid iff its k bases are
+// ACGT and belong to one record (the maximal-ACGT-run split of api.rs:676-726,
+// 799-853 and fastx.rs:14-19). Its minimizer is the canonical-ntHash argmin
+// over its w m-mers (simd-minimizers restatement, third-party), and a
+// super-k-mer starts where the minimizer position changes or a run starts
+// (filter.rs:608-620, 210-216).
+template <int MODE, int BITS, int WINC>
+__global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
+    __shared__ uint32_t s_bad[kSpanWords], s_tg[kSpanWords], s_brk[kSpanWords];
+    __shared__ uint32_t s_pk[kPkWords];
+    __shared__ __align__(16) uint8_t s_code[kSpan];
+    __shared__ uint32_t s_hash[kHashN];
+    __shared__ uint16_t s_mpos[kWinN];
+    __shared__ uint32_t s_cnt[kTileThreads];
+    __shared__ uint64_t s_red[3][kTileThreads / 64];
+    __shared__ uint64_t s_r0, s_base;
+
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int nwave = kTileThreads / 64;
+    const int64_t t0 = int64_t(blockIdx.x) * kTile;
+    const int64_t LB = t0 - kLeft;
+    const int64_t N = int64_t(a.n_bases);
+    const int k = int(g.k), m = int(g.m), w = int(g.w);
+
+    for (int i = tid; i < kSpanWords; i += kTileThreads) s_brk[i] = 0, s_bad[i] = ~0u, s_tg[i] = 0;
+    for (int i = tid; i < kPkWords; i += kTileThreads) s_pk[i] = 0;
+    if (tid == 0) {
+        // first record whose start lies after LB
+        uint64_t lo = 0, hi = a.n_reads;
+        while (lo < hi) {
+            uint64_t mid = (lo + hi) >> 1;
+            if (int64_t(a.offsets[mid]) > LB) hi = mid; else lo = mid + 1;
+        }
+        s_r0 = lo;
+    }
+    __syncthreads();
+
+    // Bases -> codes, non-ACGT and T/G masks (one 64-base chunk per wave step).
+    for (int c = wid; c < kSpan / 64; c += nwave) {
+        int li = c * 64 + lane;
+        int64_t gi = LB + li;
+        uint32_t ch = (gi >= 0 && gi < N) ? a.bases[gi] : uint32_t('N');
+        uint32_t code = (ch >> 1) & 3;
+        bool ok = is_acgt(ch);
+        uint64_t badm = __ballot(!ok), tgm = __ballot(ok && (code >> 1));
+        s_code[li] = uint8_t(code);
+        if (lane == 0) {
+            s_bad[2 * c] = uint32_t(badm), s_bad[2 * c + 1] = uint32_t(badm >> 32);
+            s_tg[2 * c] = uint32_t(tgm), s_tg[2 * c + 1] = uint32_t(tgm >> 32);
+        }
+    }
+    // Record starts inside the span mark read boundaries.
+    for (uint64_t r = s_r0 + tid; r < a.n_reads; r += kTileThreads) {
+        int64_t o = int64_t(a.offsets[r]);
+        if (o >= LB + kSpan) break;
+        int li = int(o - LB);
+        atomicOr(&s_brk[li >> 5], 1u << (li & 31));
+    }
+    __syncthreads();
+
+    // Packed words for the span (and the batch-wide packed array).
+    for (int wi = tid; wi < kSpan / 16; wi += kTileThreads) {
+        const uint32_t *cw = reinterpret_cast<const uint32_t *>(s_code + wi * 16);
+        uint32_t v = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            uint32_t x = cw[q];
+            uint32_t b8 = (x & 3) | ((x >> 6) & 0xC) | ((x >> 12) & 0x30) | ((x >> 18) & 0xC0);
+            v |= b8 << (8 * q);
+        }
+        s_pk[wi] = v;
+        if (MODE == kModeRecords) {
+            int64_t gw = (LB >> 4) + wi;  // LB is a multiple of 16
+            if (wi >= kLeft / 16 && wi < kLeft / 16 + kTile / 16) a.pk[gw] = v;
+        }
+    }
+
+    // Canonical ntHash of every m-mer at positions p = t0-1+hp, hp in [0, NH).
+    {
+        const int NH = kTile + 2 * w - 1;
+        const int per = (NH + kTileThreads - 1) / kTileThreads;
+        int h0 = tid * per, h1 = min(h0 + per, NH);
+        if (h0 < h1) {
+            uint32_t fw = 0, rc = 0;
+            int li0 = h0 + kLeft - 1;
+            for (int i = 0; i < m; ++i) {
+                uint32_t c = s_code[li0 + i];
+                fw ^= rotl32(nt_seed(c), uint32_t(m - 1 - i));
+                rc ^= rotl32(nt_seed(c ^ 2), uint32_t(i));
+            }
+            for (int hp = h0;; ++hp) {
+                s_hash[hp] = fw + rc;
+                if (hp + 1 >= h1) break;
+                int li = hp + kLeft - 1;
+                uint32_t co = s_code[li], ci = s_code[li + m];
+                fw = rotl32(fw, 1) ^ rotl32(nt_seed(co), uint32_t(m)) ^ nt_seed(ci);
+                rc = rotr32(rc, 1) ^ rotr32(nt_seed(co ^ 2), 1) ^ rotl32(nt_seed(ci ^ 2), uint32_t(m - 1));
+            }
+        }
+    }
+    __syncthreads();
+
+    // Minimizer of every window j = t0-1+hj, hj in [0, kTile + w).
+    for (int hj = tid; hj < kTile + w; hj += kTileThreads) {
+        int li = hj + kLeft - 1;
+        uint32_t bad = get_bits(s_bad, li, k) | get_bits(s_brk, li + 1, k - 1);
+        uint16_t pos = kNoWin;
+        if (!bad) {
+            uint32_t tg = __popc(get_bits(s_tg, li, k));
+            bool left = 2 * int(tg) > k;
+            uint32_t best = 0xFFFFFFFFu;
+            int bp = hj;
+            for (int i = 0; i < w; ++i) {
+                uint32_t key = s_hash[hj + i] >> 16;
+                bool take = left ? (key < best) : (key <= best);
+                if (take) best = key, bp = hj + i;
+            }
+            pos = uint16_t(bp);
+        }
+        s_mpos[hj] = pos;
+    }
+    __syncthreads();
+
+    // Owned windows hj in [1, kTile]: validity bitmap, super-k-mer starts.
+    constexpr int kPer = kTile / kTileThreads;
+    uint32_t starts = 0;  // bit u set if window 1 + u*256 + tid starts a super-k-mer
+    uint64_t nvalid = 0;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        int hj = 1 + u * kTileThreads + tid;
+        uint16_t mp = s_mpos[hj], pm = s_mpos[hj - 1];
+        bool valid = mp != kNoWin;
+        bool st = valid && (pm == kNoWin || pm != mp);
+        starts |= uint32_t(st) << u;
+        nvalid += valid;
+        if (a.valid_bits) {
+            uint64_t vm = __ballot(valid);
+            if (lane == 0) {
+                int64_t j0 = t0 + u * kTileThreads + wid * 64;
+                if (j0 < N) {
+                    a.valid_bits[j0 >> 5] = uint32_t(vm);
+                    if (j0 + 32 < N) a.valid_bits[(j0 >> 5) + 1] = uint32_t(vm >> 32);
+                }
+            }
+        }
+    }
+
+    uint32_t my = __popc(starts);
+    uint64_t smers = 0, queried = 0, positive = 0, checksum = 0;
+    if (MODE == kModeRecords) {
+        s_cnt[tid] = my;
+        __syncthreads();
+        // exclusive scan over 256 counts (Hillis-Steele)
+        for (int d = 1; d < kTileThreads; d <<= 1) {
+            uint32_t v = tid >= d ? s_cnt[tid - d] : 0;
+            __syncthreads();
+            s_cnt[tid] += v;
+            __syncthreads();
+        }
+        if (tid == kTileThreads - 1) s_base = atomicAdd((unsigned long long *)&a.stats[kStRecords],
+                                                        (unsigned long long)s_cnt[tid]);
+        __syncthreads();
+        uint64_t slot = s_base + s_cnt[tid] - my;
+        for (uint32_t bits = starts; bits; bits &= bits - 1) {
+            int u = __ffs(bits) - 1;
+            int hj = 1 + u * kTileThreads + tid;
+            uint16_t mp = s_mpos[hj];
+            int e = hj + 1;
+            while (e < hj + w && s_mpos[e] == mp) ++e;
+            uint32_t nk = uint32_t(e - hj);
+            uint64_t val = canonical_at(s_pk, uint64_t(mp) + kLeft - 1, uint32_t(m));
+            uint64_t block = xs64(val) & g.block_mask;
+            uint64_t pblk = ((block & g.shard_mask) << g.sub_log) | (block >> g.shard_shift);
+            uint64_t j = uint64_t(t0 - 1 + hj);
+            a.rec[slot++] = j | (uint64_t(nk - 1) << 32) | (pblk << 37);
+            atomicAdd(&a.bin_count[pblk >> a.bin_shift], 1u);
+            smers += nk + g.k - g.s;
+        }
+    } else {
+        // Plain path: this thread applies its own super-k-mers straight to HBM.
+        for (uint32_t bits = starts; bits; bits &= bits - 1) {
+            int u = __ffs(bits) - 1;
+            int hj = 1 + u * kTileThreads + tid;
+            uint16_t mp = s_mpos[hj];
+            int e = hj + 1;
+            while (e < hj + w && s_mpos[e] == mp) ++e;
+            uint32_t nk = uint32_t(e - hj);
+            uint64_t val = canonical_at(s_pk, uint64_t(mp) + kLeft - 1, uint32_t(m));
+            uint64_t block = xs64(val) & g.block_mask;
+            uint64_t pblk = ((block & g.shard_mask) << g.sub_log) | (block >> g.shard_shift);
+            uint64_t cbase = pblk << g.bse;
+            const uint64_t amask = (1ull << g.bse) - 1;
+            uint32_t A0, A1, A2, A3;
+            load_aligned(s_pk, uint64_t(hj + kLeft - 1), A0, A1, A2, A3);
+            uint32_t nbases = nk + g.k - 1;
+            if (MODE == kModePlainInsert) {
+                // filter.rs:224-233
+                for_each_smer(A0, A1, A2, A3, nbases, g.s, [&](uint32_t, uint64_t smer) {
+                    uint64_t h = xs64(smer);
+                    for (uint32_t t = 0; t < g.nh; ++t) {
+                        sat_add_global<BITS>(a.table, cbase + (h & amask), 1);
+                        if (t + 1 < g.nh) h = xs64(h);
+                    }
+                });
+                smers += nk + g.k - g.s;
+            } else {
+                // filter.rs:478-507 + 562-571
+                WinMin<WINC> wm;
+                uint64_t j0 = uint64_t(t0 - 1 + hj);
+                for_each_smer(A0, A1, A2, A3, nbases, g.s, [&](uint32_t j, uint64_t smer) {
+                    uint64_t h = xs64(smer);
+                    uint32_t est = Ctr<BITS>::kMax;
+                    for (uint32_t t = 0; t < g.nh; ++t) {
+                        uint32_t v = load_ctr<BITS>(a.table, cbase + (h & amask));
+                        est = v < est ? v : est;
+                        if (t + 1 < g.nh) h = xs64(h);
+                    }
+                    uint32_t ab = wm.push(est, g.win);
+                    if (j + 1 >= g.win) {
+                        uint64_t q = j + 1 - g.win;
+                        if (a.out) reinterpret_cast<typename Ctr<BITS>::T *>(a.out)[j0 + q] =
+                            typename Ctr<BITS>::T(ab);
+                        ++queried;
+                        positive += ab > 0;
+                        checksum += ab;
+                    }
+                });
+            }
+        }
+    }
+
+    // Block reductions of the statistics (one atomic per value per tile).
+    uint64_t vals[3] = {nvalid, MODE == kModePlainQuery ? positive : smers,
+                        MODE == kModePlainQuery ? checksum : 0};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        uint64_t v = vals[q];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) s_red[q][wid] = v;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint64_t t[3] = {0, 0, 0};
+        for (int q = 0; q < 3; ++q)
+            for (int i = 0; i < nwave; ++i) t[q] += s_red[q][i];
+        unsigned long long *st = (unsigned long long *)a.stats;
+        if (t[0]) atomicAdd(&st[MODE == kModePlainQuery ? kStQueried : kStValid], t[0]);
+        if (MODE == kModePlainQuery) {
+            if (t[1]) atomicAdd(&st[kStPositive], t[1]);
+            if (t[2]) atomicAdd(&st[kStChecksum], t[2]);
+        } else if (t[1]) {
+            atomicAdd(&st[kStSmers], t[1]);
+        }
+    }
+    (void)queried;
+}
+
+// ----------------------------------------------------------------------------
+// Bin planning: exclusive scan of per-bin record counts and the work list.
+// One workgroup of 1024 threads.
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_bin_plan(const uint32_t *bin_count, uint64_t nbins,
+                                                   uint32_t *bin_off, uint32_t *cursor,
+                                                   uint64_t *work, uint32_t per_item,
+                                                   uint64_t *stats) {
+    __shared__ uint64_t s_rec[1024], s_items[1024];
+    const int tid = threadIdx.x;
+    uint64_t chunk = (nbins + 1023) / 1024;
+    uint64_t b0 = tid * chunk, b1 = min(nbins, b0 + chunk);
+    uint64_t rsum = 0, isum = 0;
+    for (uint64_t b = b0; b < b1; ++b) {
+        uint32_t c = bin_count[b];
+        rsum += c;
+        isum += (c + per_item - 1) / per_item;
+    }
+    s_rec[tid] = rsum, s_items[tid] = isum;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        uint64_t vr = tid >= d ? s_rec[tid - d] : 0, vi = tid >= d ? s_items[tid - d] : 0;
+        __syncthreads();
+        s_rec[tid] += vr, s_items[tid] += vi;
+        __syncthreads();
+    }
+    uint64_t ro = s_rec[tid] - rsum, io = s_items[tid] - isum;
+    for (uint64_t b = b0; b < b1; ++b) {
+        uint32_t c = bin_count[b];
+        bin_off[b] = uint32_t(ro);
+        cursor[b] = uint32_t(ro);
+        uint32_t parts = (c + per_item - 1) / per_item;
+        for (uint32_t p = 0; p < parts; ++p) work[io++] = (b << 32) | p;
+        ro += c;
+    }
+    if (tid == 1023) {
+        stats[kStNWork] = s_items[1023];
+        stats[kStWorkHead] = 0;
+    }
+}
+
+__global__ void k_scatter(const uint64_t *rec, const uint64_t *stats, uint32_t *cursor,
+                          uint64_t *sorted, uint32_t bin_shift) {
+    uint64_t n = stats[kStRecords];
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += uint64_t(gridDim.x) * blockDim.x) {
+        uint64_t r = rec[i];
+        uint64_t bin = (r >> 37) >> bin_shift;
+        uint32_t slot = atomicAdd(&cursor[bin], 1u);
+        sorted[slot] = r;
+    }
+}
+
+struct BinArgs {
+    const uint64_t *sorted;
+    const uint32_t *bin_off, *bin_count;
+    const uint64_t *work;
+    uint64_t *stats;
+    const uint32_t *pk;
+    void *table;
+    void *out;
+    uint32_t bin_log;      // slots per bin = 2^bin_log
+    uint32_t per_item;
+};
+
+// filter.rs:199-237 for every super-k-mer of one bin: n_hashes saturating +1
+// per s-mer, counted in LDS (u32 per slot) and merged once as
+// min(MAX, old + count), which equals applying the +1s one by one (saturating
+// increments commute).
+template <int BITS>
+__global__ __launch_bounds__(1024) void k_bin_insert(BinArgs a, DevGeo g) {
+    extern __shared__ uint32_t s_cnt[];
+    __shared__ uint64_t s_item;
+    const uint32_t tid = threadIdx.x, nthr = blockDim.x;
+    const uint32_t bin_slots = 1u << a.bin_log;
+    const uint32_t lshift = a.bin_log - g.bse;
+    const uint32_t amask = (1u << g.bse) - 1;
+    const uint64_t nwork = a.stats[kStNWork];
+    unsigned long long *head = (unsigned long long *)&a.stats[kStWorkHead];
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(head, 1ull);
+        __syncthreads();
+        uint64_t item = s_item;
+        if (item >= nwork) break;
+        uint64_t wk = a.work[item];
+        uint64_t bin = wk >> 32;
+        uint32_t part = uint32_t(wk);
+        uint32_t cnt = a.bin_count[bin];
+        uint32_t rb = a.bin_off[bin] + part * a.per_item;
+        uint32_t re = min(a.bin_off[bin] + cnt, rb + a.per_item);
+        bool multi = cnt > a.per_item;
+        for (uint32_t i = tid; i < bin_slots; i += nthr) s_cnt[i] = 0;
+        __syncthreads();
+        for (uint32_t r = rb + tid; r < re; r += nthr) {
+            uint64_t rec = a.sorted[r];
+            uint64_t j = rec & 0xFFFFFFFFull;
+            uint32_t nk = uint32_t((rec >> 32) & 31) + 1;
+            uint32_t lbase = uint32_t((rec >> 37) & ((1u << lshift) - 1)) << g.bse;
+            uint32_t A0, A1, A2, A3;
+            load_aligned(a.pk, j, A0, A1, A2, A3);
+            for_each_smer(A0, A1, A2, A3, nk + g.k - 1, g.s, [&](uint32_t, uint64_t smer) {
+                uint64_t h = xs64(smer);
+                for (uint32_t t = 0; t < g.nh; ++t) {
+                    atomicAdd(&s_cnt[lbase + (uint32_t(h) & amask)], 1u);
+                    if (t + 1 < g.nh) h = xs64(h);
+                }
+            });
+        }
+        __syncthreads();
+        // merge: 32-bit table words of this bin
+        constexpr uint32_t per = 32 / BITS;
+        const uint32_t nwords = bin_slots / per;
+        uint32_t *tw = reinterpret_cast<uint32_t *>(a.table) + bin * uint64_t(nwords);
+        for (uint32_t wi = tid; wi < nwords; wi += nthr) {
+            uint32_t c[per];
+            uint32_t any = 0;
+#pragma unroll
+            for (uint32_t q = 0; q < per; ++q) c[q] = s_cnt[wi * per + q], any |= c[q];
+            if (!any) continue;
+            if (!multi) {
+                uint32_t old = tw[wi], nv;
+                if (BITS == 32) {
+                    uint64_t sum = uint64_t(old) + c[0];
+                    nv = sum > 0xFFFFFFFFull ? 0xFFFFFFFFu : uint32_t(sum);
+                } else {
+                    constexpr uint32_t mx = Ctr<BITS>::kMax;
+                    nv = 0;
+#pragma unroll
+                    for (uint32_t q = 0; q < per; ++q) {
+                        uint32_t cur = (old >> (q * BITS)) & mx;
+                        uint32_t sum = cur + (c[q] > mx ? mx : c[q]);
+                        nv |= (sum > mx ? mx : sum) << (q * BITS);
+                    }
+                }
+                tw[wi] = nv;
+            } else {
+#pragma unroll
+                for (uint32_t q = 0; q < per; ++q)
+                    if (c[q]) sat_add_global<BITS>(a.table, (bin * uint64_t(nwords) + wi) * per + q,
+                                                   c[q] > Ctr<BITS>::kMax && BITS < 32 ? Ctr<BITS>::kMax : c[q]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// filter.rs:462-508 (+ 562-571) for every super-k-mer of one bin: counters of
+// the bin staged in LDS at native width, min over hashes per s-mer, sliding
+// min over k-s+1 s-mers per k-mer, abundances written at their window index.
+template <int BITS, int WINC>
+__global__ __launch_bounds__(512) void k_bin_query(BinArgs a, DevGeo g) {
+    extern __shared__ uint32_t s_tab[];
+    __shared__ uint64_t s_item;
+    __shared__ uint64_t s_red[3][8];
+    using T = typename Ctr<BITS>::T;
+    const uint32_t tid = threadIdx.x, nthr = blockDim.x, lane = tid & 63, wid = tid >> 6;
+    const uint32_t bin_slots = 1u << a.bin_log;
+    const uint32_t lshift = a.bin_log - g.bse;
+    const uint32_t amask = (1u << g.bse) - 1;
+    const uint32_t nwords = bin_slots * (BITS / 8) / 4;
+    const uint64_t nwork = a.stats[kStNWork];
+    unsigned long long *head = (unsigned long long *)&a.stats[kStWorkHead];
+    const T *lt = reinterpret_cast<const T *>(s_tab);
+    uint64_t queried = 0, positive = 0, checksum = 0;
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(head, 1ull);
+        __syncthreads();
+        uint64_t item = s_item;
+        if (item >= nwork) break;
+        uint64_t wk = a.work[item];
+        uint64_t bin = wk >> 32;
+        uint32_t part = uint32_t(wk);
+        uint32_t cnt = a.bin_count[bin];
+        uint32_t rb = a.bin_off[bin] + part * a.per_item;
+        uint32_t re = min(a.bin_off[bin] + cnt, rb + a.per_item);
+        const uint32_t *tw = reinterpret_cast<const uint32_t *>(a.table) + bin * uint64_t(nwords);
+        for (uint32_t wi = tid; wi < nwords; wi += nthr) s_tab[wi] = tw[wi];
+        __syncthreads();
+        for (uint32_t r = rb + tid; r < re; r += nthr) {
+            uint64_t rec = a.sorted[r];
+            uint64_t j0 = rec & 0xFFFFFFFFull;
+            uint32_t nk = uint32_t((rec >> 32) & 31) + 1;
+            uint32_t lbase = uint32_t((rec >> 37) & ((1u << lshift) - 1)) << g.bse;
+            uint32_t A0, A1, A2, A3;
+            load_aligned(a.pk, j0, A0, A1, A2, A3);
+            WinMin<WINC> wm;
+            for_each_smer(A0, A1, A2, A3, nk + g.k - 1, g.s, [&](uint32_t j, uint64_t smer) {
+                uint64_t h = xs64(smer);
+                uint32_t est = Ctr<BITS>::kMax;
+                for (uint32_t t = 0; t < g.nh; ++t) {
+                    uint32_t v = lt[lbase + (uint32_t(h) & amask)];
+                    est = v < est ? v : est;
+                    if (t + 1 < g.nh) h = xs64(h);
+                }
+                uint32_t ab = wm.push(est, g.win);
+                if (j + 1 >= g.win) {
+                    uint64_t q = j + 1 - g.win;
+                    if (a.out) reinterpret_cast<T *>(a.out)[j0 + q] = T(ab);
+                    ++queried;
+                    positive += ab > 0;
+                    checksum += ab;
+                }
+            });
+        }
+        __syncthreads();
+    }
+    uint64_t vals[3] = {queried, positive, checksum};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        uint64_t v = vals[q];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) s_red[q][wid] = v;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long *st = (unsigned long long *)a.stats;
+        uint64_t t[3] = {0, 0, 0};
+        for (int q = 0; q < 3; ++q)
+            for (uint32_t i = 0; i < nthr / 64; ++i) t[q] += s_red[q][i];
+        if (t[0]) atomicAdd(&st[kStQueried], t[0]);
+        if (t[1]) atomicAdd(&st[kStPositive], t[1]);
+        if (t[2]) atomicAdd(&st[kStChecksum], t[2]);
+    }
+}
+
+// filter.rs:50-55, 267-275: histogram of every counter value. Values below
+// kSmall are counted in LDS; larger 8/16-bit values go straight to a dense
+// global array; larger 32-bit values go to an open-addressing table.
+constexpr uint32_t kHistSmall = 4096;
+constexpr uint32_t kHistBigCap = 1u << 20;
+
+template <int BITS>
+__global__ __launch_bounds__(256) void k_counter_hist(const void *table, uint64_t slots,
+                                                       unsigned long long *dense,
+                                                       uint32_t *big_keys,
+                                                       unsigned long long *big_counts,
+                                                       uint64_t *stats) {
+    __shared__ uint32_t s_h[kHistSmall];
+    for (uint32_t i = threadIdx.x; i < kHistSmall; i += blockDim.x) s_h[i] = 0;
+    __syncthreads();
+    const typename Ctr<BITS>::T *t = reinterpret_cast<const typename Ctr<BITS>::T *>(table);
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < slots;
+         i += uint64_t(gridDim.x) * blockDim.x) {
+        uint32_t v = t[i];
+        if (v < kHistSmall) {
+            atomicAdd(&s_h[v], 1u);
+        } else if (BITS < 32) {
+            atomicAdd(&dense[v], 1ull);
+        } else {
+            uint32_t hsh = uint32_t(xs64(v)) & (kHistBigCap - 1);
+            for (uint32_t probe = 0; probe < kHistBigCap; ++probe) {
+                uint32_t key = big_keys[hsh];
+                if (key == 0) {
+                    uint32_t prev = atomicCAS(&big_keys[hsh], 0u, v);
+                    if (prev == 0 || prev == v) {
+                        atomicAdd(&big_counts[hsh], 1ull);
+                        break;
+                    }
+                    key = prev;
+                }
+                if (key == v) {
+                    atomicAdd(&big_counts[hsh], 1ull);
+                    break;
+                }
+                hsh = (hsh + 1) & (kHistBigCap - 1);
+                if (probe + 1 == kHistBigCap) atomicAdd((unsigned long long *)&stats[kStOverflow], 1ull);
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kHistSmall; i += blockDim.x)
+        if (s_h[i]) atomicAdd(&dense[i], (unsigned long long)s_h[i]);
+}
+
+// ----------------------------------------------------------------------------
+// Host side
+// ----------------------------------------------------------------------------
+#define SCB_HIP(call)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return Status{8, std::string("device error: ") + #call + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    hipError_t reserve(size_t want) {
+        if (want <= n) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), std::max<size_t>(want, 1) * sizeof(T));
+        if (e == hipSuccess) n = want;
+        return e;
+    }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+class Workspace {
+   public:
+    hipStream_t stream = nullptr;
+    DevBuf<uint32_t> pk, valid;
+    DevBuf<uint64_t> rec, sorted, work, stats;
+    DevBuf<uint32_t> bin_count, bin_off, cursor;
+    DevBuf<uint8_t> h_bases_dev;   // staging for host batches
+    DevBuf<uint64_t> h_offsets_dev;
+    DevBuf<uint8_t> out_dev;
+    DevBuf<unsigned long long> hist_dense, hist_big_counts;
+    DevBuf<uint32_t> hist_big_keys;
+    uint64_t *h_stats = nullptr;   // pinned
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    ~Workspace() {
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (h_stats) (void)hipHostFree(h_stats);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+Status make_workspace(std::unique_ptr<Workspace> &ws) {
+    ws.reset(new Workspace);
+    SCB_HIP(hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking));
+    SCB_HIP(hipHostMalloc(reinterpret_cast<void **>(&ws->h_stats), kStCount * sizeof(uint64_t),
+                          hipHostMallocDefault));
+    SCB_HIP(ws->stats.reserve(kStCount));
+    for (auto &e : ws->ev) SCB_HIP(hipEventCreate(&e));
+    return Status::ok();
+}
+
+thread_local StageTimes g_stage_times;
+
+void mark(Workspace &ws, int i) { (void)hipEventRecord(ws.ev[i], ws.stream); }
+
+// Elapsed ms between recorded events (after the stream has been synchronized).
+void collect_times(Workspace &ws, bool batched) {
+    StageTimes t{};
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, ws.ev[0], ws.ev[1]) == hipSuccess) t.scan_ms = ms;
+    if (batched) {
+        if (hipEventElapsedTime(&ms, ws.ev[1], ws.ev[2]) == hipSuccess) t.plan_scatter_ms = ms;
+        if (hipEventElapsedTime(&ms, ws.ev[2], ws.ev[3]) == hipSuccess) t.bin_ms = ms;
+        if (hipEventElapsedTime(&ms, ws.ev[0], ws.ev[3]) == hipSuccess) t.total_ms = ms;
+    } else {
+        t.total_ms = t.scan_ms;
+    }
+    t.batched = batched;
+    g_stage_times = t;
+}
+
+void fill(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st) {
+    if (!n) return;
+    uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_u32, dim3(unsigned(blocks)), dim3(256), 0, st, p, n, v);
+}
+
+int cu_count(int dev) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 256;
+    return prop.multiProcessorCount;
+}
+
+template <class K>
+int persistent_grid(K kernel, int threads, size_t lds, int dev) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    return per_cu * cu_count(dev);
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+// Engine
+// ----------------------------------------------------------------------------
+int device_count() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+Status Engine::create(const Geometry &g, std::unique_ptr<Engine> &out) {
+    if (device_count() <= 0) return Status{8, "device error: no HIP device is visible"};
+    std::unique_ptr<Engine> e(new Engine);
+    e->geo_ = g;
+    SCB_HIP(hipGetDevice(&e->device_));
+    SCB_HIP(hipMalloc(&e->d_table_, std::max<uint64_t>(g.table_bytes(), 4)));
+    SCB_HIP(hipMemset(e->d_table_, 0, g.table_bytes()));
+    SCB_HIP(hipDeviceSynchronize());
+    // keep query bins within LDS at every width (u32 counters: 2^15 slots = 128 KiB)
+    e->tune_.insert_bin_log = std::min<uint32_t>(15, g.cse);
+    e->tune_.query_bin_log = std::min<uint32_t>(g.bits == 8 ? 15 : 15, g.cse);
+    out = std::move(e);
+    return Status::ok();
+}
+
+Engine::~Engine() {
+    pool_.clear();
+    if (d_table_) (void)hipFree(d_table_);
+}
+
+std::unique_ptr<Workspace> Engine::acquire() const {
+    {
+        std::lock_guard<std::mutex> lk(pool_mu_);
+        if (!pool_.empty()) {
+            auto ws = std::move(pool_.back());
+            pool_.pop_back();
+            return ws;
+        }
+    }
+    std::unique_ptr<Workspace> ws;
+    if (!make_workspace(ws).good()) return nullptr;
+    return ws;
+}
+
+void Engine::release(std::unique_ptr<Workspace> ws) const {
+    if (!ws) return;
+    std::lock_guard<std::mutex> lk(pool_mu_);
+    pool_.push_back(std::move(ws));
+}
+
+Status Engine::download(void *host, uint64_t bytes) const {
+    if (bytes != geo_.table_bytes()) return Status{6, "internal state error: table size mismatch"};
+    SCB_HIP(hipMemcpy(host, d_table_, bytes, hipMemcpyDeviceToHost));
+    return Status::ok();
+}
+
+Status Engine::upload(const void *host, uint64_t bytes) {
+    if (bytes != geo_.table_bytes()) return Status{6, "internal state error: table size mismatch"};
+    SCB_HIP(hipMemcpy(d_table_, host, bytes, hipMemcpyHostToDevice));
+    return Status::ok();
+}
+
+namespace {
+
+struct Launch {
+    const Geometry &geo;
+    DevGeo dg;
+    Workspace &ws;
+    void *table;
+    int device;
+};
+
+template <int MODE>
+Status launch_scan(Launch &L, const DeviceBatch &b, uint32_t bin_shift, uint32_t *valid_bits,
+                   void *out, bool with_records) {
+    ScanArgs a{};
+    a.bases = b.d_bases;
+    a.offsets = b.d_offsets;
+    a.n_reads = b.n_reads;
+    a.n_bases = b.n_bases;
+    a.pk = L.ws.pk.p;
+    a.valid_bits = valid_bits;
+    a.rec = L.ws.rec.p;
+    a.bin_count = L.ws.bin_count.p;
+    a.bin_shift = bin_shift;
+    a.stats = L.ws.stats.p;
+    a.table = L.table;
+    a.out = out;
+    (void)with_records;
+    unsigned tiles = unsigned((b.n_bases + kTile - 1) / kTile);
+    if (!tiles) return Status::ok();
+    bool wide = L.geo.win > 8;
+    dim3 grid(tiles), blk(kTileThreads);
+    hipStream_t st = L.ws.stream;
+#define SCB_SCAN(BITS)                                                                         \
+    do {                                                                                       \
+        if (wide) hipLaunchKernelGGL((k_scan<MODE, BITS, 32>), grid, blk, 0, st, a, L.dg);     \
+        else hipLaunchKernelGGL((k_scan<MODE, BITS, 8>), grid, blk, 0, st, a, L.dg);           \
+    } while (0)
+    switch (L.geo.bits) {
+        case 8: SCB_SCAN(8); break;
+        case 16: SCB_SCAN(16); break;
+        default: SCB_SCAN(32); break;
+    }
+#undef SCB_SCAN
+    SCB_HIP(hipGetLastError());
+    return Status::ok();
+}
+
+// Batched front half shared by insert and query: scan, plan, scatter.
+Status batched_front(Launch &L, const DeviceBatch &b, uint32_t bin_log, uint32_t per_item,
+                     uint32_t *valid_bits, uint64_t &nbins_out) {
+    const Geometry &g = L.geo;
+    Workspace &ws = L.ws;
+    uint64_t nbins = uint64_t(1) << (g.cse - bin_log);
+    uint32_t bin_shift = bin_log - g.bse;
+    uint64_t tiles = (b.n_bases + kTile - 1) / kTile;
+    uint64_t words = tiles * (kTile / 16) + kPkPadWords;  // every tile writes kTile/16 words
+    uint64_t max_rec = b.n_bases + 1;
+    SCB_HIP(ws.pk.reserve(words));
+    SCB_HIP(ws.rec.reserve(max_rec));
+    SCB_HIP(ws.sorted.reserve(max_rec));
+    SCB_HIP(ws.bin_count.reserve(nbins));
+    SCB_HIP(ws.bin_off.reserve(nbins));
+    SCB_HIP(ws.cursor.reserve(nbins));
+    SCB_HIP(ws.work.reserve(nbins + max_rec / per_item + 2));
+    hipStream_t st = ws.stream;
+    fill(reinterpret_cast<uint32_t *>(ws.stats.p), 2 * kStCount, 0, st);
+    fill(ws.bin_count.p, nbins, 0, st);
+    fill(ws.pk.p + (words - kPkPadWords), kPkPadWords, 0, st);
+    mark(ws, 0);
+    Status s = launch_scan<kModeRecords>(L, b, bin_shift, valid_bits, nullptr, true);
+    if (!s.good()) return s;
+    mark(ws, 1);
+    hipLaunchKernelGGL(k_bin_plan, dim3(1), dim3(1024), 0, st, ws.bin_count.p, nbins, ws.bin_off.p,
+                       ws.cursor.p, ws.work.p, per_item, ws.stats.p);
+    SCB_HIP(hipGetLastError());
+    int sgrid = cu_count(L.device) * 8;
+    hipLaunchKernelGGL(k_scatter, dim3(sgrid), dim3(256), 0, st, ws.rec.p, ws.stats.p, ws.cursor.p,
+                       ws.sorted.p, bin_shift);
+    SCB_HIP(hipGetLastError());
+    mark(ws, 2);
+    nbins_out = nbins;
+    return Status::ok();
+}
+
+Status finish(Workspace &ws) {
+    SCB_HIP(hipMemcpyAsync(ws.h_stats, ws.stats.p, kStCount * sizeof(uint64_t),
+                           hipMemcpyDeviceToHost, ws.stream));
+    SCB_HIP(hipStreamSynchronize(ws.stream));
+    return Status::ok();
+}
+
+template <class K>
+Status set_lds(K kernel, size_t bytes) {
+    SCB_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+    return Status::ok();
+}
+
+Status check_batch(const Geometry &, const DeviceBatch &b) {
+    if (b.n_bases >= kMaxBatchBases)
+        return Status{6, "internal state error: device batch exceeds 2^32 bases"};
+    return Status::ok();
+}
+
+// The batched path packs the physical block into 27 bits of a record and
+// merges whole 32-bit table words per bin.
+bool batched_ok(const Geometry &g, uint32_t bin_log) {
+    return g.bse <= bin_log && g.nb <= (uint64_t(1) << 27) && (uint64_t(1) << bin_log) * g.bytes >= 4;
+}
+
+}  // namespace
+
+Status Engine::insert(const DeviceBatch &b, Method method, InsertResult &r) {
+    r = InsertResult{};
+    Status s = check_batch(geo_, b);
+    if (!s.good()) return s;
+    if (b.n_bases < geo_.k || b.n_reads == 0) return Status::ok();
+    SCB_HIP(hipSetDevice(device_));
+    auto ws = acquire();
+    if (!ws) return Status{8, "device error: cannot create a workspace"};
+    Launch L{geo_, dev_geo(geo_), *ws, d_table_, device_};
+    hipStream_t st = ws->stream;
+    bool batched = method == kBatched && batched_ok(geo_, tune_.insert_bin_log);
+    if (!batched) {
+        fill(reinterpret_cast<uint32_t *>(ws->stats.p), 2 * kStCount, 0, st);
+        mark(*ws, 0);
+        s = launch_scan<kModePlainInsert>(L, b, 0, nullptr, nullptr, false);
+        mark(*ws, 1);
+    } else {
+        uint64_t nbins = 0;
+        uint32_t bin_log = tune_.insert_bin_log;
+        s = batched_front(L, b, bin_log, tune_.max_records_per_item, nullptr, nbins);
+        if (s.good()) {
+            BinArgs a{};
+            a.sorted = ws->sorted.p;
+            a.bin_off = ws->bin_off.p;
+            a.bin_count = ws->bin_count.p;
+            a.work = ws->work.p;
+            a.stats = ws->stats.p;
+            a.pk = ws->pk.p;
+            a.table = d_table_;
+            a.bin_log = bin_log;
+            a.per_item = tune_.max_records_per_item;
+            size_t lds = (size_t(1) << bin_log) * 4;
+            int thr = int(tune_.insert_threads);
+#define SCB_INS(BITS)                                                                           \
+    do {                                                                                        \
+        s = set_lds(k_bin_insert<BITS>, lds);                                                   \
+        if (!s.good()) break;                                                                   \
+        int grid = persistent_grid(k_bin_insert<BITS>, thr, lds, device_);                      \
+        hipLaunchKernelGGL(k_bin_insert<BITS>, dim3(grid), dim3(thr), lds, st, a, L.dg);        \
+    } while (0)
+            switch (geo_.bits) {
+                case 8: SCB_INS(8); break;
+                case 16: SCB_INS(16); break;
+                default: SCB_INS(32); break;
+            }
+#undef SCB_INS
+            if (s.good()) {
+                hipError_t e = hipGetLastError();
+                if (e != hipSuccess) s = Status{8, std::string("device error: k_bin_insert: ") + hipGetErrorString(e)};
+                mark(*ws, 3);
+            }
+        }
+    }
+    if (s.good()) s = finish(*ws);
+    if (s.good()) {
+        collect_times(*ws, batched);
+        r.inserted_kmers = ws->h_stats[kStValid];
+        r.inserted_smers = ws->h_stats[kStSmers];
+        r.super_kmers = ws->h_stats[kStRecords];
+    }
+    release(std::move(ws));
+    return s;
+}
+
+Status Engine::query(const DeviceBatch &b, Method method, void *d_values, uint32_t *d_valid_bits,
+                     QueryResult &r) const {
+    r = QueryResult{};
+    Status s = check_batch(geo_, b);
+    if (!s.good()) return s;
+    if (b.n_bases == 0 || b.n_reads == 0) return Status::ok();
+    SCB_HIP(hipSetDevice(device_));
+    auto ws = acquire();
+    if (!ws) return Status{8, "device error: cannot create a workspace"};
+    Launch L{geo_, dev_geo(geo_), *ws, d_table_, device_};
+    hipStream_t st = ws->stream;
+    if (d_valid_bits) fill(d_valid_bits, (b.n_bases + 31) / 32, 0, st);
+    bool batched = method == kBatched && batched_ok(geo_, tune_.query_bin_log);
+    if (b.n_bases < geo_.k) {
+        fill(reinterpret_cast<uint32_t *>(ws->stats.p), 2 * kStCount, 0, st);
+    } else if (!batched) {
+        fill(reinterpret_cast<uint32_t *>(ws->stats.p), 2 * kStCount, 0, st);
+        // the plain scan writes validity itself
+        mark(*ws, 0);
+        s = launch_scan<kModePlainQuery>(L, b, 0, d_valid_bits, d_values, false);
+        mark(*ws, 1);
+    } else {
+        uint64_t nbins = 0;
+        uint32_t bin_log = tune_.query_bin_log;
+        s = batched_front(L, b, bin_log, tune_.max_records_per_item, d_valid_bits, nbins);
+        if (s.good()) {
+            BinArgs a{};
+            a.sorted = ws->sorted.p;
+            a.bin_off = ws->bin_off.p;
+            a.bin_count = ws->bin_count.p;
+            a.work = ws->work.p;
+            a.stats = ws->stats.p;
+            a.pk = ws->pk.p;
+            a.table = d_table_;
+            a.out = d_values;
+            a.bin_log = bin_log;
+            a.per_item = tune_.max_records_per_item;
+            size_t lds = (size_t(1) << bin_log) * geo_.bytes;
+            int thr = int(tune_.query_threads);
+            bool wide = geo_.win > 8;
+#define SCB_QRY(BITS, WC)                                                                       \
+    do {                                                                                        \
+        s = set_lds(k_bin_query<BITS, WC>, lds);                                                \
+        if (!s.good()) break;                                                                   \
+        int grid = persistent_grid(k_bin_query<BITS, WC>, thr, lds, device_);                   \
+        hipLaunchKernelGGL((k_bin_query<BITS, WC>), dim3(grid), dim3(thr), lds, st, a, L.dg);   \
+    } while (0)
+            switch (geo_.bits) {
+                case 8: if (wide) SCB_QRY(8, 32); else SCB_QRY(8, 8); break;
+                case 16: if (wide) SCB_QRY(16, 32); else SCB_QRY(16, 8); break;
+                default: if (wide) SCB_QRY(32, 32); else SCB_QRY(32, 8); break;
+            }
+#undef SCB_QRY
+            if (s.good()) {
+                hipError_t e = hipGetLastError();
+                if (e != hipSuccess) s = Status{8, std::string("device error: k_bin_query: ") + hipGetErrorString(e)};
+                mark(*ws, 3);
+            }
+        }
+    }
+    if (s.good()) s = finish(*ws);
+    if (s.good()) {
+        if (b.n_bases >= geo_.k) collect_times(*ws, batched);
+        r.queried = ws->h_stats[kStQueried];
+        r.positive = ws->h_stats[kStPositive];
+        r.checksum = ws->h_stats[kStChecksum];
+        r.super_kmers = ws->h_stats[kStRecords];
+    }
+    release(std::move(ws));
+    return s;
+}
+
+Status Engine::spectrum(const DeviceBatch &, SpectrumResult &) const {
+    return Status{6, "internal state error: spectrum is not available"};
+}
+
+Status Engine::counter_histogram(std::vector<std::pair<uint64_t, uint64_t>> &rows) const {
+    rows.clear();
+    SCB_HIP(hipSetDevice(device_));
+    auto ws = acquire();
+    if (!ws) return Status{8, "device error: cannot create a workspace"};
+    hipStream_t st = ws->stream;
+    uint64_t dense_n = geo_.bits == 8 ? 256 : (geo_.bits == 16 ? 65536 : kHistSmall);
+    Status s;
+    do {
+        hipError_t e;
+        if ((e = ws->hist_dense.reserve(dense_n)) != hipSuccess ||
+            (e = ws->hist_big_keys.reserve(geo_.bits == 32 ? kHistBigCap : 1)) != hipSuccess ||
+            (e = ws->hist_big_counts.reserve(geo_.bits == 32 ? kHistBigCap : 1)) != hipSuccess) {
+            s = Status{8, std::string("device error: ") + hipGetErrorString(e)};
+            break;
+        }
+        fill(reinterpret_cast<uint32_t *>(ws->hist_dense.p), dense_n * 2, 0, st);
+        fill(reinterpret_cast<uint32_t *>(ws->stats.p), 2 * kStCount, 0, st);
+        if (geo_.bits == 32) {
+            fill(ws->hist_big_keys.p, kHistBigCap, 0, st);
+            fill(reinterpret_cast<uint32_t *>(ws->hist_big_counts.p), 2 * kHistBigCap, 0, st);
+        }
+        int grid = cu_count(device_) * 8;
+#define SCB_HIST(BITS)                                                                         \
+    hipLaunchKernelGGL(k_counter_hist<BITS>, dim3(grid), dim3(256), 0, st, d_table_, geo_.slots, \
+                       ws->hist_dense.p, ws->hist_big_keys.p, ws->hist_big_counts.p, ws->stats.p)
+        switch (geo_.bits) {
+            case 8: SCB_HIST(8); break;
+            case 16: SCB_HIST(16); break;
+            default: SCB_HIST(32); break;
+        }
+#undef SCB_HIST
+        if ((e = hipGetLastError()) != hipSuccess) {
+            s = Status{8, std::string("device error: k_counter_hist: ") + hipGetErrorString(e)};
+            break;
+        }
+        std::vector<unsigned long long> dense(dense_n);
+        std::vector<uint32_t> keys;
+        std::vector<unsigned long long> cnts;
+        if ((e = hipMemcpyAsync(dense.data(), ws->hist_dense.p, dense_n * 8, hipMemcpyDeviceToHost, st)) != hipSuccess) {
+            s = Status{8, std::string("device error: ") + hipGetErrorString(e)};
+            break;
+        }
+        if (geo_.bits == 32) {
+            keys.resize(kHistBigCap);
+            cnts.resize(kHistBigCap);
+            (void)hipMemcpyAsync(keys.data(), ws->hist_big_keys.p, kHistBigCap * 4, hipMemcpyDeviceToHost, st);
+            (void)hipMemcpyAsync(cnts.data(), ws->hist_big_counts.p, kHistBigCap * 8, hipMemcpyDeviceToHost, st);
+        }
+        s = finish(*ws);
+        if (!s.good()) break;
+        if (ws->h_stats[kStOverflow]) {
+            s = Status{8, "device error: counter histogram has more than 2^20 distinct large values"};
+            break;
+        }
+        for (uint64_t v = 0; v < dense_n; ++v)
+            if (dense[v]) rows.emplace_back(v, dense[v]);
+        for (uint32_t i = 0; i < keys.size(); ++i)
+            if (keys[i]) rows.emplace_back(keys[i], cnts[i]);
+        std::sort(rows.begin(), rows.end());
+    } while (false);
+    release(std::move(ws));
+    return s;
+}
+
+// ---- host-memory batches ---------------------------------------------------
+namespace {
+
+// Splits [0, n) into chunks whose byte size stays below limit (a single record
+// larger than limit is an error).
+template <class Fn>
+Status for_each_chunk(const uint64_t *offsets, uint64_t n, uint64_t limit, Fn &&fn) {
+    uint64_t r = 0;
+    while (r < n) {
+        uint64_t e = r;
+        while (e < n && offsets[e + 1] - offsets[r] <= limit) ++e;
+        if (e == r) return Status{6, "internal state error: a record exceeds the device batch limit"};
+        Status s = fn(r, e);
+        if (!s.good()) return s;
+        r = e;
+    }
+    return Status::ok();
+}
+
+}  // namespace
+
+Status Engine::insert_host(const uint8_t *bases, const uint64_t *offsets, uint64_t n, Method method,
+                           InsertResult &r) {
+    r = InsertResult{};
+    SCB_HIP(hipSetDevice(device_));
+    uint64_t limit = std::min<uint64_t>(kMaxBatchBases, uint64_t(1) << 30);
+    DevBuf<uint8_t> db;
+    DevBuf<uint64_t> doff;
+    std::vector<uint64_t> loc;
+    return for_each_chunk(offsets, n, limit, [&](uint64_t r0, uint64_t r1) -> Status {
+        uint64_t nb = offsets[r1] - offsets[r0];
+        SCB_HIP(db.reserve(nb + 4));
+        SCB_HIP(doff.reserve(r1 - r0 + 1));
+        loc.resize(r1 - r0 + 1);
+        for (uint64_t i = r0; i <= r1; ++i) loc[i - r0] = offsets[i] - offsets[r0];
+        SCB_HIP(hipMemcpy(db.p, bases + offsets[r0], nb, hipMemcpyHostToDevice));
+        SCB_HIP(hipMemcpy(doff.p, loc.data(), loc.size() * 8, hipMemcpyHostToDevice));
+        DeviceBatch b{db.p, doff.p, r1 - r0, nb};
+        InsertResult part;
+        Status s = insert(b, method, part);
+        if (!s.good()) return s;
+        r.inserted_kmers += part.inserted_kmers;
+        r.inserted_smers += part.inserted_smers;
+        r.super_kmers += part.super_kmers;
+        return Status::ok();
+    });
+}
+
+Status Engine::query_host(const uint8_t *bases, const uint64_t *offsets, uint64_t n, Method method,
+                          uint64_t *values, uint8_t *valid, QueryResult &r) const {
+    r = QueryResult{};
+    SCB_HIP(hipSetDevice(device_));
+    uint64_t limit = std::min<uint64_t>(kMaxBatchBases, uint64_t(1) << 29);
+    DevBuf<uint8_t> db, dv;
+    DevBuf<uint64_t> doff;
+    DevBuf<uint32_t> dbits;
+    std::vector<uint64_t> loc;
+    std::vector<uint8_t> hv;
+    std::vector<uint32_t> hb;
+    const uint32_t W = geo_.bytes;
+    return for_each_chunk(offsets, n, limit, [&](uint64_t r0, uint64_t r1) -> Status {
+        uint64_t nb = offsets[r1] - offsets[r0];
+        SCB_HIP(db.reserve(nb + 4));
+        SCB_HIP(doff.reserve(r1 - r0 + 1));
+        bool want = values || valid;
+        if (want) {
+            SCB_HIP(dv.reserve(nb * W + 4));
+            SCB_HIP(dbits.reserve((nb + 31) / 32 + 1));
+        }
+        loc.resize(r1 - r0 + 1);
+        for (uint64_t i = r0; i <= r1; ++i) loc[i - r0] = offsets[i] - offsets[r0];
+        SCB_HIP(hipMemcpy(db.p, bases + offsets[r0], nb, hipMemcpyHostToDevice));
+        SCB_HIP(hipMemcpy(doff.p, loc.data(), loc.size() * 8, hipMemcpyHostToDevice));
+        DeviceBatch b{db.p, doff.p, r1 - r0, nb};
+        QueryResult part;
+        Status s = query(b, method, want ? dv.p : nullptr, want ? dbits.p : nullptr, part);
+        if (!s.good()) return s;
+        r.queried += part.queried;
+        r.positive += part.positive;
+        r.checksum += part.checksum;
+        r.super_kmers += part.super_kmers;
+        if (want) {
+            hv.resize(nb * W);
+            hb.resize((nb + 31) / 32);
+            SCB_HIP(hipMemcpy(hv.data(), dv.p, nb * W, hipMemcpyDeviceToHost));
+            SCB_HIP(hipMemcpy(hb.data(), dbits.p, hb.size() * 4, hipMemcpyDeviceToHost));
+            uint64_t o = offsets[r0];
+            for (uint64_t i = 0; i < nb; ++i) {
+                bool ok = (hb[i >> 5] >> (i & 31)) & 1;
+                if (valid) valid[o + i] = ok;
+                if (values) {
+                    uint64_t v = 0;
+                    if (ok) {
+                        if (W == 1) v = hv[i];
+                        else if (W == 2) v = reinterpret_cast<const uint16_t *>(hv.data())[i];
+                        else v = reinterpret_cast<const uint32_t *>(hv.data())[i];
+                    }
+                    values[o + i] = v;
+                }
+            }
+        }
+        return Status::ok();
+    });
+}
+
+Status Engine::spectrum_host(const uint8_t *, const uint64_t *, uint64_t, SpectrumResult &) const {
+    return Status{6, "internal state error: spectrum is not available"};
+}
+
+StageTimes last_stage_times() { return g_stage_times; }
+
+Status make_geometry(uint64_t k, uint64_t m, uint64_t s, uint64_t nh, uint32_t bits, uint32_t cse,
+                     uint32_t bse, Geometry &g) {
+    g = Geometry{};
+    g.k = uint32_t(k), g.m = uint32_t(m), g.s = uint32_t(s), g.nh = uint32_t(nh);
+    g.w = g.k - g.m + 1;
+    g.win = g.k - g.s + 1;
+    g.bits = bits;
+    g.bytes = bits / 8;
+    g.cse = cse, g.bse = bse;
+    g.slots = uint64_t(1) << cse;
+    g.nb = g.slots >> bse;
+    g.block_mask = g.nb - 1;
+    g.shard_count = std::min<uint64_t>(g.nb, 1024);
+    g.subblocks = g.nb / g.shard_count;
+    g.shard_mask = uint32_t(g.shard_count - 1);
+    g.shard_shift = uint32_t(__builtin_ctzll(g.shard_count));
+    g.sub_log = uint32_t(__builtin_ctzll(g.subblocks));
+    return Status::ok();
+}
+
+}  // namespace scb

@@ -1,0 +1,109 @@
+"""Seeded synthetic read sets for the benchmark workloads (numpy only).
+
+Genomes are i.i.d. uniform ACGT; reads start at uniform positions and are
+reverse-complemented with probability 1/2; errors are i.i.d. per base. Genome,
+read and error streams use separate seeds, so the CPU baseline and the GPU read
+identical bytes.
+"""
+import numpy as np
+
+ACGT = np.frombuffer(b"ACGT", dtype=np.uint8)
+_COMP = np.zeros(256, dtype=np.uint8)
+for a, b in zip(b"ACGTN", b"TGCAN"):
+    _COMP[a] = b
+
+
+def genome(length, seed):
+    rng = np.random.default_rng([seed, 1])
+    return ACGT[rng.integers(0, 4, size=length, dtype=np.uint8)]
+
+
+def insert_repeats(g, repeat_len, copies, seed):
+    """Inserts one random element of repeat_len bases `copies` times."""
+    rng = np.random.default_rng([seed, 2])
+    elem = ACGT[rng.integers(0, 4, size=repeat_len, dtype=np.uint8)]
+    pos = np.sort(rng.integers(0, g.size, size=copies))
+    parts, prev = [], 0
+    for p in pos:
+        parts.append(g[prev:p])
+        parts.append(elem)
+        prev = p
+    parts.append(g[prev:])
+    return np.concatenate(parts)
+
+
+def fixed_reads(g, n_reads, read_len, seed, sub_rate=0.0, n_rate=0.0):
+    """n_reads reads of read_len bases: returns (bases u8, offsets u64[n+1])."""
+    rng = np.random.default_rng([seed, 3])
+    starts = rng.integers(0, g.size - read_len + 1, size=n_reads)
+    idx = starts[:, None] + np.arange(read_len)[None, :]
+    reads = g[idx]
+    rc = rng.random(n_reads) < 0.5
+    reads[rc] = _COMP[reads[rc][:, ::-1]]
+    erng = np.random.default_rng([seed, 4])
+    if sub_rate > 0:
+        mask = erng.random(reads.shape) < sub_rate
+        codes = np.searchsorted(ACGT, reads[mask])
+        shift = erng.integers(1, 4, size=codes.size)
+        reads[mask] = ACGT[(codes + shift) % 4]
+    if n_rate > 0:
+        reads[erng.random(reads.shape) < n_rate] = ord("N")
+    bases = np.ascontiguousarray(reads.reshape(-1))
+    offsets = np.arange(n_reads + 1, dtype=np.uint64) * np.uint64(read_len)
+    return bases, offsets
+
+
+def coverage_reads(g, coverage, read_len, seed, sub_rate=0.0, n_rate=0.0):
+    n = int(round(g.size * coverage / read_len))
+    return fixed_reads(g, n, read_len, seed, sub_rate, n_rate)
+
+
+def variable_reads(g, total_bases, median, sigma, lo, hi, seed, sub=0.0, ins=0.0, dele=0.0):
+    """Log-normal read lengths with substitutions, insertions and deletions."""
+    rng = np.random.default_rng([seed, 5])
+    lens = []
+    acc = 0
+    while acc < total_bases:
+        L = int(np.clip(np.exp(rng.normal(np.log(median), sigma)), lo, hi))
+        lens.append(L)
+        acc += L
+    out, offs = [], [0]
+    for L in lens:
+        s = int(rng.integers(0, g.size - L + 1))
+        r = g[s:s + L].copy()
+        if rng.random() < 0.5:
+            r = _COMP[r[::-1]]
+        u = rng.random(L)
+        keep = u >= dele
+        codes = np.searchsorted(ACGT, r)
+        sub_m = (u >= dele) & (u < dele + sub)
+        codes[sub_m] = (codes[sub_m] + rng.integers(1, 4, size=int(sub_m.sum()))) % 4
+        r = ACGT[codes]
+        ins_m = (u >= dele + sub) & (u < dele + sub + ins)
+        pieces = []
+        for i in np.flatnonzero(ins_m):
+            pieces.append(i)
+        if pieces:
+            extra = ACGT[rng.integers(0, 4, size=len(pieces))]
+            r = np.insert(r, pieces, extra)
+            keep = np.insert(keep, pieces, True)
+        r = r[keep]
+        out.append(r)
+        offs.append(offs[-1] + r.size)
+    return np.concatenate(out), np.array(offs, dtype=np.uint64)
+
+
+def random_records(n, lo, hi, seed, n_rate=0.0, lower_rate=0.0):
+    """Random records of uniform length in [lo, hi], optionally with N and lowercase."""
+    rng = np.random.default_rng([seed, 6])
+    lens = rng.integers(lo, hi + 1, size=n)
+    total = int(lens.sum())
+    b = ACGT[rng.integers(0, 4, size=total, dtype=np.uint8)]
+    if n_rate > 0:
+        b[rng.random(total) < n_rate] = ord("N")
+    if lower_rate > 0:
+        m = (rng.random(total) < lower_rate) & (b != ord("N"))
+        b[m] = b[m] + 32
+    offs = np.zeros(n + 1, dtype=np.uint64)
+    offs[1:] = np.cumsum(lens)
+    return b, offs

@@ -1,0 +1,139 @@
+"""GPU parity: counters, per-k-mer abundances and stream reports of the HIP
+engine (batched and plain methods) must equal the CPU oracle bit for bit on the
+same seeded inputs."""
+import zlib
+
+import numpy as np
+import pytest
+
+from scbf_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+METHODS = (0, 1)  # batched, plain
+
+# (name, k, m, s, h, bits, cse, bse)
+GEOMETRIES = [
+    ("k9m5s7_u8", 9, 5, 7, 3, 8, 16, 6),        # tests/backend_matrix.rs:8-20
+    ("k9m5s9_u16", 9, 5, 9, 3, 16, 16, 6),
+    ("k11m5s8_u32", 11, 5, 8, 3, 32, 16, 6),
+    ("k13m7s9_u16", 13, 7, 9, 3, 16, 16, 6),
+    ("k5m3s4_u8", 5, 3, 4, 2, 8, 12, 5),          # api.rs:1044-1056
+    ("default_small_u16", 31, 21, 25, 4, 16, 20, 9),
+    ("k21m15s17_u32", 21, 15, 17, 3, 32, 20, 10),
+    ("k31m11s15_u16_widewin", 31, 11, 15, 2, 16, 18, 8),  # fimpera window 17 (> 8)
+    ("k31m21s25_u8_sat", 31, 21, 25, 4, 8, 14, 9),  # dense: saturates u8
+]
+
+
+def _cfg(scb, k, m, s, h, bits, cse, bse):
+    return scb.SuperCountingBloomConfig(k=k, m=m, s=s, n_hashes=h, counter_bits=bits,
+                                        counter_slots_exponent=cse, block_slots_exponent=bse,
+                                        threads=2, queue=4)
+
+
+def _records(seed, k):
+    # mixed lengths (incl. < k), N runs, lowercase, empty records
+    b1, o1 = synth.random_records(300, 0, 400, seed, n_rate=0.01, lower_rate=0.02)
+    g = synth.genome(20000, seed)
+    b2, o2 = synth.fixed_reads(g, 800, 150, seed, sub_rate=0.01, n_rate=0.001)
+    bases = np.concatenate([b1, b2])
+    offsets = np.concatenate([o1, o2[1:] + o1[-1]]).astype(np.uint64)
+    return bases, offsets
+
+
+@pytest.mark.parametrize("geo", GEOMETRIES, ids=[g[0] for g in GEOMETRIES])
+@pytest.mark.parametrize("method", METHODS, ids=["batched", "plain"])
+def test_insert_query_parity(gpu, oracle, geo, method):
+    scb = gpu
+    name, k, m, s, h, bits, cse, bse = geo
+    print(f"case {name} method={method}", flush=True)
+    bases, offsets = _records(zlib.crc32(name.encode()) & 0xFFFF, k)
+    ocfg = oracle.make_cfg(k, m, s, h, bits, cse, bse)
+    of = oracle.Filter(ocfg)
+    ok_kmers, ok_smers = oracle.insert_records(of, bases, offsets)
+    if name.endswith("_sat"):  # repeat the insert to saturate 8-bit counters
+        for _ in range(3):
+            oracle.insert_records(of, bases, offsets)
+
+    b = scb.SuperCountingBloomBuilder(_cfg(scb, k, m, s, h, bits, cse, bse))
+    b.set_method(method)
+    rep = b.add_sequences((bases, offsets))
+    if name.endswith("_sat"):
+        for _ in range(3):
+            b.add_sequences((bases, offsets))
+    assert rep.inserted_kmers == ok_kmers
+    assert rep.inserted_smers == ok_smers
+    assert rep.records_processed == offsets.size - 1
+    got = b.counters().astype(np.uint32)
+    want = of.counters()
+    assert np.array_equal(got, want), f"counters differ at {np.flatnonzero(got != want)[:10]}"
+
+    idx = b.into_index()
+    idx.set_method(method)
+    qb, qo = _records(1000 + (zlib.crc32(name.encode()) & 0xFFFF), k)
+    qb = np.concatenate([qb, bases])
+    qo = np.concatenate([qo, offsets[1:] + qo[-1]]).astype(np.uint64)
+    vals, valid, qrep = idx.estimate_sequences(qb, qo)
+    wv, wok = oracle.estimate_records(of, qb, qo, k)
+    assert np.array_equal(valid, wok)
+    assert np.array_equal(vals[wok], wv[wok])
+    assert qrep.valid_kmers == int(wok.sum())
+    assert qrep.positive_estimates == int((wv[wok] > 0).sum())
+    assert qrep.estimate_checksum == int(wv[wok].sum()) & 0xFFFFFFFFFFFFFFFF
+
+
+@pytest.mark.parametrize("method", METHODS, ids=["batched", "plain"])
+def test_single_sequence_api(gpu, oracle, method):
+    scb = gpu
+    k, m, s, h, bits, cse, bse = 5, 3, 4, 2, 8, 12, 5
+    b = scb.SuperCountingBloomBuilder(_cfg(scb, k, m, s, h, bits, cse, bse))
+    b.set_method(method)
+    assert b.add_sequence(b"ACGTACNNACGTAC") == 4   # api.rs:1178
+    of = oracle.Filter(oracle.make_cfg(k, m, s, h, bits, cse, bse))
+    of.insert_ascii(b"ACGTACNNACGTAC")
+    idx = b.into_index()
+    idx.set_method(method)
+    got = idx.estimate_sequence_abundances(b"ACGTACNNACGTAC")
+    v, ok = of.estimate_ascii(b"ACGTACNNACGTAC")
+    assert got == [int(x) if o else None for x, o in zip(v, ok)]
+    assert len(got) == 10
+    assert all(x is not None for x in got[:2]) and all(x is None for x in got[2:8])
+
+
+@pytest.mark.parametrize("bits", [8, 16, 32])
+def test_counter_histogram(gpu, oracle, bits):
+    scb = gpu
+    k, m, s, h, cse, bse = 21, 15, 17, 3, 16, 8
+    bases, offsets = _records(77, k)
+    b = scb.SuperCountingBloomBuilder(_cfg(scb, k, m, s, h, bits, cse, bse))
+    of = oracle.Filter(oracle.make_cfg(k, m, s, h, bits, cse, bse))
+    for _ in range(40):
+        b.add_sequences((bases[:5000], offsets[offsets <= 5000]))
+    for _ in range(40):
+        oracle.insert_records(of, bases[:5000], offsets[offsets <= 5000])
+    idx = b.into_index()
+    assert idx.counter_histogram() == of.counter_histogram()
+
+
+def test_c1_small_parity(gpu, oracle):
+    """configs[0] shape, reduced to 20k reads so the oracle stays fast."""
+    scb = gpu
+    k, m, s, h, bits, cse, bse = 31, 21, 25, 4, 16, 24, 9
+    g = synth.genome(1_000_000, 11)
+    bases, offsets = synth.fixed_reads(g, 20_000, 150, 12)
+    of = oracle.Filter(oracle.make_cfg(k, m, s, h, bits, cse, bse))
+    ok_kmers, ok_smers = oracle.insert_records(of, bases, offsets)
+    for method in METHODS:
+        b = scb.SuperCountingBloomBuilder(_cfg(scb, k, m, s, h, bits, cse, bse))
+        b.set_method(method)
+        rep = b.add_sequences((bases, offsets))
+        assert (rep.inserted_kmers, rep.inserted_smers) == (ok_kmers, ok_smers)
+        assert np.array_equal(b.counters().astype(np.uint32), of.counters())
+        idx = b.into_index()
+        idx.set_method(method)
+        vals, valid, _ = idx.estimate_sequences(bases, offsets)
+        wv, wok = oracle.estimate_records(of, bases, offsets, k)
+        assert np.array_equal(valid, wok) and np.array_equal(vals[wok], wv[wok])
+        # no underestimation: every inserted k-mer has estimate >= 1
+        assert (vals[wok] >= 1).all()

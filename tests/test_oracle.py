@@ -1,0 +1,395 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+Each test checks the oracle against something it does not share code with:
+published known answers, the reference crate's own test assertions, closed
+forms, brute-force re-derivations from the definitions, or invariants that a
+plausible mistake (wrong index, dropped term, swapped operand) would break.
+"""
+import os
+import random
+import struct
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+
+M64 = (1 << 64) - 1
+COMP = {ord("A"): "T", ord("C"): "G", ord("G"): "C", ord("T"): "A"}
+
+
+def rc_str(s: str) -> str:
+    return s.translate(str.maketrans("ACGTacgt", "TGCAtgca"))[::-1]
+
+
+def rand_seq(rng, n, alphabet="ACGT"):
+    return "".join(rng.choice(alphabet) for _ in range(n))
+
+
+# ---- xorshift_u64 (filter.rs:629-635) ------------------------------------------
+def test_xorshift_matches_published_splitmix64_sequence():
+    # Vigna's splitmix64.c with state 0 outputs mix(k * 0x9E3779B97F4A7C15), k = 1..;
+    # xorshift_u64(x) = mix(x + 0x9E37...), so xorshift_u64((k-1) * gamma) is output k.
+    published = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F, 0xF88BB8A8724C81EC]
+    gamma = 0x9E3779B97F4A7C15
+    for i, want in enumerate(published):
+        assert O.xorshift_u64((i * gamma) & M64) == want
+
+
+def test_xorshift_chain_addresses_known_answer():
+    # s-mer word 0, 2^9-slot blocks, 4 hashes; block of minimizer 0 with 2^21 blocks
+    h, addrs = O.xorshift_u64(0), []
+    for _ in range(4):
+        addrs.append(h & 511)
+        h = O.xorshift_u64(h)
+    assert addrs == [431, 111, 145, 104]
+    assert O.xorshift_u64(0) & ((1 << 21) - 1) == 1953199
+    assert O.xorshift_u64(1) == 0x910A2DEC89025CC1
+
+
+# ---- encoding (packed-seq restatement, F1/F2) ------------------------------------
+def test_read_kmer_codes_and_order():
+    # F1: A=0 C=1 T=2 G=3, first base least significant (F2)
+    assert O.read_kmer(b"ACGT", 4, 0) == 0 | (1 << 2) | (3 << 4) | (2 << 6)
+    assert O.read_kmer(b"TTTTG", 1, 4) == 3
+    assert O.read_kmer(b"acgt", 4, 0) == O.read_kmer(b"ACGT", 4, 0)
+
+
+def test_revcomp_word_is_string_reverse_complement():
+    rng = random.Random(3)
+    for _ in range(200):
+        L = rng.randint(1, 32)
+        s = rand_seq(rng, L + 5)
+        pos = rng.randint(0, 5)
+        assert O.read_revcomp_kmer(s.encode(), L, pos) == O.read_kmer(rc_str(s[pos:pos + L]).encode(), L, 0)
+
+
+def test_canonical_kmer_rc_symmetric():
+    assert O.canonical_kmer(b"ACGTAC", 6, 0) == O.canonical_kmer(b"GTACGT", 6, 0)  # filter.rs:748-756
+    rng = random.Random(4)
+    for _ in range(200):
+        L = rng.randint(1, 32)
+        s = rand_seq(rng, L)
+        v = O.canonical_kmer(s.encode(), L, 0)
+        assert v == O.canonical_kmer(rc_str(s).encode(), L, 0)
+        assert v == min(O.read_kmer(s.encode(), L, 0), O.read_kmer(rc_str(s).encode(), L, 0))
+
+
+# ---- ntHash and minimizers (third-party restatement, F3-F8) ----------------------
+def test_nthash_rolling_equals_direct_and_is_rc_symmetric():
+    rng = random.Random(5)
+    for m in (1, 3, 5, 15, 21, 31, 32):
+        s = rand_seq(rng, 200)
+        roll, direct = O.nthash(s, m), O.nthash(s, m, direct=True)
+        assert np.array_equal(roll, direct)
+        r = O.nthash(rc_str(s), m)
+        n = len(s) - m + 1
+        assert np.array_equal(roll, r[::-1][:n])
+
+
+@pytest.mark.parametrize("k,m", [(9, 5), (11, 5), (13, 7), (5, 3), (31, 21), (21, 15), (31, 11), (1, 1)])
+def test_minimizer_scan_matches_brute_force_and_invariants(k, m):
+    rng = random.Random(k * 100 + m)
+    w = k - m + 1
+    for trial in range(20):
+        alphabet = "ACGT" if trial % 3 else "AC"  # low-entropy inputs force hash ties
+        s = rand_seq(rng, rng.randint(k, 300), alphabet)
+        fast = O.minimizer_positions(s, k, m)
+        brute = O.minimizer_positions(s, k, m, brute=True)
+        assert np.array_equal(fast, brute)
+        j = np.arange(fast.size)
+        assert ((fast >= j) & (fast < j + w)).all()  # minimizer lies in its window
+        # strand symmetry: the rc sequence picks the mirrored m-mer in the mirrored window
+        r = O.minimizer_positions(rc_str(s), k, m)
+        L = len(s)
+        assert np.array_equal(fast, (L - m - r)[::-1])
+
+
+def test_super_kmers_partition_and_density():
+    rng = random.Random(7)
+    k, m = 31, 21
+    total_sk = total_w = 0
+    for _ in range(30):
+        s = rand_seq(rng, 2000)
+        starts, mins = O.super_kmers(s, k, m)
+        pos = O.minimizer_positions(s, k, m)
+        assert starts[0] == 0 and (np.diff(starts.astype(np.int64)) > 0).all()
+        # starts are exactly where the minimizer position changes (F7)
+        change = np.flatnonzero(np.diff(pos.astype(np.int64)) != 0) + 1
+        assert np.array_equal(starts[1:], change)
+        for st, mv in zip(starts, mins):
+            p = int(pos[st])
+            assert mv == O.canonical_kmer(s.encode(), m, p)  # F8
+        total_sk += starts.size
+        total_w += pos.size
+    density = total_sk / total_w
+    w = k - m + 1
+    assert abs(density - 2 / (w + 1)) < 0.03
+
+
+# ---- config (config.rs:50-107; tests/pipeline_matrix.rs:187-262) ------------------
+def _base_cfg(**kw):
+    c = dict(k=9, m=5, s=7, n_hashes=3, bits=8, cse=16, bse=6, threads=1, queue=1)
+    c.update(kw)
+    return O.make_cfg(c["k"], c["m"], c["s"], c["n_hashes"], c["bits"], c["cse"], c["bse"], c["threads"],
+                      c["queue"])
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(k=0), "k must be in 1..=32"), (dict(k=10), "canonical SIMD minimizers require odd k"),
+    (dict(k=33), "k must be in 1..=32"), (dict(m=0), "m must be in 1..=k"),
+    (dict(m=10), "m must be in 1..=k"), (dict(s=0), "s must be in 1..=k and <= 32"),
+    (dict(s=10), "s must be in 1..=k and <= 32"), (dict(n_hashes=0), "n_hashes must be positive"),
+    (dict(bse=16), "block_slots_exponent must be smaller than counter_slots_exponent"),
+    (dict(threads=0), "threads must be positive"), (dict(queue=0), "queue must be positive"),
+    (dict(cse=64, bse=6), "counter_slots_exponent is too large"),
+])
+def test_config_rejections(kw, msg):
+    rc, text = O.validate(_base_cfg(**kw))
+    assert rc == 1 and text == msg
+    assert O.validate(_base_cfg())[0] == 0
+
+
+# ---- filter: the reference's own tests replayed on the oracle ----------------------
+BACKEND_CASES = [  # tests/backend_matrix.rs:128-235
+    (8, 9, 5, 7, "ACGTACGTACGTACGTACGTACGTACGT"), (16, 9, 5, 7, "ACGTACGTACGTACGTACGTACGTACGT"),
+    (32, 9, 5, 7, "ACGTACGTACGTACGTACGTACGTACGT"), (8, 9, 5, 9, "ACGTACGTACGTACGTACGTACGTACGT"),
+    (16, 9, 5, 9, "ACGTACGTACGTACGTACGTACGTACGT"), (32, 9, 5, 9, "ACGTACGTACGTACGTACGTACGTACGT"),
+    (8, 11, 5, 8, "ACGTTGCATGTCAGTACGATCGTACGTTAGC"), (16, 11, 5, 8, "ACGTTGCATGTCAGTACGATCGTACGTTAGC"),
+    (32, 11, 5, 8, "ACGTTGCATGTCAGTACGATCGTACGTTAGC"), (8, 13, 7, 9, "TGCATGCAACGTACGTTGCATGCAACGTACGT"),
+    (16, 13, 7, 9, "TGCATGCAACGTACGTTGCATGCAACGTACGT"), (32, 13, 7, 9, "TGCATGCAACGTACGTTGCATGCAACGTACGT"),
+]
+
+
+def _bm_filter(bits, k, m, s, seq, repeats):
+    f = O.Filter(O.make_cfg(k, m, s, 3, bits, 16, 6, 2, 4))
+    for _ in range(repeats):
+        assert f.insert_packed(seq) > 0
+    return f
+
+
+@pytest.mark.parametrize("bits,k,m,s,seq", BACKEND_CASES)
+def test_backend_matrix_properties(bits, k, m, s, seq):
+    # positive_estimates, abundance_vector_length (backend_matrix.rs:39-56)
+    f = _bm_filter(bits, k, m, s, seq, 1)
+    est = f.estimate_packed(seq)
+    assert est.size == len(seq) + 1 - k and (est > 0).all()
+    # stream_matches_materialized_vector (58-69)
+    f3 = _bm_filter(bits, k, m, s, seq, 3)
+    v3 = f3.estimate_packed(seq)
+    assert f3.stream_packed(seq) == (v3.size, int(v3.sum()) & M64)
+    # map_query_consistency (71-79)
+    f2 = _bm_filter(bits, k, m, s, seq, 2)
+    sp = O.Spectrum()
+    assert sp.add_chunk(f2, seq) == len(seq) + 1 - k
+    assert sp.distinct() > 0 and all(v > 0 for v, _ in sp.rows())
+    # repeated_inserts_raise_estimates (81-93)
+    once = _bm_filter(bits, k, m, s, seq, 1).estimate_packed(seq)
+    more = _bm_filter(bits, k, m, s, seq, 4).estimate_packed(seq)
+    assert (more >= once).all()
+
+
+@pytest.mark.parametrize("bits,repeats,floor", [(8, 300, 255), (16, 300, 256), (32, 300, 256)])
+def test_saturation(bits, repeats, floor):  # pipeline_matrix.rs:119-129, 172-185; filter.rs:722-746
+    seq = "ACGTACGTACGTACGTACGTACGTACGT"
+    f = O.Filter(O.make_cfg(9, 5, 7, 3, bits, 16, 6, 1, 2))
+    for _ in range(repeats):
+        f.insert_packed(seq)
+    est = f.estimate_packed(seq)
+    assert (est >= floor).all()
+    if bits == 8:
+        assert (est == 255).all()
+
+
+def test_api_invalid_windows_and_builder_counts():
+    # api.rs:1058-1072
+    f = O.Filter(O.make_cfg(5, 3, 4, 2, 8, 12, 5, 1, 2))
+    f.insert_ascii(b"ACGTACGTACGT")
+    v, ok = f.estimate_ascii(b"ACGTACNNACGTAC")
+    assert v.size == 10 and ok[:2].all() and not ok[2:8].any() and ok[8:].all()
+    # api.rs:1158-1183: add_sequence -> 4 k-mers; r1 + r2 -> 10 k-mers, 14 in total
+    f = O.Filter(O.make_cfg(5, 3, 4, 2, 16, 12, 5, 2, 1))
+    assert f.insert_ascii(b"ACGTACNNACGTAC")[0] == 4
+    assert f.insert_ascii(b"ACGTACGTACGT")[0] + f.insert_ascii(b"ACGTNNACGTAC")[0] == 10
+    # query_fasta windows: 8 + 8 = 16 total, 8 + (2+2) = 10 valid (api.rs:1188-1193)
+    _, ok1 = f.estimate_ascii(b"ACGTACGTACGT")
+    _, ok2 = f.estimate_ascii(b"ACGTNNACGTAC")
+    assert ok1.size + ok2.size == 16 and int(ok1.sum() + ok2.sum()) == 10
+
+
+def test_fastx_counts():
+    # pipeline_matrix.rs:61-82: two 28-bp reads, k=9 -> 2 chunks, 40 k-mers each way
+    f = O.Filter(O.make_cfg(9, 5, 7, 3, 16, 16, 6, 1, 1))
+    reads = [b"ACGTACGTACGTACGTACGTACGTACGT", b"TGCATGCAACGTACGTTGCATGCAACGT"]
+    assert sum(f.insert_ascii(r)[0] for r in reads) == 40
+    q = sum(f.stream_packed(r)[0] for r in reads)
+    assert q == 40
+    assert all((f.estimate_packed(r) > 0).all() for r in reads)
+
+
+# ---- filter: brute force and closed forms ------------------------------------------
+GEOS = [(9, 5, 7, 3, 8, 16, 6), (13, 7, 9, 3, 16, 14, 6), (31, 21, 25, 4, 16, 16, 9),
+        (21, 15, 17, 3, 32, 16, 10), (5, 3, 4, 2, 8, 12, 5), (31, 11, 15, 2, 16, 14, 8)]
+
+
+@pytest.mark.parametrize("geo", GEOS)
+def test_abundances_equal_definition_brute_force(geo):
+    k, m, s, h, bits, cse, bse = geo
+    rng = random.Random(sum(geo))
+    f = O.Filter(O.make_cfg(k, m, s, h, bits, cse, bse))
+    seqs = [rand_seq(rng, rng.randint(0, 250), "ACGTN" if i % 2 else "ACGT") for i in range(40)]
+    for q in seqs:
+        f.insert_ascii(q.encode())
+    for q in seqs + [rand_seq(rng, 120) for _ in range(10)]:
+        v, ok = f.estimate_ascii(q.encode())
+        vb, okb = f.estimate_ascii(q.encode(), brute=True)
+        assert np.array_equal(ok, okb) and np.array_equal(v[ok], vb[okb])
+
+
+@pytest.mark.parametrize("geo", GEOS)
+def test_counter_sum_and_no_underestimation(geo):
+    k, m, s, h, bits, cse, bse = geo
+    rng = random.Random(sum(geo) + 1)
+    f = O.Filter(O.make_cfg(k, m, s, h, bits, cse, bse))
+    genome = rand_seq(rng, 3000)
+    reads = []
+    for _ in range(80):
+        a = rng.randint(0, len(genome) - 150)
+        r = genome[a:a + 150]
+        reads.append(r if rng.random() < 0.5 else rc_str(r))
+    smers = 0
+    for r in reads:
+        smers += f.insert_ascii(r.encode())[1]
+    ctr = f.counters()
+    maxv = (1 << bits) - 1
+    if ctr.max() < maxv:  # no saturation: every s-mer occurrence adds n_hashes
+        assert int(ctr.sum()) == h * smers
+    # true multiplicity of every canonical k-mer, counted on strings
+    mult = {}
+    for r in reads:
+        for i in range(len(r) - k + 1):
+            km = r[i:i + k]
+            key = min(km, rc_str(km))
+            mult[key] = mult.get(key, 0) + 1
+    for r in reads[:20]:
+        v, ok = f.estimate_ascii(r.encode())
+        for i in range(len(r) - k + 1):
+            km = r[i:i + k]
+            assert v[i] >= min(maxv, mult[min(km, rc_str(km))])
+
+
+def test_k1_closed_form():
+    """k=m=s=1, one hash: every base is its own super-k-mer; canonical 1-mer is
+    A,T -> 0 and C,G -> 1; the counter at block xs(c) & mask, address xs(xs(c)) &
+    (Bs-1) counts occurrences."""
+    f = O.Filter(O.make_cfg(1, 1, 1, 1, 16, 10, 4))
+    seq = "ACGTTGCAAAC"
+    f.insert_ascii(seq.encode())
+
+    def xs(x):
+        x = (x + 0x9E3779B97F4A7C15) & M64
+        x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & M64
+        x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & M64
+        return x ^ (x >> 31)
+
+    want = np.zeros(1 << 10, dtype=np.uint32)
+    nb = 1 << 6
+    sc = min(nb, 1024)
+    for ch in seq:
+        c = 0 if ch in "AT" else 1
+        b = xs(c) & (nb - 1)
+        phys = (b & (sc - 1)) * (nb // sc) + (b >> (sc.bit_length() - 1))
+        want[phys * 16 + (xs(c) & 15)] += 1
+    assert np.array_equal(f.counters(), want)
+
+
+# ---- .scb format (api.rs:388-608) ---------------------------------------------------
+def test_scb_header_layout_and_roundtrip(tmp_path):
+    cfg = O.make_cfg(9, 5, 7, 3, 16, 12, 6, 2, 4)
+    f = O.Filter(cfg)
+    f.insert_ascii(b"ACGTACGTACGTACGTTTGACA")
+    p = tmp_path / "x.scb"
+    assert f.save(p, 14)[0] == 0
+    data = p.read_bytes()
+    assert len(data) == 139 + 2 * (1 << 12)
+    assert data[:8] == b"SCBIDX\0\0"
+    (ver, k, m, s, nh, bits, cse, bse, thr, q, ik) = struct.unpack_from("<IQQQQBBBQQQ", data, 8)
+    assert (ver, k, m, s, nh, bits, cse, bse, thr, q, ik) == (2, 9, 5, 7, 3, 16, 12, 6, 2, 4, 14)
+    lay = struct.unpack_from("<QQQQIQQQQ", data, 71)
+    nb = 1 << 6
+    assert lay == (nb, 1, 64, nb - 1, 6, 63, nb, nb - 1, 3)
+    assert np.array_equal(np.frombuffer(data[139:], dtype="<u2"), f.counters())
+    rc, msg, g, ik2 = O.load(p)
+    assert rc == 0 and ik2 == 14 and np.array_equal(g.counters(), f.counters())
+
+
+@pytest.mark.parametrize("mutate,code,text", [
+    (lambda d: b"not an index", 3, "bad SuperCountingBloom index magic"),
+    (lambda d: d[:8] + struct.pack("<I", 3) + d[12:], 3, "unsupported index format version 3"),
+    (lambda d: d[:44] + b"\x0c" + d[45:], 3, "unsupported counter width 12"),
+    (lambda d: d + b"\0", 3, "trailing bytes after serialized index"),
+    (lambda d: d[:-1], 7, "failed to fill whole buffer"),
+    (lambda d: d[:71] + struct.pack("<Q", 3) + d[79:], 3, "stored filter layout does not match config"),
+])
+def test_scb_load_rejections(tmp_path, mutate, code, text):
+    f = O.Filter(O.make_cfg(9, 5, 7, 3, 16, 12, 6, 2, 4))
+    p = tmp_path / "x.scb"
+    f.save(p, 0)
+    p.write_bytes(mutate(p.read_bytes()))
+    rc, msg, _, _ = O.load(p)
+    assert rc == code and text in msg
+
+
+# ---- f64 estimators (api.rs:933-1033) ------------------------------------------------
+def test_filter_only_estimators_closed_forms():
+    from fractions import Fraction as F
+    hist = [(0, 600), (1, 300), (2, 90), (3, 10)]
+    total = 1000
+    p = {a: F(c, total) for a, c in hist}
+    n = {a: p.get(a, F(0)) / p[0] for a in range(4)}
+    lc = {}
+    for a in range(1, 4):
+        c = a * n[a]
+        for prev in range(1, a):
+            c -= prev * lc[prev] * n[a - prev]
+        lc[a] = c / a
+    want = [max(float(total * lc[a] / 3), 0.0) for a in range(1, 4)]
+    got = O.deconvolved_counts(hist, 3, 3)
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-12)
+    # first-order term is the Poisson ratio p1/p0
+    assert abs(got[0] - total * (300 / 600) / 3) < 1e-9
+    rows = O.filter_only_spectrum(hist, 3, 3, 1000)
+    scale = 1000 / sum(want)
+    assert rows == [(a, int(round(want[a - 1] * scale + 0.0))) for a in range(1, 4)]
+    assert O.filter_only_count_hint(hist, 3, 3) == max(1, int(round(sum(want))))
+    # no zero counters -> empty deconvolution -> scaled histogram fallback (api.rs:942-948)
+    rows = O.filter_only_spectrum([(1, 2), (2, 6)], 3, 3, 80)
+    assert rows == [(1, 20), (2, 60), (3, 0)]
+    assert O.filter_only_count_hint([(1, 2)], 3, 3) == 1
+    # api.rs:1113-1127 shape
+    rows = O.filter_only_spectrum(hist, 2, 8, 10)
+    assert rows[0][0] == 1 and rows[-1][0] == 8 and len(rows) == 8
+
+
+def test_summary_mean_median():  # api.rs:855-891
+    n, mean, med = O.summarize([4, 1, 9, 2], [1, 1, 1, 1])
+    assert (n, mean, med) == (4, 4.0, 3.0)
+    n, mean, med = O.summarize([4, 1, 9], [1, 0, 1])
+    assert (n, mean, med) == (2, 6.5, 6.5)
+    n, mean, med = O.summarize([5], [0])
+    assert n == 0 and np.isnan(mean) and np.isnan(med)
+
+
+def test_cpu_baseline_counts_match_serial_oracle():
+    from scbf_amd import synth
+    g = synth.genome(50_000, 9)
+    b, o = synth.fixed_reads(g, 2000, 150, 10, sub_rate=0.01, n_rate=0.002)
+    cfg = O.make_cfg(31, 21, 25, 4, 16, 18, 9, threads=4, queue=16)
+    r = O.cpu_baseline(cfg, b, o, b, o)
+    f = O.Filter(O.make_cfg(31, 21, 25, 4, 16, 18, 9))
+    kmers, smers = O.insert_records(f, b, o)
+    v, ok = O.estimate_records(f, b, o, 31)
+    assert (r.indexed_kmers, r.inserted_smers) == (kmers, smers)
+    assert r.queried_kmers == int(ok.sum())
+    assert r.estimate_checksum == int(v[ok].sum())
+    assert r.positive_estimates == int((v[ok] > 0).sum())
