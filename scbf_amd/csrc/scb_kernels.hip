@@ -60,6 +60,7 @@ enum StatSlot {
     kStWorkHead = 7,
     kStOverflow = 8,
     kStDistinct = 9,
+    kStGlobalKmers = 10,
     kStCount = 16
 };
 
@@ -267,6 +268,7 @@ struct ScanArgs {
     uint32_t *valid_bits;  // per-window validity out or null
     uint64_t *rec;         // records out (batched)
     uint32_t *bin_count;   // records per bin (batched)
+    uint32_t *bin_kmers;   // k-mers per bin (spectrum) or null
     uint32_t bin_shift;    // bin = pblk >> bin_shift
     uint64_t *stats;
     void *table;           // plain paths
@@ -451,6 +453,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
             uint64_t j = uint64_t(t0 - 1 + hj);
             a.rec[slot++] = j | (uint64_t(nk - 1) << 32) | (pblk << 37);
             atomicAdd(&a.bin_count[pblk >> a.bin_shift], 1u);
+            if (a.bin_kmers) atomicAdd(&a.bin_kmers[pblk >> a.bin_shift], nk);
             smers += nk + g.k - g.s;
         }
     } else {
@@ -757,11 +760,187 @@ __global__ __launch_bounds__(512) void k_bin_query(BinArgs a, DevGeo g) {
     }
 }
 
+constexpr uint32_t kHistSmall = 4096;
+constexpr uint32_t kHistBigCap = 1u << 20;
+
+// ---- distinct-k-mer spectrum (filter.rs:407-460; fastx.rs:255-308, 392-410) ----
+// A canonical k-mer always lands in the same counter block (its minimizer is
+// strand-symmetric), so every occurrence of a key is processed by the work
+// items of one bin. Single-item bins whose k-mers fit dedup in an LDS hash set;
+// split or oversized bins dedup in a global open-addressing set. The estimate
+// of a key is the same at every occurrence, so "first wins" needs no order.
+constexpr uint32_t kSetCap = 8192;            // LDS key set (64 KiB)
+constexpr uint32_t kSetFill = kSetCap * 3 / 4;
+constexpr uint64_t kEmptyKey = ~0ull;          // keys use at most 62 bits (k <= 31)
+
+__device__ __forceinline__ uint32_t key_slot(uint64_t key, uint32_t log2cap) {
+    return uint32_t((key * 0x9E3779B97F4A7C15ull) >> (64 - log2cap));
+}
+
+// Returns true if key was not in the set (this call inserted it).
+__device__ __forceinline__ bool set_insert(unsigned long long *set, uint32_t log2cap, uint64_t key) {
+    const uint32_t mask = (1u << log2cap) - 1;
+    uint32_t h = key_slot(key, log2cap);
+    for (;;) {
+        unsigned long long cur = set[h];
+        if (cur == key) return false;
+        if (cur == kEmptyKey) {
+            unsigned long long prev = atomicCAS(&set[h], kEmptyKey, (unsigned long long)key);
+            if (prev == kEmptyKey) return true;
+            if (prev == key) return false;
+        }
+        h = (h + 1) & mask;
+    }
+}
+
+// One workgroup: total k-mers of the bins that must use the global set.
+__global__ __launch_bounds__(1024) void k_spectrum_plan(const uint32_t *bin_count, const uint32_t *bin_kmers,
+                                                        uint64_t nbins, uint32_t per_item, uint64_t *stats) {
+    __shared__ unsigned long long s_sum;
+    if (threadIdx.x == 0) s_sum = 0;
+    __syncthreads();
+    unsigned long long loc = 0;
+    for (uint64_t b = threadIdx.x; b < nbins; b += blockDim.x)
+        if (bin_count[b] > per_item || bin_kmers[b] > kSetFill) loc += bin_kmers[b];
+    atomicAdd(&s_sum, loc);
+    __syncthreads();
+    if (threadIdx.x == 0) stats[kStGlobalKmers] = s_sum;
+}
+
+struct SpecArgs {
+    BinArgs b;
+    const uint32_t *bin_kmers;
+    unsigned long long *gset;
+    uint32_t glog2;
+    unsigned long long *dense;   // histogram of estimates < dense_n
+    uint32_t dense_n;
+    uint32_t *big_keys;          // 32-bit estimates >= dense_n
+    unsigned long long *big_counts;
+    uint32_t stage;              // counters staged in LDS
+};
+
+template <int BITS>
+__device__ __forceinline__ void spec_hist_add(const SpecArgs &a, uint32_t *s_hist, uint32_t v) {
+    if (v < kHistSmall) {
+        atomicAdd(&s_hist[v], 1u);
+    } else if (v < a.dense_n) {
+        atomicAdd(&a.dense[v], 1ull);
+    } else {
+        uint32_t hsh = uint32_t(xs64(v)) & (kHistBigCap - 1);
+        for (uint32_t probe = 0; probe < kHistBigCap; ++probe) {
+            uint32_t key = a.big_keys[hsh];
+            if (key == 0) {
+                uint32_t prev = atomicCAS(&a.big_keys[hsh], 0u, v);
+                if (prev == 0 || prev == v) {
+                    atomicAdd(&a.big_counts[hsh], 1ull);
+                    return;
+                }
+                key = prev;
+            }
+            if (key == v) {
+                atomicAdd(&a.big_counts[hsh], 1ull);
+                return;
+            }
+            hsh = (hsh + 1) & (kHistBigCap - 1);
+        }
+        atomicAdd((unsigned long long *)&a.b.stats[kStOverflow], 1ull);
+    }
+}
+
+template <int BITS, int WINC>
+__global__ __launch_bounds__(512) void k_bin_spectrum(SpecArgs sa, DevGeo g) {
+    extern __shared__ __align__(16) uint8_t s_raw[];
+    __shared__ uint64_t s_item;
+    __shared__ uint32_t s_global;
+    __shared__ uint64_t s_red[3][8];
+    using T = typename Ctr<BITS>::T;
+    const BinArgs &a = sa.b;
+    const uint32_t tid = threadIdx.x, nthr = blockDim.x, lane = tid & 63, wid = tid >> 6;
+    const uint32_t bin_slots = 1u << a.bin_log;
+    const uint32_t lshift = a.bin_log - g.bse;
+    const uint32_t amask = (1u << g.bse) - 1;
+    const uint32_t tab_bytes = sa.stage ? bin_slots * (BITS / 8) : 0;
+    const uint32_t nwords = tab_bytes / 4;
+    uint32_t *s_tab = reinterpret_cast<uint32_t *>(s_raw);
+    unsigned long long *s_set = reinterpret_cast<unsigned long long *>(s_raw + tab_bytes);
+    uint32_t *s_hist = reinterpret_cast<uint32_t *>(s_raw + tab_bytes + kSetCap * 8);
+    const T *lt = reinterpret_cast<const T *>(s_raw);
+    const uint64_t nwork = a.stats[kStNWork];
+    unsigned long long *head = (unsigned long long *)&a.stats[kStWorkHead];
+    for (uint32_t i = tid; i < kHistSmall; i += nthr) s_hist[i] = 0;
+    uint64_t queried = 0, distinct = 0;
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(head, 1ull);
+        __syncthreads();
+        uint64_t item = s_item;
+        if (item >= nwork) break;
+        uint64_t wk = a.work[item];
+        uint64_t bin = wk >> 32;
+        uint32_t part = uint32_t(wk);
+        uint32_t cnt = a.bin_count[bin];
+        uint32_t rb = a.bin_off[bin] + part * a.per_item;
+        uint32_t re = min(a.bin_off[bin] + cnt, rb + a.per_item);
+        bool global = cnt > a.per_item || sa.bin_kmers[bin] > kSetFill;
+        const uint32_t *tw = reinterpret_cast<const uint32_t *>(a.table) + bin * uint64_t(bin_slots * (BITS / 8) / 4);
+        for (uint32_t wi = tid; wi < nwords; wi += nthr) s_tab[wi] = tw[wi];
+        if (!global)
+            for (uint32_t i = tid; i < kSetCap; i += nthr) s_set[i] = kEmptyKey;
+        __syncthreads();
+        const T *tbase = sa.stage ? lt : reinterpret_cast<const T *>(a.table) + (bin << a.bin_log);
+        for (uint32_t r = rb + tid; r < re; r += nthr) {
+            uint64_t rec = a.sorted[r];
+            uint64_t j0 = rec & 0xFFFFFFFFull;
+            uint32_t nk = uint32_t((rec >> 32) & 31) + 1;
+            uint32_t lbase = uint32_t((rec >> 37) & ((1u << lshift) - 1)) << g.bse;
+            uint32_t A0, A1, A2, A3;
+            load_aligned(a.pk, j0, A0, A1, A2, A3);
+            WinMin<WINC> wm;
+            for_each_smer(A0, A1, A2, A3, nk + g.k - 1, g.s, [&](uint32_t j, uint64_t smer) {
+                uint64_t h = xs64(smer);
+                uint32_t est = Ctr<BITS>::kMax;
+                for (uint32_t t = 0; t < g.nh; ++t) {
+                    uint32_t v = tbase[lbase + (uint32_t(h) & amask)];
+                    est = v < est ? v : est;
+                    if (t + 1 < g.nh) h = xs64(h);
+                }
+                uint32_t ab = wm.push(est, g.win);
+                if (j + 1 >= g.win) {
+                    uint64_t key = canonical_at(a.pk, j0 + (j + 1 - g.win), g.k);
+                    bool fresh = global ? set_insert(sa.gset, sa.glog2, key)
+                                        : set_insert(s_set, 13, key);
+                    ++queried;
+                    if (fresh) {
+                        ++distinct;
+                        spec_hist_add<BITS>(sa, s_hist, ab);
+                    }
+                }
+            });
+        }
+        __syncthreads();
+    }
+    for (uint32_t i = tid; i < kHistSmall; i += nthr)
+        if (s_hist[i]) atomicAdd(&sa.dense[i], (unsigned long long)s_hist[i]);
+    uint64_t vals[2] = {queried, distinct};
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        uint64_t v = vals[q];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) s_red[q][wid] = v;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long *st = (unsigned long long *)a.stats;
+        uint64_t t[2] = {0, 0};
+        for (int q = 0; q < 2; ++q)
+            for (uint32_t i = 0; i < nthr / 64; ++i) t[q] += s_red[q][i];
+        if (t[0]) atomicAdd(&st[kStQueried], t[0]);
+        if (t[1]) atomicAdd(&st[kStDistinct], t[1]);
+    }
+}
+
 // filter.rs:50-55, 267-275: histogram of every counter value. Values below
 // kSmall are counted in LDS; larger 8/16-bit values go straight to a dense
 // global array; larger 32-bit values go to an open-addressing table.
-constexpr uint32_t kHistSmall = 4096;
-constexpr uint32_t kHistBigCap = 1u << 20;
 
 template <int BITS>
 __global__ __launch_bounds__(256) void k_counter_hist(const void *table, uint64_t slots,
@@ -847,6 +1026,8 @@ class Workspace {
     DevBuf<uint8_t> out_dev;
     DevBuf<unsigned long long> hist_dense, hist_big_counts;
     DevBuf<uint32_t> hist_big_keys;
+    DevBuf<uint32_t> spec_bin_kmers;
+    DevBuf<unsigned long long> spec_gset;
     uint64_t *h_stats = nullptr;   // pinned
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     ~Workspace() {
@@ -985,7 +1166,7 @@ struct Launch {
 
 template <int MODE>
 Status launch_scan(Launch &L, const DeviceBatch &b, uint32_t bin_shift, uint32_t *valid_bits,
-                   void *out, bool with_records) {
+                   void *out, bool with_records, uint32_t *bin_kmers = nullptr) {
     ScanArgs a{};
     a.bases = b.d_bases;
     a.offsets = b.d_offsets;
@@ -995,6 +1176,7 @@ Status launch_scan(Launch &L, const DeviceBatch &b, uint32_t bin_shift, uint32_t
     a.valid_bits = valid_bits;
     a.rec = L.ws.rec.p;
     a.bin_count = L.ws.bin_count.p;
+    a.bin_kmers = bin_kmers;
     a.bin_shift = bin_shift;
     a.stats = L.ws.stats.p;
     a.table = L.table;
@@ -1022,7 +1204,7 @@ Status launch_scan(Launch &L, const DeviceBatch &b, uint32_t bin_shift, uint32_t
 
 // Batched front half shared by insert and query: scan, plan, scatter.
 Status batched_front(Launch &L, const DeviceBatch &b, uint32_t bin_log, uint32_t per_item,
-                     uint32_t *valid_bits, uint64_t &nbins_out) {
+                     uint32_t *valid_bits, uint64_t &nbins_out, uint32_t *bin_kmers = nullptr) {
     const Geometry &g = L.geo;
     Workspace &ws = L.ws;
     uint64_t nbins = uint64_t(1) << (g.cse - bin_log);
@@ -1040,9 +1222,10 @@ Status batched_front(Launch &L, const DeviceBatch &b, uint32_t bin_log, uint32_t
     hipStream_t st = ws.stream;
     fill(reinterpret_cast<uint32_t *>(ws.stats.p), 2 * kStCount, 0, st);
     fill(ws.bin_count.p, nbins, 0, st);
+    if (bin_kmers) fill(bin_kmers, nbins, 0, st);
     fill(ws.pk.p + (words - kPkPadWords), kPkPadWords, 0, st);
     mark(ws, 0);
-    Status s = launch_scan<kModeRecords>(L, b, bin_shift, valid_bits, nullptr, true);
+    Status s = launch_scan<kModeRecords>(L, b, bin_shift, valid_bits, nullptr, true, bin_kmers);
     if (!s.good()) return s;
     mark(ws, 1);
     hipLaunchKernelGGL(k_bin_plan, dim3(1), dim3(1024), 0, st, ws.bin_count.p, nbins, ws.bin_off.p,
@@ -1221,8 +1404,116 @@ Status Engine::query(const DeviceBatch &b, Method method, void *d_values, uint32
     return s;
 }
 
-Status Engine::spectrum(const DeviceBatch &, SpectrumResult &) const {
-    return Status{6, "internal state error: spectrum is not available"};
+Status Engine::spectrum(const DeviceBatch &b, SpectrumResult &r) const {
+    r = SpectrumResult{};
+    Status s = check_batch(geo_, b);
+    if (!s.good()) return s;
+    if (b.n_bases < geo_.k || b.n_reads == 0) return Status::ok();
+    if (geo_.nb > (uint64_t(1) << 27))
+        return Status{6, "internal state error: the GPU spectrum supports at most 2^27 counter blocks"};
+    SCB_HIP(hipSetDevice(device_));
+    auto wsp = acquire();
+    if (!wsp) return Status{8, "device error: cannot create a workspace"};
+    Workspace &ws = *wsp;
+    Launch L{geo_, dev_geo(geo_), ws, d_table_, device_};
+    hipStream_t st = ws.stream;
+    const uint32_t bin_log = geo_.bse <= 14 ? std::min<uint32_t>(14, geo_.cse) : geo_.bse;
+    const uint32_t stage = (geo_.bse <= 14 && (uint64_t(1) << bin_log) * geo_.bytes >= 4) ? 1 : 0;
+    const uint32_t per_item = tune_.max_records_per_item;
+    uint64_t nbins = uint64_t(1) << (geo_.cse - bin_log);
+    const uint32_t dense_n = geo_.bits == 8 ? 256 : (geo_.bits == 16 ? 65536 : kHistSmall);
+    do {
+        hipError_t e;
+        if ((e = ws.spec_bin_kmers.reserve(nbins)) != hipSuccess ||
+            (e = ws.hist_dense.reserve(dense_n)) != hipSuccess ||
+            (e = ws.hist_big_keys.reserve(geo_.bits == 32 ? kHistBigCap : 1)) != hipSuccess ||
+            (e = ws.hist_big_counts.reserve(geo_.bits == 32 ? kHistBigCap : 1)) != hipSuccess) {
+            s = Status{8, std::string("device error: ") + hipGetErrorString(e)};
+            break;
+        }
+        uint64_t nb_unused = 0;
+        s = batched_front(L, b, bin_log, per_item, nullptr, nb_unused, ws.spec_bin_kmers.p);
+        if (!s.good()) break;
+        hipLaunchKernelGGL(k_spectrum_plan, dim3(1), dim3(1024), 0, st, ws.bin_count.p, ws.spec_bin_kmers.p,
+                           nbins, per_item, ws.stats.p);
+        if ((s = finish(ws)).code) break;
+        uint64_t gk = ws.h_stats[kStGlobalKmers];
+        uint32_t glog2 = 10;
+        while ((uint64_t(1) << glog2) < 2 * gk + 2) ++glog2;
+        if ((e = ws.spec_gset.reserve(uint64_t(1) << glog2)) != hipSuccess) {
+            s = Status{8, std::string("device error: ") + hipGetErrorString(e)};
+            break;
+        }
+        fill(reinterpret_cast<uint32_t *>(ws.spec_gset.p), (uint64_t(1) << glog2) * 2, 0xFFFFFFFFu, st);
+        fill(reinterpret_cast<uint32_t *>(ws.hist_dense.p), uint64_t(dense_n) * 2, 0, st);
+        if (geo_.bits == 32) {
+            fill(ws.hist_big_keys.p, kHistBigCap, 0, st);
+            fill(reinterpret_cast<uint32_t *>(ws.hist_big_counts.p), 2 * kHistBigCap, 0, st);
+        }
+        SpecArgs sa{};
+        sa.b.sorted = ws.sorted.p;
+        sa.b.bin_off = ws.bin_off.p;
+        sa.b.bin_count = ws.bin_count.p;
+        sa.b.work = ws.work.p;
+        sa.b.stats = ws.stats.p;
+        sa.b.pk = ws.pk.p;
+        sa.b.table = d_table_;
+        sa.b.bin_log = bin_log;
+        sa.b.per_item = per_item;
+        sa.bin_kmers = ws.spec_bin_kmers.p;
+        sa.gset = ws.spec_gset.p;
+        sa.glog2 = glog2;
+        sa.dense = ws.hist_dense.p;
+        sa.dense_n = dense_n;
+        sa.big_keys = ws.hist_big_keys.p;
+        sa.big_counts = ws.hist_big_counts.p;
+        sa.stage = stage;
+        size_t lds = (stage ? (size_t(1) << bin_log) * geo_.bytes : 0) + kSetCap * 8 + kHistSmall * 4;
+        const int thr = 512;
+        bool wide = geo_.win > 8;
+#define SCB_SPEC(BITS, WC)                                                                      \
+    do {                                                                                        \
+        s = set_lds(k_bin_spectrum<BITS, WC>, lds);                                             \
+        if (!s.good()) break;                                                                   \
+        int grid = persistent_grid(k_bin_spectrum<BITS, WC>, thr, lds, device_);                \
+        hipLaunchKernelGGL((k_bin_spectrum<BITS, WC>), dim3(grid), dim3(thr), lds, st, sa, L.dg); \
+    } while (0)
+        switch (geo_.bits) {
+            case 8: if (wide) SCB_SPEC(8, 32); else SCB_SPEC(8, 8); break;
+            case 16: if (wide) SCB_SPEC(16, 32); else SCB_SPEC(16, 8); break;
+            default: if (wide) SCB_SPEC(32, 32); else SCB_SPEC(32, 8); break;
+        }
+#undef SCB_SPEC
+        if (!s.good()) break;
+        if ((e = hipGetLastError()) != hipSuccess) {
+            s = Status{8, std::string("device error: k_bin_spectrum: ") + hipGetErrorString(e)};
+            break;
+        }
+        std::vector<unsigned long long> dense(dense_n);
+        std::vector<uint32_t> keys;
+        std::vector<unsigned long long> cnts;
+        (void)hipMemcpyAsync(dense.data(), ws.hist_dense.p, uint64_t(dense_n) * 8, hipMemcpyDeviceToHost, st);
+        if (geo_.bits == 32) {
+            keys.resize(kHistBigCap);
+            cnts.resize(kHistBigCap);
+            (void)hipMemcpyAsync(keys.data(), ws.hist_big_keys.p, kHistBigCap * 4, hipMemcpyDeviceToHost, st);
+            (void)hipMemcpyAsync(cnts.data(), ws.hist_big_counts.p, kHistBigCap * 8, hipMemcpyDeviceToHost, st);
+        }
+        if ((s = finish(ws)).code) break;
+        if (ws.h_stats[kStOverflow]) {
+            s = Status{8, "device error: spectrum has more than 2^20 distinct large estimates"};
+            break;
+        }
+        for (uint64_t v = 0; v < dense_n; ++v)
+            if (dense[v]) r.rows.emplace_back(v, dense[v]);
+        for (uint32_t i = 0; i < keys.size(); ++i)
+            if (keys[i]) r.rows.emplace_back(keys[i], cnts[i]);
+        std::sort(r.rows.begin(), r.rows.end());
+        r.queried = ws.h_stats[kStQueried];
+        r.distinct = ws.h_stats[kStDistinct];
+    } while (false);
+    release(std::move(wsp));
+    return s;
 }
 
 Status Engine::counter_histogram(std::vector<std::pair<uint64_t, uint64_t>> &rows) const {
@@ -1395,8 +1686,23 @@ Status Engine::query_host(const uint8_t *bases, const uint64_t *offsets, uint64_
     });
 }
 
-Status Engine::spectrum_host(const uint8_t *, const uint64_t *, uint64_t, SpectrumResult &) const {
-    return Status{6, "internal state error: spectrum is not available"};
+Status Engine::spectrum_host(const uint8_t *bases, const uint64_t *offsets, uint64_t n,
+                             SpectrumResult &r) const {
+    r = SpectrumResult{};
+    SCB_HIP(hipSetDevice(device_));
+    uint64_t nb = offsets[n] - offsets[0];
+    if (nb >= kMaxBatchBases)
+        return Status{6, "internal state error: spectrum query exceeds 2^32 bases in one call"};
+    DevBuf<uint8_t> db;
+    DevBuf<uint64_t> doff;
+    SCB_HIP(db.reserve(nb + 4));
+    SCB_HIP(doff.reserve(n + 1));
+    std::vector<uint64_t> loc(n + 1);
+    for (uint64_t i = 0; i <= n; ++i) loc[i] = offsets[i] - offsets[0];
+    SCB_HIP(hipMemcpy(db.p, bases + offsets[0], nb, hipMemcpyHostToDevice));
+    SCB_HIP(hipMemcpy(doff.p, loc.data(), loc.size() * 8, hipMemcpyHostToDevice));
+    DeviceBatch b{db.p, doff.p, n, nb};
+    return spectrum(b, r);
 }
 
 StageTimes last_stage_times() { return g_stage_times; }
