@@ -109,8 +109,6 @@ class Engine {
     Tuning tune_;
     void *d_table_ = nullptr;
     int device_ = 0;
-    mutable std::mutex pool_mu_;
-    mutable std::vector<std::unique_ptr<Workspace>> pool_;
     std::unique_ptr<Workspace> acquire() const;
     void release(std::unique_ptr<Workspace> ws) const;
 };

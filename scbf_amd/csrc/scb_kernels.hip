@@ -18,7 +18,8 @@
 //               masks, canonical ntHash of every m-mer, minimizer of every k-mer
 //               window, super-k-mer cut, block hash -> rec + bin histogram
 //   k_bin_plan  one workgroup: bin offsets + work list (hot bins split)
-//   k_scatter   rec -> sorted (counting sort by bin)
+//   k_partition rec -> records grouped by bin (tile-aggregated counting sort,
+//               one or two passes)
 //   k_bin_insert / k_bin_query  persistent work queue, one bin per item, counters
 //               staged in LDS, written back once (insert) or read once (query)
 // Plain pipeline: k_scan alone, one thread per super-k-mer, global atomics/loads.
@@ -542,7 +543,8 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
 __global__ __launch_bounds__(1024) void k_bin_plan(const uint32_t *bin_count, uint64_t nbins,
                                                    uint32_t *bin_off, uint32_t *cursor,
                                                    uint64_t *work, uint32_t per_item,
-                                                   uint64_t *stats) {
+                                                   uint64_t *stats, uint32_t *ccursor,
+                                                   uint32_t fine_bits) {
     __shared__ uint64_t s_rec[1024], s_items[1024];
     const int tid = threadIdx.x;
     uint64_t chunk = (nbins + 1023) / 1024;
@@ -566,6 +568,7 @@ __global__ __launch_bounds__(1024) void k_bin_plan(const uint32_t *bin_count, ui
         uint32_t c = bin_count[b];
         bin_off[b] = uint32_t(ro);
         cursor[b] = uint32_t(ro);
+        if (ccursor && (b & ((1u << fine_bits) - 1)) == 0) ccursor[b >> fine_bits] = uint32_t(ro);
         uint32_t parts = (c + per_item - 1) / per_item;
         for (uint32_t p = 0; p < parts; ++p) work[io++] = (b << 32) | p;
         ro += c;
@@ -576,15 +579,69 @@ __global__ __launch_bounds__(1024) void k_bin_plan(const uint32_t *bin_count, ui
     }
 }
 
-__global__ void k_scatter(const uint64_t *rec, const uint64_t *stats, uint32_t *cursor,
-                          uint64_t *sorted, uint32_t bin_shift) {
-    uint64_t n = stats[kStRecords];
-    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-         i += uint64_t(gridDim.x) * blockDim.x) {
-        uint64_t r = rec[i];
-        uint64_t bin = (r >> 37) >> bin_shift;
-        uint32_t slot = atomicAdd(&cursor[bin], 1u);
-        sorted[slot] = r;
+// Counting sort of the records by bin without one global atomic per record:
+// tiles of kPartTile records rank themselves in an LDS histogram and reserve
+// one contiguous range per key with a single global atomicAdd. When there are
+// more bins than LDS entries, a coarse pass (bin >> fine_bits) runs first; the
+// fine pass then sees tiles whose records share a few coarse buckets.
+constexpr int kPartTile = 4096;
+constexpr int kPartThreads = 256;
+constexpr int kPartItems = kPartTile / kPartThreads;
+constexpr uint32_t kPartRange = 2048;
+
+// key(rec) = (bin >> key_shift) - base, where base is the key range start of
+// the tile (fine pass over coarse-ordered input) or 0.
+__global__ __launch_bounds__(kPartThreads) void k_partition(const uint64_t *src, const uint64_t *stats,
+                                                            uint32_t *cursor, uint64_t *dst,
+                                                            uint32_t bin_shift, uint32_t key_shift,
+                                                            uint32_t fine_bits, uint32_t tile_based) {
+    __shared__ uint32_t s_hist[kPartRange];
+    __shared__ uint32_t s_lo;
+    const uint64_t n = stats[kStRecords];
+    const uint64_t ntiles = (n + kPartTile - 1) / kPartTile;
+    const uint32_t tid = threadIdx.x;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint64_t t0 = tile * kPartTile;
+        for (uint32_t i = tid; i < kPartRange; i += kPartThreads) s_hist[i] = 0;
+        if (tid == 0) {
+            uint32_t lo = 0;
+            if (tile_based) {
+                uint64_t b0 = (src[t0] >> 37) >> bin_shift;
+                lo = uint32_t((b0 >> fine_bits) << fine_bits);
+            }
+            s_lo = lo;
+        }
+        __syncthreads();
+        const uint32_t lo = s_lo;
+        uint64_t r[kPartItems];
+        uint32_t key[kPartItems], rank[kPartItems];
+#pragma unroll
+        for (int j = 0; j < kPartItems; ++j) {
+            uint64_t i = t0 + uint64_t(j) * kPartThreads + tid;
+            key[j] = 0xFFFFFFFFu;
+            if (i < n) {
+                r[j] = src[i];
+                uint32_t bin = uint32_t((r[j] >> 37) >> bin_shift);
+                uint32_t kk = (bin >> key_shift) - lo;
+                if (kk < kPartRange) {
+                    key[j] = kk;
+                    rank[j] = atomicAdd(&s_hist[kk], 1u);
+                } else {
+                    // far outside the tile's range: reserve one slot directly
+                    dst[atomicAdd(&cursor[bin >> key_shift], 1u)] = r[j];
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t kk = tid; kk < kPartRange; kk += kPartThreads) {
+            uint32_t c = s_hist[kk];
+            if (c) s_hist[kk] = atomicAdd(&cursor[kk + lo], c);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kPartItems; ++j)
+            if (key[j] != 0xFFFFFFFFu) dst[s_hist[key[j]] + rank[j]] = r[j];
+        __syncthreads();
     }
 }
 
@@ -1020,7 +1077,8 @@ class Workspace {
     hipStream_t stream = nullptr;
     DevBuf<uint32_t> pk, valid;
     DevBuf<uint64_t> rec, sorted, work, stats;
-    DevBuf<uint32_t> bin_count, bin_off, cursor;
+    DevBuf<uint32_t> bin_count, bin_off, cursor, ccursor;
+    uint64_t *final_sorted = nullptr;  // records grouped by bin after the partition passes
     DevBuf<uint8_t> h_bases_dev;   // staging for host batches
     DevBuf<uint64_t> h_offsets_dev;
     DevBuf<uint8_t> out_dev;
@@ -1118,17 +1176,34 @@ Status Engine::create(const Geometry &g, std::unique_ptr<Engine> &out) {
 }
 
 Engine::~Engine() {
-    pool_.clear();
     if (d_table_) (void)hipFree(d_table_);
 }
 
+// Scratch workspaces are pooled per device for the whole process, so a new
+// builder or index reuses the batch buffers of the previous one instead of
+// allocating them again. The pool is never destroyed (the HIP runtime may
+// already be gone when static destructors run).
+namespace {
+struct WorkspacePool {
+    std::mutex mu;
+    std::vector<std::pair<int, std::unique_ptr<Workspace>>> free;
+};
+WorkspacePool &workspace_pool() {
+    static WorkspacePool *pool = new WorkspacePool;
+    return *pool;
+}
+}  // namespace
+
 std::unique_ptr<Workspace> Engine::acquire() const {
+    WorkspacePool &pool = workspace_pool();
     {
-        std::lock_guard<std::mutex> lk(pool_mu_);
-        if (!pool_.empty()) {
-            auto ws = std::move(pool_.back());
-            pool_.pop_back();
-            return ws;
+        std::lock_guard<std::mutex> lk(pool.mu);
+        for (size_t i = pool.free.size(); i-- > 0;) {
+            if (pool.free[i].first == device_) {
+                auto ws = std::move(pool.free[i].second);
+                pool.free.erase(pool.free.begin() + i);
+                return ws;
+            }
         }
     }
     std::unique_ptr<Workspace> ws;
@@ -1138,8 +1213,9 @@ std::unique_ptr<Workspace> Engine::acquire() const {
 
 void Engine::release(std::unique_ptr<Workspace> ws) const {
     if (!ws) return;
-    std::lock_guard<std::mutex> lk(pool_mu_);
-    pool_.push_back(std::move(ws));
+    WorkspacePool &pool = workspace_pool();
+    std::lock_guard<std::mutex> lk(pool.mu);
+    pool.free.emplace_back(device_, std::move(ws));
 }
 
 Status Engine::download(void *host, uint64_t bytes) const {
@@ -1228,12 +1304,30 @@ Status batched_front(Launch &L, const DeviceBatch &b, uint32_t bin_log, uint32_t
     Status s = launch_scan<kModeRecords>(L, b, bin_shift, valid_bits, nullptr, true, bin_kmers);
     if (!s.good()) return s;
     mark(ws, 1);
+    // two partition passes when the bins outnumber the LDS histogram
+    const bool two_pass = nbins > kPartRange;
+    uint32_t fine_bits = 0;
+    if (two_pass) {
+        uint32_t lb = uint32_t(__builtin_ctzll(nbins));
+        fine_bits = std::max<uint32_t>(7, lb > 11 ? lb - 11 : 0);
+        SCB_HIP(ws.ccursor.reserve(nbins >> fine_bits));
+    }
     hipLaunchKernelGGL(k_bin_plan, dim3(1), dim3(1024), 0, st, ws.bin_count.p, nbins, ws.bin_off.p,
-                       ws.cursor.p, ws.work.p, per_item, ws.stats.p);
+                       ws.cursor.p, ws.work.p, per_item, ws.stats.p, two_pass ? ws.ccursor.p : nullptr,
+                       fine_bits);
     SCB_HIP(hipGetLastError());
-    int sgrid = cu_count(L.device) * 8;
-    hipLaunchKernelGGL(k_scatter, dim3(sgrid), dim3(256), 0, st, ws.rec.p, ws.stats.p, ws.cursor.p,
-                       ws.sorted.p, bin_shift);
+    int pgrid = cu_count(L.device) * 8;
+    if (two_pass) {
+        hipLaunchKernelGGL(k_partition, dim3(pgrid), dim3(kPartThreads), 0, st, ws.rec.p, ws.stats.p,
+                           ws.ccursor.p, ws.sorted.p, bin_shift, fine_bits, 0u, 0u);
+        hipLaunchKernelGGL(k_partition, dim3(pgrid), dim3(kPartThreads), 0, st, ws.sorted.p, ws.stats.p,
+                           ws.cursor.p, ws.rec.p, bin_shift, 0u, fine_bits, 1u);
+        ws.final_sorted = ws.rec.p;
+    } else {
+        hipLaunchKernelGGL(k_partition, dim3(pgrid), dim3(kPartThreads), 0, st, ws.rec.p, ws.stats.p,
+                           ws.cursor.p, ws.sorted.p, bin_shift, 0u, 0u, 0u);
+        ws.final_sorted = ws.sorted.p;
+    }
     SCB_HIP(hipGetLastError());
     mark(ws, 2);
     nbins_out = nbins;
@@ -1290,7 +1384,7 @@ Status Engine::insert(const DeviceBatch &b, Method method, InsertResult &r) {
         s = batched_front(L, b, bin_log, tune_.max_records_per_item, nullptr, nbins);
         if (s.good()) {
             BinArgs a{};
-            a.sorted = ws->sorted.p;
+            a.sorted = ws->final_sorted;
             a.bin_off = ws->bin_off.p;
             a.bin_count = ws->bin_count.p;
             a.work = ws->work.p;
@@ -1359,7 +1453,7 @@ Status Engine::query(const DeviceBatch &b, Method method, void *d_values, uint32
         s = batched_front(L, b, bin_log, tune_.max_records_per_item, d_valid_bits, nbins);
         if (s.good()) {
             BinArgs a{};
-            a.sorted = ws->sorted.p;
+            a.sorted = ws->final_sorted;
             a.bin_off = ws->bin_off.p;
             a.bin_count = ws->bin_count.p;
             a.work = ws->work.p;
@@ -1451,7 +1545,7 @@ Status Engine::spectrum(const DeviceBatch &b, SpectrumResult &r) const {
             fill(reinterpret_cast<uint32_t *>(ws.hist_big_counts.p), 2 * kHistBigCap, 0, st);
         }
         SpecArgs sa{};
-        sa.b.sorted = ws.sorted.p;
+        sa.b.sorted = ws.final_sorted;
         sa.b.bin_off = ws.bin_off.p;
         sa.b.bin_count = ws.bin_count.p;
         sa.b.work = ws.work.p;
