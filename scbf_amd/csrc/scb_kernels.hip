@@ -261,19 +261,24 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
 // ----------------------------------------------------------------------------
 // Scan tile (front end shared by both pipelines)
 // ----------------------------------------------------------------------------
+constexpr int kSeg = 64;         // record segments (one reservation counter each)
+constexpr int kStSpread = 16;    // stats[kStSpread + slot * kSeg + (tile & 63)]
+enum SpreadSlot { kSpValid = 0, kSpSmers = 1, kSpQueried = 2, kSpPositive = 3, kSpChecksum = 4, kSpCount = 5 };
+constexpr int kStatsLen = kStSpread + kSpCount * kSeg;
+
 struct ScanArgs {
     const uint8_t *bases;
     const uint64_t *offsets;
     uint64_t n_reads, n_bases;
     uint32_t *pk;          // packed codes out (batched) or null
     uint32_t *valid_bits;  // per-window validity out or null
-    uint64_t *rec;         // records out (batched)
-    uint32_t *bin_count;   // records per bin (batched)
-    uint32_t *bin_kmers;   // k-mers per bin (spectrum) or null
-    uint32_t bin_shift;    // bin = pblk >> bin_shift
+    uint64_t *rec;         // records out (batched): kSeg segments of seg_cap records
+    uint32_t *seg_count;   // records written per segment
+    uint64_t seg_cap;
     uint64_t *stats;
     void *table;           // plain paths
     void *out;             // plain query abundances (position-indexed) or null
+    uint32_t aligned16;    // bases pointer is 16-byte aligned
 };
 
 enum ScanMode { kModeRecords = 0, kModePlainInsert = 1, kModePlainQuery = 2 };
@@ -284,26 +289,39 @@ enum ScanMode { kModeRecords = 0, kModePlainInsert = 1, kModePlainQuery = 2 };
 // over its w m-mers (simd-minimizers restatement, third-party), and a
 // super-k-mer starts where the minimizer position changes or a run starts
 // (filter.rs:608-620, 210-216).
+//
+// Phases: (1) 16 bases per thread from one 16-byte load -> 2-bit word and
+// non-ACGT / T-G masks; (2) ntHash as XOR prefixes G(i) over the span, so the
+// hash of the m-mer at i is rotl(Gf(i+m)^Gf(i), i+m-1) + rotr(Gr(i+m)^Gr(i), i);
+// (3) van Herk/Gil-Werman block prefix/suffix minima of (key16 << 16 | pos)
+// for the leftmost and (key16 << 16 | ~pos) for the rightmost argmin; (4) each
+// window takes the min of one suffix and one prefix value.
 template <int MODE, int BITS, int WINC>
 __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
     __shared__ uint32_t s_bad[kSpanWords], s_tg[kSpanWords], s_brk[kSpanWords];
     __shared__ uint32_t s_pk[kPkWords];
-    __shared__ __align__(16) uint8_t s_code[kSpan];
-    __shared__ uint32_t s_hash[kHashN];
+    __shared__ uint32_t s_big[5 * kHashN];  // H | Gf, Gr  then  H | PL, SL, PR, SR
     __shared__ uint16_t s_mpos[kWinN];
     __shared__ uint32_t s_cnt[kTileThreads];
+    __shared__ uint32_t s_wx[2][kTileThreads / 64];
     __shared__ uint64_t s_red[3][kTileThreads / 64];
     __shared__ uint64_t s_r0, s_base;
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int nwave = kTileThreads / 64;
+    constexpr int nwave = kTileThreads / 64;
+    constexpr int kChunks = kSpan / 16;
     const int64_t t0 = int64_t(blockIdx.x) * kTile;
     const int64_t LB = t0 - kLeft;
     const int64_t N = int64_t(a.n_bases);
     const int k = int(g.k), m = int(g.m), w = int(g.w);
+    uint16_t *s_bad16 = reinterpret_cast<uint16_t *>(s_bad);
+    uint16_t *s_tg16 = reinterpret_cast<uint16_t *>(s_tg);
+    uint32_t *s_H = s_big;
+    uint32_t *s_Gf = s_big + kHashN, *s_Gr = s_big + kHashN + kSpan;
+    uint32_t *s_PL = s_big + kHashN, *s_SL = s_big + 2 * kHashN;
+    uint32_t *s_PR = s_big + 3 * kHashN, *s_SR = s_big + 4 * kHashN;
 
-    for (int i = tid; i < kSpanWords; i += kTileThreads) s_brk[i] = 0, s_bad[i] = ~0u, s_tg[i] = 0;
-    for (int i = tid; i < kPkWords; i += kTileThreads) s_pk[i] = 0;
+    for (int i = tid; i < kSpanWords; i += kTileThreads) s_brk[i] = 0;
     if (tid == 0) {
         // first record whose start lies after LB
         uint64_t lo = 0, hi = a.n_reads;
@@ -313,89 +331,134 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
         }
         s_r0 = lo;
     }
-    __syncthreads();
 
-    // Bases -> codes, non-ACGT and T/G masks (one 64-base chunk per wave step).
-    for (int c = wid; c < kSpan / 64; c += nwave) {
-        int li = c * 64 + lane;
-        int64_t gi = LB + li;
-        uint32_t ch = (gi >= 0 && gi < N) ? a.bases[gi] : uint32_t('N');
-        uint32_t code = (ch >> 1) & 3;
-        bool ok = is_acgt(ch);
-        uint64_t badm = __ballot(!ok), tgm = __ballot(ok && (code >> 1));
-        s_code[li] = uint8_t(code);
-        if (lane == 0) {
-            s_bad[2 * c] = uint32_t(badm), s_bad[2 * c + 1] = uint32_t(badm >> 32);
-            s_tg[2 * c] = uint32_t(tgm), s_tg[2 * c + 1] = uint32_t(tgm >> 32);
+    // (1) 16 bases per thread.
+    uint32_t cw = 0;
+    if (tid < kChunks) {
+        const int64_t g0 = LB + 16 * tid;
+        uint32_t wv[4];
+        if (a.aligned16 && g0 >= 0 && g0 + 16 <= N) {
+            uint4 v = *reinterpret_cast<const uint4 *>(a.bases + g0);
+            wv[0] = v.x, wv[1] = v.y, wv[2] = v.z, wv[3] = v.w;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                uint32_t x = 0;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    int64_t gi = g0 + 4 * q + t;
+                    uint32_t ch = (gi >= 0 && gi < N) ? a.bases[gi] : uint32_t('N');
+                    x |= ch << (8 * t);
+                }
+                wv[q] = x;
+            }
+        }
+        uint32_t badm = 0, tgm = 0;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            uint32_t ch = (wv[t >> 2] >> (8 * (t & 3))) & 0xFF;
+            uint32_t code = (ch >> 1) & 3;
+            bool ok = is_acgt(ch);
+            cw |= code << (2 * t);
+            badm |= uint32_t(!ok) << t;
+            tgm |= uint32_t(ok && (code >> 1)) << t;
+        }
+        s_pk[tid] = cw;
+        s_bad16[tid] = uint16_t(badm);
+        s_tg16[tid] = uint16_t(tgm);
+        if (MODE == kModeRecords && tid >= kLeft / 16 && tid < kLeft / 16 + kTile / 16)
+            a.pk[(LB >> 4) + tid] = cw;  // LB is a multiple of 16
+    } else if (tid < kPkWords) {
+        s_pk[tid] = 0;
+        if (tid < 2 * kSpanWords) s_bad16[tid] = 0xFFFF, s_tg16[tid] = 0;
+    }
+
+    // (2) ntHash XOR prefixes: local exclusive prefixes of this thread's 16 terms.
+    uint32_t xf = 0, xr = 0, lf[16], lr[16];
+    if (tid < kChunks) {
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            uint32_t c = (cw >> (2 * t)) & 3;
+            uint32_t j = uint32_t(16 * tid + t);
+            lf[t] = xf, lr[t] = xr;
+            xf ^= rotl32(nt_seed(c), (32 - (j & 31)) & 31);
+            xr ^= rotl32(nt_seed(c ^ 2), j & 31);
         }
     }
-    // Record starts inside the span mark read boundaries.
+    uint32_t sf = xf, sr = xr;  // inclusive XOR scan of the per-thread totals
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t yf = __shfl_up(sf, off, 64), yr = __shfl_up(sr, off, 64);
+        if (lane >= off) sf ^= yf, sr ^= yr;
+    }
+    if (lane == 63) s_wx[0][wid] = sf, s_wx[1][wid] = sr;
+    __syncthreads();
+    // record starts inside the span mark read boundaries
     for (uint64_t r = s_r0 + tid; r < a.n_reads; r += kTileThreads) {
         int64_t o = int64_t(a.offsets[r]);
         if (o >= LB + kSpan) break;
         int li = int(o - LB);
         atomicOr(&s_brk[li >> 5], 1u << (li & 31));
     }
-    __syncthreads();
-
-    // Packed words for the span (and the batch-wide packed array).
-    for (int wi = tid; wi < kSpan / 16; wi += kTileThreads) {
-        const uint32_t *cw = reinterpret_cast<const uint32_t *>(s_code + wi * 16);
-        uint32_t v = 0;
+    if (tid < kChunks) {
+        uint32_t pf = 0, pr = 0;
+        for (int i = 0; i < wid; ++i) pf ^= s_wx[0][i], pr ^= s_wx[1][i];
+        uint32_t ef = pf ^ sf ^ xf, er = pr ^ sr ^ xr;  // exclusive prefix of this thread
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            uint32_t x = cw[q];
-            uint32_t b8 = (x & 3) | ((x >> 6) & 0xC) | ((x >> 12) & 0x30) | ((x >> 18) & 0xC0);
-            v |= b8 << (8 * q);
-        }
-        s_pk[wi] = v;
-        if (MODE == kModeRecords) {
-            int64_t gw = (LB >> 4) + wi;  // LB is a multiple of 16
-            if (wi >= kLeft / 16 && wi < kLeft / 16 + kTile / 16) a.pk[gw] = v;
+        for (int t = 0; t < 16; ++t) {
+            s_Gf[16 * tid + t] = ef ^ lf[t];
+            s_Gr[16 * tid + t] = er ^ lr[t];
         }
     }
+    __syncthreads();
 
-    // Canonical ntHash of every m-mer at positions p = t0-1+hp, hp in [0, NH).
+    // Canonical hash of every m-mer at positions p = t0-1+hp, hp in [0, NH).
+    const int NH = kTile + 2 * w - 1;
+    for (int hp = tid; hp < NH; hp += kTileThreads) {
+        int li = hp + kLeft - 1;
+        uint32_t fw = rotl32(s_Gf[li + m] ^ s_Gf[li], uint32_t(li + m - 1) & 31);
+        uint32_t rc = rotr32(s_Gr[li + m] ^ s_Gr[li], uint32_t(li) & 31);
+        s_H[hp] = fw + rc;
+    }
+    __syncthreads();
+
+    // (3) block prefix/suffix minima (blocks of w hash positions).
     {
-        const int NH = kTile + 2 * w - 1;
-        const int per = (NH + kTileThreads - 1) / kTileThreads;
-        int h0 = tid * per, h1 = min(h0 + per, NH);
-        if (h0 < h1) {
-            uint32_t fw = 0, rc = 0;
-            int li0 = h0 + kLeft - 1;
-            for (int i = 0; i < m; ++i) {
-                uint32_t c = s_code[li0 + i];
-                fw ^= rotl32(nt_seed(c), uint32_t(m - 1 - i));
-                rc ^= rotl32(nt_seed(c ^ 2), uint32_t(i));
+        const int nblk = (NH + w - 1) / w;
+        for (int b = tid; b < nblk; b += kTileThreads) {
+            const int h0 = b * w, h1 = min(h0 + w, NH);
+            uint32_t cl = ~0u, cr = ~0u;
+            for (int hp = h0; hp < h1; ++hp) {
+                uint32_t key = s_H[hp] & 0xFFFF0000u;
+                cl = min(cl, key | uint32_t(hp));
+                cr = min(cr, key | uint32_t(0xFFFF - hp));
+                s_PL[hp] = cl, s_PR[hp] = cr;
             }
-            for (int hp = h0;; ++hp) {
-                s_hash[hp] = fw + rc;
-                if (hp + 1 >= h1) break;
-                int li = hp + kLeft - 1;
-                uint32_t co = s_code[li], ci = s_code[li + m];
-                fw = rotl32(fw, 1) ^ rotl32(nt_seed(co), uint32_t(m)) ^ nt_seed(ci);
-                rc = rotr32(rc, 1) ^ rotr32(nt_seed(co ^ 2), 1) ^ rotl32(nt_seed(ci ^ 2), uint32_t(m - 1));
+            cl = cr = ~0u;
+            for (int hp = h1 - 1; hp >= h0; --hp) {
+                uint32_t key = s_H[hp] & 0xFFFF0000u;
+                cl = min(cl, key | uint32_t(hp));
+                cr = min(cr, key | uint32_t(0xFFFF - hp));
+                s_SL[hp] = cl, s_SR[hp] = cr;
             }
         }
     }
     __syncthreads();
 
-    // Minimizer of every window j = t0-1+hj, hj in [0, kTile + w).
+    // (4) minimizer position of every window j = t0-1+hj, hj in [0, kTile + w).
     for (int hj = tid; hj < kTile + w; hj += kTileThreads) {
         int li = hj + kLeft - 1;
         uint32_t bad = get_bits(s_bad, li, k) | get_bits(s_brk, li + 1, k - 1);
         uint16_t pos = kNoWin;
         if (!bad) {
             uint32_t tg = __popc(get_bits(s_tg, li, k));
-            bool left = 2 * int(tg) > k;
-            uint32_t best = 0xFFFFFFFFu;
-            int bp = hj;
-            for (int i = 0; i < w; ++i) {
-                uint32_t key = s_hash[hj + i] >> 16;
-                bool take = left ? (key < best) : (key <= best);
-                if (take) best = key, bp = hj + i;
+            int e = hj + w - 1;
+            if (2 * int(tg) > k) {  // leftmost tie
+                uint32_t v = min(s_SL[hj], s_PL[e]);
+                pos = uint16_t(v & 0xFFFF);
+            } else {  // rightmost tie
+                uint32_t v = min(s_SR[hj], s_PR[e]);
+                pos = uint16_t(0xFFFF - (v & 0xFFFF));
             }
-            pos = uint16_t(bp);
         }
         s_mpos[hj] = pos;
     }
@@ -426,19 +489,20 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
     }
 
     uint32_t my = __popc(starts);
-    uint64_t smers = 0, queried = 0, positive = 0, checksum = 0;
+    uint64_t smers = 0, positive = 0, checksum = 0;
     if (MODE == kModeRecords) {
         s_cnt[tid] = my;
         __syncthreads();
-        // exclusive scan over 256 counts (Hillis-Steele)
-        for (int d = 1; d < kTileThreads; d <<= 1) {
+        for (int d = 1; d < kTileThreads; d <<= 1) {  // inclusive scan (Hillis-Steele)
             uint32_t v = tid >= d ? s_cnt[tid - d] : 0;
             __syncthreads();
             s_cnt[tid] += v;
             __syncthreads();
         }
-        if (tid == kTileThreads - 1) s_base = atomicAdd((unsigned long long *)&a.stats[kStRecords],
-                                                        (unsigned long long)s_cnt[tid]);
+        if (tid == kTileThreads - 1) {
+            uint32_t seg = blockIdx.x & (kSeg - 1);
+            s_base = seg * a.seg_cap + atomicAdd(&a.seg_count[seg], s_cnt[tid]);
+        }
         __syncthreads();
         uint64_t slot = s_base + s_cnt[tid] - my;
         for (uint32_t bits = starts; bits; bits &= bits - 1) {
@@ -453,8 +517,6 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
             uint64_t pblk = ((block & g.shard_mask) << g.sub_log) | (block >> g.shard_shift);
             uint64_t j = uint64_t(t0 - 1 + hj);
             a.rec[slot++] = j | (uint64_t(nk - 1) << 32) | (pblk << 37);
-            atomicAdd(&a.bin_count[pblk >> a.bin_shift], 1u);
-            if (a.bin_kmers) atomicAdd(&a.bin_kmers[pblk >> a.bin_shift], nk);
             smers += nk + g.k - g.s;
         }
     } else {
@@ -501,7 +563,6 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
                         uint64_t q = j + 1 - g.win;
                         if (a.out) reinterpret_cast<typename Ctr<BITS>::T *>(a.out)[j0 + q] =
                             typename Ctr<BITS>::T(ab);
-                        ++queried;
                         positive += ab > 0;
                         checksum += ab;
                     }
@@ -510,7 +571,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
         }
     }
 
-    // Block reductions of the statistics (one atomic per value per tile).
+    // Block reductions of the statistics, added into one of kSeg spread slots.
     uint64_t vals[3] = {nvalid, MODE == kModePlainQuery ? positive : smers,
                         MODE == kModePlainQuery ? checksum : 0};
 #pragma unroll
@@ -524,16 +585,15 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
         uint64_t t[3] = {0, 0, 0};
         for (int q = 0; q < 3; ++q)
             for (int i = 0; i < nwave; ++i) t[q] += s_red[q][i];
-        unsigned long long *st = (unsigned long long *)a.stats;
-        if (t[0]) atomicAdd(&st[MODE == kModePlainQuery ? kStQueried : kStValid], t[0]);
+        unsigned long long *st = (unsigned long long *)a.stats + kStSpread + (blockIdx.x & (kSeg - 1));
+        if (t[0]) atomicAdd(&st[kSeg * (MODE == kModePlainQuery ? kSpQueried : kSpValid)], t[0]);
         if (MODE == kModePlainQuery) {
-            if (t[1]) atomicAdd(&st[kStPositive], t[1]);
-            if (t[2]) atomicAdd(&st[kStChecksum], t[2]);
+            if (t[1]) atomicAdd(&st[kSeg * kSpPositive], t[1]);
+            if (t[2]) atomicAdd(&st[kSeg * kSpChecksum], t[2]);
         } else if (t[1]) {
-            atomicAdd(&st[kStSmers], t[1]);
+            atomicAdd(&st[kSeg * kSpSmers], t[1]);
         }
     }
-    (void)queried;
 }
 
 // ----------------------------------------------------------------------------
@@ -543,8 +603,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
 __global__ __launch_bounds__(1024) void k_bin_plan(const uint32_t *bin_count, uint64_t nbins,
                                                    uint32_t *bin_off, uint32_t *cursor,
                                                    uint64_t *work, uint32_t per_item,
-                                                   uint64_t *stats, uint32_t *ccursor,
-                                                   uint32_t fine_bits) {
+                                                   uint64_t *stats) {
     __shared__ uint64_t s_rec[1024], s_items[1024];
     const int tid = threadIdx.x;
     uint64_t chunk = (nbins + 1023) / 1024;
@@ -568,7 +627,6 @@ __global__ __launch_bounds__(1024) void k_bin_plan(const uint32_t *bin_count, ui
         uint32_t c = bin_count[b];
         bin_off[b] = uint32_t(ro);
         cursor[b] = uint32_t(ro);
-        if (ccursor && (b & ((1u << fine_bits) - 1)) == 0) ccursor[b >> fine_bits] = uint32_t(ro);
         uint32_t parts = (c + per_item - 1) / per_item;
         for (uint32_t p = 0; p < parts; ++p) work[io++] = (b << 32) | p;
         ro += c;
@@ -576,6 +634,7 @@ __global__ __launch_bounds__(1024) void k_bin_plan(const uint32_t *bin_count, ui
     if (tid == 1023) {
         stats[kStNWork] = s_items[1023];
         stats[kStWorkHead] = 0;
+        stats[kStRecords] = s_rec[1023];
     }
 }
 
@@ -589,46 +648,128 @@ constexpr int kPartThreads = 256;
 constexpr int kPartItems = kPartTile / kPartThreads;
 constexpr uint32_t kPartRange = 2048;
 
-// key(rec) = (bin >> key_shift) - base, where base is the key range start of
-// the tile (fine pass over coarse-ordered input) or 0.
-__global__ __launch_bounds__(kPartThreads) void k_partition(const uint64_t *src, const uint64_t *stats,
-                                                            uint32_t *cursor, uint64_t *dst,
-                                                            uint32_t bin_shift, uint32_t key_shift,
-                                                            uint32_t fine_bits, uint32_t tile_based) {
+// Records laid out as nseg segments of seg_cap slots; counts[s] are valid.
+struct SegIn {
+    const uint64_t *src;
+    const uint32_t *counts;
+    uint32_t nseg;
+    uint64_t seg_cap;
+};
+
+// Key of a record: (bin >> key_shift) - lo, where lo is the first bin of the
+// coarse bucket of the tile's first record (tile_based) or 0.
+struct KeySpec {
+    uint32_t bin_shift, key_shift, fine_bits, tile_based;
+};
+
+__device__ __forceinline__ uint32_t rec_bin(uint64_t r, uint32_t bin_shift) {
+    return uint32_t((r >> 37) >> bin_shift);
+}
+
+__device__ __forceinline__ uint32_t tile_lo(const uint64_t *tile, const KeySpec &ks) {
+    if (!ks.tile_based) return 0;
+    uint32_t b0 = rec_bin(tile[0], ks.bin_shift);
+    return (b0 >> ks.fine_bits) << ks.fine_bits;
+}
+
+// Histogram of keys (records per key, and k-mers per key when kmers != null).
+__global__ __launch_bounds__(kPartThreads) void k_seg_hist(SegIn in, KeySpec ks, uint32_t *hist,
+                                                           uint32_t *kmers) {
+    __shared__ uint32_t s_h[kPartRange], s_k[kPartRange];
+    __shared__ uint32_t s_lo;
+    const uint32_t tid = threadIdx.x;
+    const uint64_t tps = (in.seg_cap + kPartTile - 1) / kPartTile;
+    for (uint64_t u = blockIdx.x; u < in.nseg * tps; u += gridDim.x) {
+        const uint32_t seg = uint32_t(u / tps);
+        const uint64_t t0 = (u % tps) * kPartTile;
+        const uint64_t n = in.counts[seg];
+        if (t0 >= n) continue;  // uniform across the workgroup
+        const uint64_t *tile = in.src + seg * in.seg_cap + t0;
+        const uint32_t cnt = uint32_t(min<uint64_t>(kPartTile, n - t0));
+        for (uint32_t i = tid; i < kPartRange; i += kPartThreads) s_h[i] = 0, s_k[i] = 0;
+        if (tid == 0) s_lo = tile_lo(tile, ks);
+        __syncthreads();
+        const uint32_t lo = s_lo;
+        for (uint32_t i = tid; i < cnt; i += kPartThreads) {
+            uint64_t r = tile[i];
+            uint32_t key = rec_bin(r, ks.bin_shift) >> ks.key_shift;
+            uint32_t nk = uint32_t((r >> 32) & 31) + 1;
+            if (key - lo < kPartRange) {
+                atomicAdd(&s_h[key - lo], 1u);
+                if (kmers) atomicAdd(&s_k[key - lo], nk);
+            } else {
+                atomicAdd(&hist[key], 1u);
+                if (kmers) atomicAdd(&kmers[key], nk);
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = tid; i < kPartRange; i += kPartThreads) {
+            if (s_h[i]) atomicAdd(&hist[lo + i], s_h[i]);
+            if (kmers && s_k[i]) atomicAdd(&kmers[lo + i], s_k[i]);
+        }
+        __syncthreads();
+    }
+}
+
+// Exclusive scan of a small histogram (n <= 2048) into cursors; the total
+// becomes the record count.
+__global__ __launch_bounds__(1024) void k_small_scan(const uint32_t *hist, uint32_t n, uint32_t *cursor,
+                                                     uint64_t *stats, uint32_t *total_out) {
+    __shared__ uint32_t s[2048];
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i < 2048; i += 1024) s[i] = i < n ? hist[i] : 0;
+    __syncthreads();
+    // two elements per thread: Hillis-Steele over 1024 pair sums
+    uint32_t a0 = s[2 * tid], a1 = s[2 * tid + 1];
+    __shared__ uint32_t p[1024];
+    p[tid] = a0 + a1;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024; d <<= 1) {
+        uint32_t v = tid >= d ? p[tid - d] : 0;
+        __syncthreads();
+        p[tid] += v;
+        __syncthreads();
+    }
+    uint32_t ex = p[tid] - a0 - a1;
+    if (2 * tid < n) cursor[2 * tid] = ex;
+    if (2 * tid + 1 < n) cursor[2 * tid + 1] = ex + a0;
+    if (tid == 1023) {
+        stats[kStRecords] = p[1023];
+        *total_out = p[1023];
+    }
+}
+
+__global__ __launch_bounds__(kPartThreads) void k_partition(SegIn in, KeySpec ks, uint32_t *cursor,
+                                                            uint64_t *dst) {
     __shared__ uint32_t s_hist[kPartRange];
     __shared__ uint32_t s_lo;
-    const uint64_t n = stats[kStRecords];
-    const uint64_t ntiles = (n + kPartTile - 1) / kPartTile;
     const uint32_t tid = threadIdx.x;
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const uint64_t t0 = tile * kPartTile;
+    const uint64_t tps = (in.seg_cap + kPartTile - 1) / kPartTile;
+    for (uint64_t u = blockIdx.x; u < in.nseg * tps; u += gridDim.x) {
+        const uint32_t seg = uint32_t(u / tps);
+        const uint64_t t0 = (u % tps) * kPartTile;
+        const uint64_t n = in.counts[seg];
+        if (t0 >= n) continue;  // uniform across the workgroup
+        const uint64_t *tile = in.src + seg * in.seg_cap + t0;
+        const uint32_t cnt = uint32_t(min<uint64_t>(kPartTile, n - t0));
         for (uint32_t i = tid; i < kPartRange; i += kPartThreads) s_hist[i] = 0;
-        if (tid == 0) {
-            uint32_t lo = 0;
-            if (tile_based) {
-                uint64_t b0 = (src[t0] >> 37) >> bin_shift;
-                lo = uint32_t((b0 >> fine_bits) << fine_bits);
-            }
-            s_lo = lo;
-        }
+        if (tid == 0) s_lo = tile_lo(tile, ks);
         __syncthreads();
         const uint32_t lo = s_lo;
         uint64_t r[kPartItems];
         uint32_t key[kPartItems], rank[kPartItems];
 #pragma unroll
         for (int j = 0; j < kPartItems; ++j) {
-            uint64_t i = t0 + uint64_t(j) * kPartThreads + tid;
+            uint32_t i = uint32_t(j) * kPartThreads + tid;
             key[j] = 0xFFFFFFFFu;
-            if (i < n) {
-                r[j] = src[i];
-                uint32_t bin = uint32_t((r[j] >> 37) >> bin_shift);
-                uint32_t kk = (bin >> key_shift) - lo;
+            if (i < cnt) {
+                r[j] = tile[i];
+                uint32_t kk = (rec_bin(r[j], ks.bin_shift) >> ks.key_shift) - lo;
                 if (kk < kPartRange) {
                     key[j] = kk;
                     rank[j] = atomicAdd(&s_hist[kk], 1u);
-                } else {
-                    // far outside the tile's range: reserve one slot directly
-                    dst[atomicAdd(&cursor[bin >> key_shift], 1u)] = r[j];
+                } else {  // outside the tile's range: reserve one slot directly
+                    dst[atomicAdd(&cursor[kk + lo], 1u)] = r[j];
                 }
             }
         }
@@ -1077,7 +1218,7 @@ class Workspace {
     hipStream_t stream = nullptr;
     DevBuf<uint32_t> pk, valid;
     DevBuf<uint64_t> rec, sorted, work, stats;
-    DevBuf<uint32_t> bin_count, bin_off, cursor, ccursor;
+    DevBuf<uint32_t> bin_count, bin_off, cursor, ccursor, coarse_hist, seg_count;
     uint64_t *final_sorted = nullptr;  // records grouped by bin after the partition passes
     DevBuf<uint8_t> h_bases_dev;   // staging for host batches
     DevBuf<uint64_t> h_offsets_dev;
@@ -1101,9 +1242,10 @@ namespace {
 Status make_workspace(std::unique_ptr<Workspace> &ws) {
     ws.reset(new Workspace);
     SCB_HIP(hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking));
-    SCB_HIP(hipHostMalloc(reinterpret_cast<void **>(&ws->h_stats), kStCount * sizeof(uint64_t),
+    SCB_HIP(hipHostMalloc(reinterpret_cast<void **>(&ws->h_stats), kStatsLen * sizeof(uint64_t),
                           hipHostMallocDefault));
-    SCB_HIP(ws->stats.reserve(kStCount));
+    SCB_HIP(ws->stats.reserve(kStatsLen));
+    SCB_HIP(ws->seg_count.reserve(kSeg + 1));
     for (auto &e : ws->ev) SCB_HIP(hipEventCreate(&e));
     return Status::ok();
 }
@@ -1241,8 +1383,7 @@ struct Launch {
 };
 
 template <int MODE>
-Status launch_scan(Launch &L, const DeviceBatch &b, uint32_t bin_shift, uint32_t *valid_bits,
-                   void *out, bool with_records, uint32_t *bin_kmers = nullptr) {
+Status launch_scan(Launch &L, const DeviceBatch &b, uint64_t seg_cap, uint32_t *valid_bits, void *out) {
     ScanArgs a{};
     a.bases = b.d_bases;
     a.offsets = b.d_offsets;
@@ -1251,13 +1392,12 @@ Status launch_scan(Launch &L, const DeviceBatch &b, uint32_t bin_shift, uint32_t
     a.pk = L.ws.pk.p;
     a.valid_bits = valid_bits;
     a.rec = L.ws.rec.p;
-    a.bin_count = L.ws.bin_count.p;
-    a.bin_kmers = bin_kmers;
-    a.bin_shift = bin_shift;
+    a.seg_count = L.ws.seg_count.p;
+    a.seg_cap = seg_cap;
     a.stats = L.ws.stats.p;
     a.table = L.table;
     a.out = out;
-    (void)with_records;
+    a.aligned16 = (reinterpret_cast<uintptr_t>(b.d_bases) & 15) == 0;
     unsigned tiles = unsigned((b.n_bases + kTile - 1) / kTile);
     if (!tiles) return Status::ok();
     bool wide = L.geo.win > 8;
@@ -1278,54 +1418,65 @@ Status launch_scan(Launch &L, const DeviceBatch &b, uint32_t bin_shift, uint32_t
     return Status::ok();
 }
 
-// Batched front half shared by insert and query: scan, plan, scatter.
+// Batched front half shared by insert, query and spectrum: scan to records,
+// then group the records by bin (one or two tile-aggregated counting-sort
+// passes) and build the work list.
 Status batched_front(Launch &L, const DeviceBatch &b, uint32_t bin_log, uint32_t per_item,
                      uint32_t *valid_bits, uint64_t &nbins_out, uint32_t *bin_kmers = nullptr) {
     const Geometry &g = L.geo;
     Workspace &ws = L.ws;
-    uint64_t nbins = uint64_t(1) << (g.cse - bin_log);
-    uint32_t bin_shift = bin_log - g.bse;
-    uint64_t tiles = (b.n_bases + kTile - 1) / kTile;
-    uint64_t words = tiles * (kTile / 16) + kPkPadWords;  // every tile writes kTile/16 words
-    uint64_t max_rec = b.n_bases + 1;
+    const uint64_t nbins = uint64_t(1) << (g.cse - bin_log);
+    const uint32_t bin_shift = bin_log - g.bse;
+    const uint64_t tiles = (b.n_bases + kTile - 1) / kTile;
+    const uint64_t words = tiles * (kTile / 16) + kPkPadWords;  // every tile writes kTile/16 words
+    const uint64_t seg_cap = (tiles + kSeg - 1) / kSeg * kTile;  // a tile emits at most kTile records
+    const uint64_t cap = seg_cap * kSeg;
     SCB_HIP(ws.pk.reserve(words));
-    SCB_HIP(ws.rec.reserve(max_rec));
-    SCB_HIP(ws.sorted.reserve(max_rec));
+    SCB_HIP(ws.rec.reserve(cap));
+    SCB_HIP(ws.sorted.reserve(cap));
     SCB_HIP(ws.bin_count.reserve(nbins));
     SCB_HIP(ws.bin_off.reserve(nbins));
     SCB_HIP(ws.cursor.reserve(nbins));
-    SCB_HIP(ws.work.reserve(nbins + max_rec / per_item + 2));
+    SCB_HIP(ws.work.reserve(nbins + cap / per_item + 2));
     hipStream_t st = ws.stream;
-    fill(reinterpret_cast<uint32_t *>(ws.stats.p), 2 * kStCount, 0, st);
+    fill(reinterpret_cast<uint32_t *>(ws.stats.p), 2 * kStatsLen, 0, st);
+    fill(ws.seg_count.p, kSeg + 1, 0, st);
     fill(ws.bin_count.p, nbins, 0, st);
     if (bin_kmers) fill(bin_kmers, nbins, 0, st);
     fill(ws.pk.p + (words - kPkPadWords), kPkPadWords, 0, st);
     mark(ws, 0);
-    Status s = launch_scan<kModeRecords>(L, b, bin_shift, valid_bits, nullptr, true, bin_kmers);
+    Status s = launch_scan<kModeRecords>(L, b, seg_cap, valid_bits, nullptr);
     if (!s.good()) return s;
     mark(ws, 1);
-    // two partition passes when the bins outnumber the LDS histogram
-    const bool two_pass = nbins > kPartRange;
-    uint32_t fine_bits = 0;
-    if (two_pass) {
-        uint32_t lb = uint32_t(__builtin_ctzll(nbins));
-        fine_bits = std::max<uint32_t>(7, lb > 11 ? lb - 11 : 0);
-        SCB_HIP(ws.ccursor.reserve(nbins >> fine_bits));
-    }
-    hipLaunchKernelGGL(k_bin_plan, dim3(1), dim3(1024), 0, st, ws.bin_count.p, nbins, ws.bin_off.p,
-                       ws.cursor.p, ws.work.p, per_item, ws.stats.p, two_pass ? ws.ccursor.p : nullptr,
-                       fine_bits);
-    SCB_HIP(hipGetLastError());
-    int pgrid = cu_count(L.device) * 8;
-    if (two_pass) {
-        hipLaunchKernelGGL(k_partition, dim3(pgrid), dim3(kPartThreads), 0, st, ws.rec.p, ws.stats.p,
-                           ws.ccursor.p, ws.sorted.p, bin_shift, fine_bits, 0u, 0u);
-        hipLaunchKernelGGL(k_partition, dim3(pgrid), dim3(kPartThreads), 0, st, ws.sorted.p, ws.stats.p,
-                           ws.cursor.p, ws.rec.p, bin_shift, 0u, fine_bits, 1u);
+    const int pgrid = cu_count(L.device) * 8;
+    const dim3 pg(pgrid), pb(kPartThreads);
+    SegIn segs{ws.rec.p, ws.seg_count.p, kSeg, seg_cap};
+    if (nbins > kPartRange) {
+        // coarse pass by bin >> fine_bits, then a fine pass by bin
+        const uint32_t lb = uint32_t(__builtin_ctzll(nbins));
+        const uint32_t fine_bits = std::max<uint32_t>(7, lb > 11 ? lb - 11 : 0);
+        const uint32_t ncoarse = uint32_t(nbins >> fine_bits);
+        SCB_HIP(ws.coarse_hist.reserve(ncoarse));
+        SCB_HIP(ws.ccursor.reserve(ncoarse));
+        fill(ws.coarse_hist.p, ncoarse, 0, st);
+        KeySpec coarse{bin_shift, fine_bits, 0, 0};
+        hipLaunchKernelGGL(k_seg_hist, pg, pb, 0, st, segs, coarse, ws.coarse_hist.p, (uint32_t *)nullptr);
+        hipLaunchKernelGGL(k_small_scan, dim3(1), dim3(1024), 0, st, ws.coarse_hist.p, ncoarse, ws.ccursor.p,
+                           ws.stats.p, ws.seg_count.p + kSeg);
+        hipLaunchKernelGGL(k_partition, pg, pb, 0, st, segs, coarse, ws.ccursor.p, ws.sorted.p);
+        SegIn comp{ws.sorted.p, ws.seg_count.p + kSeg, 1, cap};
+        KeySpec fine{bin_shift, 0, fine_bits, 1};
+        hipLaunchKernelGGL(k_seg_hist, pg, pb, 0, st, comp, fine, ws.bin_count.p, bin_kmers);
+        hipLaunchKernelGGL(k_bin_plan, dim3(1), dim3(1024), 0, st, ws.bin_count.p, nbins, ws.bin_off.p,
+                           ws.cursor.p, ws.work.p, per_item, ws.stats.p);
+        hipLaunchKernelGGL(k_partition, pg, pb, 0, st, comp, fine, ws.cursor.p, ws.rec.p);
         ws.final_sorted = ws.rec.p;
     } else {
-        hipLaunchKernelGGL(k_partition, dim3(pgrid), dim3(kPartThreads), 0, st, ws.rec.p, ws.stats.p,
-                           ws.cursor.p, ws.sorted.p, bin_shift, 0u, 0u, 0u);
+        KeySpec one{bin_shift, 0, 0, 0};
+        hipLaunchKernelGGL(k_seg_hist, pg, pb, 0, st, segs, one, ws.bin_count.p, bin_kmers);
+        hipLaunchKernelGGL(k_bin_plan, dim3(1), dim3(1024), 0, st, ws.bin_count.p, nbins, ws.bin_off.p,
+                           ws.cursor.p, ws.work.p, per_item, ws.stats.p);
+        hipLaunchKernelGGL(k_partition, pg, pb, 0, st, segs, one, ws.cursor.p, ws.sorted.p);
         ws.final_sorted = ws.sorted.p;
     }
     SCB_HIP(hipGetLastError());
@@ -1335,9 +1486,13 @@ Status batched_front(Launch &L, const DeviceBatch &b, uint32_t bin_log, uint32_t
 }
 
 Status finish(Workspace &ws) {
-    SCB_HIP(hipMemcpyAsync(ws.h_stats, ws.stats.p, kStCount * sizeof(uint64_t),
+    SCB_HIP(hipMemcpyAsync(ws.h_stats, ws.stats.p, kStatsLen * sizeof(uint64_t),
                            hipMemcpyDeviceToHost, ws.stream));
     SCB_HIP(hipStreamSynchronize(ws.stream));
+    // fold the per-tile spread counters into their totals
+    const int direct[kSpCount] = {kStValid, kStSmers, kStQueried, kStPositive, kStChecksum};
+    for (int q = 0; q < kSpCount; ++q)
+        for (int i = 0; i < kSeg; ++i) ws.h_stats[direct[q]] += ws.h_stats[kStSpread + q * kSeg + i];
     return Status::ok();
 }
 
@@ -1374,9 +1529,9 @@ Status Engine::insert(const DeviceBatch &b, Method method, InsertResult &r) {
     hipStream_t st = ws->stream;
     bool batched = method == kBatched && batched_ok(geo_, tune_.insert_bin_log);
     if (!batched) {
-        fill(reinterpret_cast<uint32_t *>(ws->stats.p), 2 * kStCount, 0, st);
+        fill(reinterpret_cast<uint32_t *>(ws->stats.p), 2 * kStatsLen, 0, st);
         mark(*ws, 0);
-        s = launch_scan<kModePlainInsert>(L, b, 0, nullptr, nullptr, false);
+        s = launch_scan<kModePlainInsert>(L, b, 0, nullptr, nullptr);
         mark(*ws, 1);
     } else {
         uint64_t nbins = 0;
@@ -1440,12 +1595,12 @@ Status Engine::query(const DeviceBatch &b, Method method, void *d_values, uint32
     if (d_valid_bits) fill(d_valid_bits, (b.n_bases + 31) / 32, 0, st);
     bool batched = method == kBatched && batched_ok(geo_, tune_.query_bin_log);
     if (b.n_bases < geo_.k) {
-        fill(reinterpret_cast<uint32_t *>(ws->stats.p), 2 * kStCount, 0, st);
+        fill(reinterpret_cast<uint32_t *>(ws->stats.p), 2 * kStatsLen, 0, st);
     } else if (!batched) {
-        fill(reinterpret_cast<uint32_t *>(ws->stats.p), 2 * kStCount, 0, st);
+        fill(reinterpret_cast<uint32_t *>(ws->stats.p), 2 * kStatsLen, 0, st);
         // the plain scan writes validity itself
         mark(*ws, 0);
-        s = launch_scan<kModePlainQuery>(L, b, 0, d_valid_bits, d_values, false);
+        s = launch_scan<kModePlainQuery>(L, b, 0, d_valid_bits, d_values);
         mark(*ws, 1);
     } else {
         uint64_t nbins = 0;
@@ -1627,7 +1782,7 @@ Status Engine::counter_histogram(std::vector<std::pair<uint64_t, uint64_t>> &row
             break;
         }
         fill(reinterpret_cast<uint32_t *>(ws->hist_dense.p), dense_n * 2, 0, st);
-        fill(reinterpret_cast<uint32_t *>(ws->stats.p), 2 * kStCount, 0, st);
+        fill(reinterpret_cast<uint32_t *>(ws->stats.p), 2 * kStatsLen, 0, st);
         if (geo_.bits == 32) {
             fill(ws->hist_big_keys.p, kHistBigCap, 0, st);
             fill(reinterpret_cast<uint32_t *>(ws->hist_big_counts.p), 2 * kHistBigCap, 0, st);
