@@ -786,6 +786,65 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(SegIn in, KeySpec ks
     }
 }
 
+
+// Copies nwords 32-bit words global -> LDS with U independent 16-byte loads in
+// flight per thread (one memory round trip per U*16 bytes instead of one per
+// word).
+template <int U>
+__device__ __forceinline__ void stage_to_lds(uint32_t *lds, const uint32_t *src, uint32_t nwords,
+                                             uint32_t tid, uint32_t nthr) {
+    if ((nwords & 3) == 0) {
+        const uint4 *sv = reinterpret_cast<const uint4 *>(src);
+        uint4 *dv = reinterpret_cast<uint4 *>(lds);
+        const uint32_t nvec = nwords >> 2;
+        for (uint32_t v0 = tid; v0 < nvec; v0 += nthr * U) {
+            uint4 x[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                uint32_t vi = v0 + u * nthr;
+                if (vi < nvec) x[u] = sv[vi];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                uint32_t vi = v0 + u * nthr;
+                if (vi < nvec) dv[vi] = x[u];
+            }
+        }
+    } else {
+        for (uint32_t wi = tid; wi < nwords; wi += nthr) lds[wi] = src[wi];
+    }
+}
+
+__device__ __forceinline__ void zero_lds(uint32_t *lds, uint32_t nwords, uint32_t tid, uint32_t nthr) {
+    if ((nwords & 3) == 0) {
+        uint4 *dv = reinterpret_cast<uint4 *>(lds);
+        for (uint32_t i = tid; i < (nwords >> 2); i += nthr) dv[i] = make_uint4(0, 0, 0, 0);
+    } else {
+        for (uint32_t i = tid; i < nwords; i += nthr) lds[i] = 0;
+    }
+}
+
+// One 32-bit table word plus the increment counts of its counters, saturated
+// per counter (counter.rs:20-23 applied count times).
+template <int BITS>
+__device__ __forceinline__ uint32_t sat_merge_word(uint32_t old, const uint32_t *c) {
+    if (BITS == 32) {
+        uint64_t sum = uint64_t(old) + c[0];
+        return sum > 0xFFFFFFFFull ? 0xFFFFFFFFu : uint32_t(sum);
+    } else {
+        constexpr uint32_t per = 32 / BITS;
+        constexpr uint32_t mx = Ctr<BITS>::kMax;
+        uint32_t nv = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < per; ++q) {
+            uint32_t cur = (old >> (q * BITS)) & mx;
+            uint32_t sum = cur + (c[q] > mx ? mx : c[q]);
+            nv |= (sum > mx ? mx : sum) << (q * BITS);
+        }
+        return nv;
+    }
+}
+
 struct BinArgs {
     const uint64_t *sorted;
     const uint32_t *bin_off, *bin_count;
@@ -804,7 +863,7 @@ struct BinArgs {
 // increments commute).
 template <int BITS>
 __global__ __launch_bounds__(1024) void k_bin_insert(BinArgs a, DevGeo g) {
-    extern __shared__ uint32_t s_cnt[];
+    extern __shared__ __align__(16) uint32_t s_cnt[];
     __shared__ uint64_t s_item;
     const uint32_t tid = threadIdx.x, nthr = blockDim.x;
     const uint32_t bin_slots = 1u << a.bin_log;
@@ -824,7 +883,7 @@ __global__ __launch_bounds__(1024) void k_bin_insert(BinArgs a, DevGeo g) {
         uint32_t rb = a.bin_off[bin] + part * a.per_item;
         uint32_t re = min(a.bin_off[bin] + cnt, rb + a.per_item);
         bool multi = cnt > a.per_item;
-        for (uint32_t i = tid; i < bin_slots; i += nthr) s_cnt[i] = 0;
+        zero_lds(s_cnt, bin_slots, tid, nthr);
         __syncthreads();
         for (uint32_t r = rb + tid; r < re; r += nthr) {
             uint64_t rec = a.sorted[r];
@@ -846,33 +905,56 @@ __global__ __launch_bounds__(1024) void k_bin_insert(BinArgs a, DevGeo g) {
         constexpr uint32_t per = 32 / BITS;
         const uint32_t nwords = bin_slots / per;
         uint32_t *tw = reinterpret_cast<uint32_t *>(a.table) + bin * uint64_t(nwords);
-        for (uint32_t wi = tid; wi < nwords; wi += nthr) {
-            uint32_t c[per];
-            uint32_t any = 0;
+        if (!multi && (nwords & 3) == 0) {
+            // 16-byte read-modify-write with kU loads in flight per thread
+            constexpr int kU = 4;
+            uint4 *tv = reinterpret_cast<uint4 *>(tw);
+            const uint4 *cv = reinterpret_cast<const uint4 *>(s_cnt);
+            const uint32_t nvec = nwords >> 2;
+            for (uint32_t v0 = tid; v0 < nvec; v0 += nthr * kU) {
+                uint4 old[kU];
 #pragma unroll
-            for (uint32_t q = 0; q < per; ++q) c[q] = s_cnt[wi * per + q], any |= c[q];
-            if (!any) continue;
-            if (!multi) {
-                uint32_t old = tw[wi], nv;
-                if (BITS == 32) {
-                    uint64_t sum = uint64_t(old) + c[0];
-                    nv = sum > 0xFFFFFFFFull ? 0xFFFFFFFFu : uint32_t(sum);
-                } else {
-                    constexpr uint32_t mx = Ctr<BITS>::kMax;
-                    nv = 0;
+                for (int u = 0; u < kU; ++u) {
+                    uint32_t vi = v0 + u * nthr;
+                    if (vi < nvec) old[u] = tv[vi];
+                }
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    uint32_t vi = v0 + u * nthr;
+                    if (vi >= nvec) continue;
+                    uint32_t c[4 * per];
+                    uint32_t any = 0;
 #pragma unroll
                     for (uint32_t q = 0; q < per; ++q) {
-                        uint32_t cur = (old >> (q * BITS)) & mx;
-                        uint32_t sum = cur + (c[q] > mx ? mx : c[q]);
-                        nv |= (sum > mx ? mx : sum) << (q * BITS);
+                        uint4 x = cv[vi * per + q];
+                        c[4 * q] = x.x, c[4 * q + 1] = x.y, c[4 * q + 2] = x.z, c[4 * q + 3] = x.w;
+                        any |= x.x | x.y | x.z | x.w;
                     }
+                    if (!any) continue;
+                    uint4 nv;
+                    nv.x = sat_merge_word<BITS>(old[u].x, c);
+                    nv.y = sat_merge_word<BITS>(old[u].y, c + per);
+                    nv.z = sat_merge_word<BITS>(old[u].z, c + 2 * per);
+                    nv.w = sat_merge_word<BITS>(old[u].w, c + 3 * per);
+                    tv[vi] = nv;
                 }
-                tw[wi] = nv;
-            } else {
+            }
+        } else {
+            for (uint32_t wi = tid; wi < nwords; wi += nthr) {
+                uint32_t c[per];
+                uint32_t any = 0;
 #pragma unroll
-                for (uint32_t q = 0; q < per; ++q)
-                    if (c[q]) sat_add_global<BITS>(a.table, (bin * uint64_t(nwords) + wi) * per + q,
-                                                   c[q] > Ctr<BITS>::kMax && BITS < 32 ? Ctr<BITS>::kMax : c[q]);
+                for (uint32_t q = 0; q < per; ++q) c[q] = s_cnt[wi * per + q], any |= c[q];
+                if (!any) continue;
+                if (!multi) {
+                    tw[wi] = sat_merge_word<BITS>(tw[wi], c);
+                } else {
+#pragma unroll
+                    for (uint32_t q = 0; q < per; ++q)
+                        if (c[q])
+                            sat_add_global<BITS>(a.table, (bin * uint64_t(nwords) + wi) * per + q,
+                                                 c[q] > Ctr<BITS>::kMax && BITS < 32 ? Ctr<BITS>::kMax : c[q]);
+                }
             }
         }
         __syncthreads();
@@ -884,7 +966,7 @@ __global__ __launch_bounds__(1024) void k_bin_insert(BinArgs a, DevGeo g) {
 // min over k-s+1 s-mers per k-mer, abundances written at their window index.
 template <int BITS, int WINC>
 __global__ __launch_bounds__(512) void k_bin_query(BinArgs a, DevGeo g) {
-    extern __shared__ uint32_t s_tab[];
+    extern __shared__ __align__(16) uint32_t s_tab[];
     __shared__ uint64_t s_item;
     __shared__ uint64_t s_red[3][8];
     using T = typename Ctr<BITS>::T;
@@ -909,7 +991,7 @@ __global__ __launch_bounds__(512) void k_bin_query(BinArgs a, DevGeo g) {
         uint32_t rb = a.bin_off[bin] + part * a.per_item;
         uint32_t re = min(a.bin_off[bin] + cnt, rb + a.per_item);
         const uint32_t *tw = reinterpret_cast<const uint32_t *>(a.table) + bin * uint64_t(nwords);
-        for (uint32_t wi = tid; wi < nwords; wi += nthr) s_tab[wi] = tw[wi];
+        stage_to_lds<8>(s_tab, tw, nwords, tid, nthr);
         __syncthreads();
         for (uint32_t r = rb + tid; r < re; r += nthr) {
             uint64_t rec = a.sorted[r];
@@ -1080,7 +1162,7 @@ __global__ __launch_bounds__(512) void k_bin_spectrum(SpecArgs sa, DevGeo g) {
         uint32_t re = min(a.bin_off[bin] + cnt, rb + a.per_item);
         bool global = cnt > a.per_item || sa.bin_kmers[bin] > kSetFill;
         const uint32_t *tw = reinterpret_cast<const uint32_t *>(a.table) + bin * uint64_t(bin_slots * (BITS / 8) / 4);
-        for (uint32_t wi = tid; wi < nwords; wi += nthr) s_tab[wi] = tw[wi];
+        stage_to_lds<8>(s_tab, tw, nwords, tid, nthr);
         if (!global)
             for (uint32_t i = tid; i < kSetCap; i += nthr) s_set[i] = kEmptyKey;
         __syncthreads();
