@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "scb_engine.h"
@@ -47,6 +48,9 @@ constexpr int kSpanWords = kSpan / 32 + 2;
 constexpr int kPkWords = kSpan / 16 + 8;
 constexpr int kHashN = kTile + 64;
 constexpr int kWinN = kTile + 32;
+constexpr int kChunksMax = kSpan / 16;
+constexpr int kGPad = kSpan + kSpan / 16;  // padded XOR-prefix array length
+static_assert(2 * kGPad <= 4 * kHashN, "prefix arrays must fit the argmin arrays' space");
 constexpr uint16_t kNoWin = 0xFFFF;
 constexpr int kPkPadWords = 16;        // zero words after the packed batch
 
@@ -279,7 +283,23 @@ struct ScanArgs {
     void *table;           // plain paths
     void *out;             // plain query abundances (position-indexed) or null
     uint32_t aligned16;    // bases pointer is 16-byte aligned
+    const uint32_t *tile_r0;  // first record starting after each tile's left edge
+    uint32_t ntiles;
 };
+
+// For every scan tile, the first record whose start lies after the tile's left
+// edge t*kTile - kLeft (a binary search per tile, all tiles in parallel).
+__global__ void k_tile_r0(const uint64_t *offsets, uint64_t n_reads, uint32_t ntiles, uint32_t *r0) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    int64_t lb = int64_t(t) * kTile - kLeft;
+    uint64_t lo = 0, hi = n_reads;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (int64_t(offsets[mid]) > lb) hi = mid; else lo = mid + 1;
+    }
+    r0[t] = uint32_t(lo);
+}
 
 enum ScanMode { kModeRecords = 0, kModePlainInsert = 1, kModePlainQuery = 2 };
 
@@ -300,280 +320,295 @@ template <int MODE, int BITS, int WINC>
 __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
     __shared__ uint32_t s_bad[kSpanWords], s_tg[kSpanWords], s_brk[kSpanWords];
     __shared__ uint32_t s_pk[kPkWords];
-    __shared__ uint32_t s_big[5 * kHashN];  // H | Gf, Gr  then  H | PL, SL, PR, SR
+    __shared__ uint32_t s_H[kHashN];
+    // Gf | Gr (padded: index li + li/16 keeps the 16-per-thread writes
+    // bank-conflict free), then the four block prefix/suffix minimum arrays
+    __shared__ uint32_t s_G[4 * kHashN];
     __shared__ uint16_t s_mpos[kWinN];
-    __shared__ uint32_t s_cnt[kTileThreads];
     __shared__ uint32_t s_wx[2][kTileThreads / 64];
+    __shared__ uint32_t s_of[2][kChunksMax];  // XOR prefix of all chunks before chunk c
+    __shared__ uint32_t s_wc[kTileThreads / 64];
     __shared__ uint64_t s_red[3][kTileThreads / 64];
-    __shared__ uint64_t s_r0, s_base;
+    __shared__ uint64_t s_base;
+    __shared__ uint32_t s_r0;
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     constexpr int nwave = kTileThreads / 64;
     constexpr int kChunks = kSpan / 16;
-    const int64_t t0 = int64_t(blockIdx.x) * kTile;
-    const int64_t LB = t0 - kLeft;
     const int64_t N = int64_t(a.n_bases);
     const int k = int(g.k), m = int(g.m), w = int(g.w);
     uint16_t *s_bad16 = reinterpret_cast<uint16_t *>(s_bad);
     uint16_t *s_tg16 = reinterpret_cast<uint16_t *>(s_tg);
-    uint32_t *s_H = s_big;
-    uint32_t *s_Gf = s_big + kHashN, *s_Gr = s_big + kHashN + kSpan;
-    uint32_t *s_PL = s_big + kHashN, *s_SL = s_big + 2 * kHashN;
-    uint32_t *s_PR = s_big + 3 * kHashN, *s_SR = s_big + 4 * kHashN;
+    uint32_t *s_Gf = s_G, *s_Gr = s_G + kGPad;
+    // (key16 << 16 | pos) minima give the leftmost argmin, (key16 << 16 | ~pos)
+    // the rightmost one
+    uint32_t *s_PL = s_G, *s_SL = s_G + kHashN, *s_PR = s_G + 2 * kHashN, *s_SR = s_G + 3 * kHashN;
+    const int NH = kTile + 2 * w - 1;
+    uint64_t acc_valid = 0, acc_smers = 0, acc_pos = 0, acc_sum = 0;
 
-    for (int i = tid; i < kSpanWords; i += kTileThreads) s_brk[i] = 0;
-    if (tid == 0) {
-        // first record whose start lies after LB
-        uint64_t lo = 0, hi = a.n_reads;
-        while (lo < hi) {
-            uint64_t mid = (lo + hi) >> 1;
-            if (int64_t(a.offsets[mid]) > LB) hi = mid; else lo = mid + 1;
-        }
-        s_r0 = lo;
-    }
-
-    // (1) 16 bases per thread.
-    uint32_t cw = 0;
-    if (tid < kChunks) {
-        const int64_t g0 = LB + 16 * tid;
-        uint32_t wv[4];
+    // 16 bases per thread are prefetched one tile ahead.
+    auto fetch = [&](int64_t tile_lb, uint32_t (&wv)[4]) {
+        const int64_t g0 = tile_lb + 16 * tid;
         if (a.aligned16 && g0 >= 0 && g0 + 16 <= N) {
             uint4 v = *reinterpret_cast<const uint4 *>(a.bases + g0);
             wv[0] = v.x, wv[1] = v.y, wv[2] = v.z, wv[3] = v.w;
-        } else {
-#pragma unroll
+        } else {  // unaligned or at the batch edges: byte loads
+#pragma unroll 1
             for (int q = 0; q < 4; ++q) {
                 uint32_t x = 0;
-#pragma unroll
+#pragma unroll 1
                 for (int t = 0; t < 4; ++t) {
                     int64_t gi = g0 + 4 * q + t;
                     uint32_t ch = (gi >= 0 && gi < N) ? a.bases[gi] : uint32_t('N');
                     x |= ch << (8 * t);
                 }
-                wv[q] = x;
+                if (q == 0) wv[0] = x;
+                if (q == 1) wv[1] = x;
+                if (q == 2) wv[2] = x;
+                if (q == 3) wv[3] = x;
             }
         }
-        uint32_t badm = 0, tgm = 0;
-#pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            uint32_t ch = (wv[t >> 2] >> (8 * (t & 3))) & 0xFF;
-            uint32_t code = (ch >> 1) & 3;
-            bool ok = is_acgt(ch);
-            cw |= code << (2 * t);
-            badm |= uint32_t(!ok) << t;
-            tgm |= uint32_t(ok && (code >> 1)) << t;
-        }
-        s_pk[tid] = cw;
-        s_bad16[tid] = uint16_t(badm);
-        s_tg16[tid] = uint16_t(tgm);
-        if (MODE == kModeRecords && tid >= kLeft / 16 && tid < kLeft / 16 + kTile / 16)
-            a.pk[(LB >> 4) + tid] = cw;  // LB is a multiple of 16
-    } else if (tid < kPkWords) {
-        s_pk[tid] = 0;
-        if (tid < 2 * kSpanWords) s_bad16[tid] = 0xFFFF, s_tg16[tid] = 0;
-    }
-
-    // (2) ntHash XOR prefixes: local exclusive prefixes of this thread's 16 terms.
-    uint32_t xf = 0, xr = 0, lf[16], lr[16];
-    if (tid < kChunks) {
-#pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            uint32_t c = (cw >> (2 * t)) & 3;
-            uint32_t j = uint32_t(16 * tid + t);
-            lf[t] = xf, lr[t] = xr;
-            xf ^= rotl32(nt_seed(c), (32 - (j & 31)) & 31);
-            xr ^= rotl32(nt_seed(c ^ 2), j & 31);
-        }
-    }
-    uint32_t sf = xf, sr = xr;  // inclusive XOR scan of the per-thread totals
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t yf = __shfl_up(sf, off, 64), yr = __shfl_up(sr, off, 64);
-        if (lane >= off) sf ^= yf, sr ^= yr;
-    }
-    if (lane == 63) s_wx[0][wid] = sf, s_wx[1][wid] = sr;
-    __syncthreads();
-    // record starts inside the span mark read boundaries
-    for (uint64_t r = s_r0 + tid; r < a.n_reads; r += kTileThreads) {
-        int64_t o = int64_t(a.offsets[r]);
-        if (o >= LB + kSpan) break;
-        int li = int(o - LB);
-        atomicOr(&s_brk[li >> 5], 1u << (li & 31));
-    }
-    if (tid < kChunks) {
-        uint32_t pf = 0, pr = 0;
-        for (int i = 0; i < wid; ++i) pf ^= s_wx[0][i], pr ^= s_wx[1][i];
-        uint32_t ef = pf ^ sf ^ xf, er = pr ^ sr ^ xr;  // exclusive prefix of this thread
-#pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            s_Gf[16 * tid + t] = ef ^ lf[t];
-            s_Gr[16 * tid + t] = er ^ lr[t];
-        }
-    }
-    __syncthreads();
-
-    // Canonical hash of every m-mer at positions p = t0-1+hp, hp in [0, NH).
-    const int NH = kTile + 2 * w - 1;
-    for (int hp = tid; hp < NH; hp += kTileThreads) {
-        int li = hp + kLeft - 1;
-        uint32_t fw = rotl32(s_Gf[li + m] ^ s_Gf[li], uint32_t(li + m - 1) & 31);
-        uint32_t rc = rotr32(s_Gr[li + m] ^ s_Gr[li], uint32_t(li) & 31);
-        s_H[hp] = fw + rc;
-    }
-    __syncthreads();
-
-    // (3) block prefix/suffix minima (blocks of w hash positions).
+    };
+    // One tile per workgroup (a persistent loop lets the compiler hoist
+    // per-thread invariants into ~60 more VGPRs and halves occupancy).
     {
-        const int nblk = (NH + w - 1) / w;
-        for (int b = tid; b < nblk; b += kTileThreads) {
-            const int h0 = b * w, h1 = min(h0 + w, NH);
-            uint32_t cl = ~0u, cr = ~0u;
-            for (int hp = h0; hp < h1; ++hp) {
-                uint32_t key = s_H[hp] & 0xFFFF0000u;
-                cl = min(cl, key | uint32_t(hp));
-                cr = min(cr, key | uint32_t(0xFFFF - hp));
-                s_PL[hp] = cl, s_PR[hp] = cr;
-            }
-            cl = cr = ~0u;
-            for (int hp = h1 - 1; hp >= h0; --hp) {
-                uint32_t key = s_H[hp] & 0xFFFF0000u;
-                cl = min(cl, key | uint32_t(hp));
-                cr = min(cr, key | uint32_t(0xFFFF - hp));
-                s_SL[hp] = cl, s_SR[hp] = cr;
-            }
-        }
-    }
-    __syncthreads();
+        const uint32_t tile = blockIdx.x;
+        uint32_t wv[4] = {0, 0, 0, 0};
+        if (tid < kChunks) fetch(int64_t(tile) * kTile - kLeft, wv);
+        const int64_t t0 = int64_t(tile) * kTile;
+        const int64_t LB = t0 - kLeft;
+        for (int i = tid; i < kSpanWords; i += kTileThreads) s_brk[i] = 0;
+        if (tid == 0) s_r0 = a.tile_r0[tile];
 
-    // (4) minimizer position of every window j = t0-1+hj, hj in [0, kTile + w).
-    for (int hj = tid; hj < kTile + w; hj += kTileThreads) {
-        int li = hj + kLeft - 1;
-        uint32_t bad = get_bits(s_bad, li, k) | get_bits(s_brk, li + 1, k - 1);
-        uint16_t pos = kNoWin;
-        if (!bad) {
-            uint32_t tg = __popc(get_bits(s_tg, li, k));
-            int e = hj + w - 1;
-            if (2 * int(tg) > k) {  // leftmost tie
-                uint32_t v = min(s_SL[hj], s_PL[e]);
-                pos = uint16_t(v & 0xFFFF);
-            } else {  // rightmost tie
-                uint32_t v = min(s_SR[hj], s_PR[e]);
-                pos = uint16_t(0xFFFF - (v & 0xFFFF));
-            }
-        }
-        s_mpos[hj] = pos;
-    }
-    __syncthreads();
-
-    // Owned windows hj in [1, kTile]: validity bitmap, super-k-mer starts.
-    constexpr int kPer = kTile / kTileThreads;
-    uint32_t starts = 0;  // bit u set if window 1 + u*256 + tid starts a super-k-mer
-    uint64_t nvalid = 0;
+        // (1) codes and masks of this thread's 16 bases
+        uint32_t cw = 0;
+        if (tid < kChunks) {
+            uint32_t badm = 0, tgm = 0;
 #pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-        int hj = 1 + u * kTileThreads + tid;
-        uint16_t mp = s_mpos[hj], pm = s_mpos[hj - 1];
-        bool valid = mp != kNoWin;
-        bool st = valid && (pm == kNoWin || pm != mp);
-        starts |= uint32_t(st) << u;
-        nvalid += valid;
-        if (a.valid_bits) {
-            uint64_t vm = __ballot(valid);
-            if (lane == 0) {
-                int64_t j0 = t0 + u * kTileThreads + wid * 64;
-                if (j0 < N) {
-                    a.valid_bits[j0 >> 5] = uint32_t(vm);
-                    if (j0 + 32 < N) a.valid_bits[(j0 >> 5) + 1] = uint32_t(vm >> 32);
+            for (int t = 0; t < 16; ++t) {
+                uint32_t ch = (wv[t >> 2] >> (8 * (t & 3))) & 0xFF;
+                uint32_t code = (ch >> 1) & 3;
+                bool ok = is_acgt(ch);
+                cw |= code << (2 * t);
+                badm |= uint32_t(!ok) << t;
+                tgm |= uint32_t(ok && (code >> 1)) << t;
+            }
+            s_pk[tid] = cw;
+            s_bad16[tid] = uint16_t(badm);
+            s_tg16[tid] = uint16_t(tgm);
+            if (MODE == kModeRecords && tid >= kLeft / 16 && tid < kLeft / 16 + kTile / 16)
+                a.pk[(LB >> 4) + tid] = cw;  // LB is a multiple of 16
+        } else if (tid < kPkWords) {
+            s_pk[tid] = 0;
+            if (tid < 2 * kSpanWords) s_bad16[tid] = 0xFFFF, s_tg16[tid] = 0;
+        }
+
+        // (2) ntHash XOR prefixes: chunk-local exclusive prefixes go straight
+        // to LDS; the prefix of the preceding chunks is applied when read
+        uint32_t xf = 0, xr = 0;
+        if (tid < kChunks) {
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                uint32_t c = (cw >> (2 * t)) & 3;
+                uint32_t j = uint32_t(16 * tid + t);
+                s_Gf[17 * tid + t] = xf;
+                s_Gr[17 * tid + t] = xr;
+                xf ^= rotl32(nt_seed(c), (32 - (j & 31)) & 31);
+                xr ^= rotl32(nt_seed(c ^ 2), j & 31);
+            }
+        }
+        uint32_t sf = xf, sr = xr;
+        for (int off = 1; off < 64; off <<= 1) {
+            uint32_t yf = __shfl_up(sf, off, 64), yr = __shfl_up(sr, off, 64);
+            if (lane >= off) sf ^= yf, sr ^= yr;
+        }
+        if (lane == 63) s_wx[0][wid] = sf, s_wx[1][wid] = sr;
+        __syncthreads();
+        {   // record starts inside the span mark read boundaries
+            const uint32_t r0 = s_r0;
+            for (uint64_t r = uint64_t(r0) + tid; r < a.n_reads; r += kTileThreads) {
+                int64_t o = int64_t(a.offsets[r]);
+                if (o >= LB + kSpan) break;
+                int li = int(o - LB);
+                atomicOr(&s_brk[li >> 5], 1u << (li & 31));
+            }
+        }
+        if (tid < kChunks) {
+            uint32_t pf = 0, pr = 0;
+            for (int i = 0; i < wid; ++i) pf ^= s_wx[0][i], pr ^= s_wx[1][i];
+            s_of[0][tid] = pf ^ sf ^ xf;
+            s_of[1][tid] = pr ^ sr ^ xr;
+        }
+        __syncthreads();
+
+        // canonical hash of the m-mer at p = t0-1+hp, hp in [0, NH)
+        for (int hp = tid; hp < NH; hp += kTileThreads) {
+            int li = hp + kLeft - 1, lm = li + m;
+            int pi = li + (li >> 4), pm = lm + (lm >> 4);
+            uint32_t gf = s_Gf[pm] ^ s_Gf[pi] ^ s_of[0][lm >> 4] ^ s_of[0][li >> 4];
+            uint32_t gr = s_Gr[pm] ^ s_Gr[pi] ^ s_of[1][lm >> 4] ^ s_of[1][li >> 4];
+            uint32_t fw = rotl32(gf, uint32_t(lm - 1) & 31);
+            uint32_t rc = rotr32(gr, uint32_t(li) & 31);
+            s_H[hp] = fw + rc;
+        }
+        __syncthreads();
+
+        // (3) van Herk/Gil-Werman: per block of w hash positions, the positions
+        // of the leftmost and rightmost minimum of every prefix and suffix
+        {
+            const int nblk = (NH + w - 1) / w;
+            for (int b = tid; b < nblk; b += kTileThreads) {
+                const int h0 = b * w, h1 = min(h0 + w, NH);
+                uint32_t cl = ~0u, cr = ~0u;
+                for (int hp = h0; hp < h1; ++hp) {
+                    uint32_t key = s_H[hp] & 0xFFFF0000u;
+                    cl = min(cl, key | uint32_t(hp));
+                    cr = min(cr, key | uint32_t(0xFFFF - hp));
+                    s_PL[hp] = cl, s_PR[hp] = cr;
+                }
+                cl = cr = ~0u;
+                for (int hp = h1 - 1; hp >= h0; --hp) {
+                    uint32_t key = s_H[hp] & 0xFFFF0000u;
+                    cl = min(cl, key | uint32_t(hp));
+                    cr = min(cr, key | uint32_t(0xFFFF - hp));
+                    s_SL[hp] = cl, s_SR[hp] = cr;
+                }
+            }
+        }
+        __syncthreads();
+
+        // (4) minimizer position of every window j = t0-1+hj, hj in [0, kTile + w)
+        for (int hj = tid; hj < kTile + w; hj += kTileThreads) {
+            int li = hj + kLeft - 1;
+            uint32_t bad = get_bits(s_bad, li, k) | get_bits(s_brk, li + 1, k - 1);
+            uint16_t pos = kNoWin;
+            if (!bad) {
+                uint32_t tg = __popc(get_bits(s_tg, li, k));
+                int e = hj + w - 1;
+                if (2 * int(tg) > k) {  // leftmost minimum
+                    pos = uint16_t(min(s_SL[hj], s_PL[e]) & 0xFFFF);
+                } else {  // rightmost minimum
+                    pos = uint16_t(0xFFFF - (min(s_SR[hj], s_PR[e]) & 0xFFFF));
+                }
+            }
+            s_mpos[hj] = pos;
+        }
+        __syncthreads();
+
+        // owned windows hj in [1, kTile]: validity bitmap, super-k-mer starts
+        constexpr int kPer = kTile / kTileThreads;
+        uint32_t starts = 0;  // bit u set if window 1 + u*256 + tid starts a super-k-mer
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            int hj = 1 + u * kTileThreads + tid;
+            uint16_t mp = s_mpos[hj], pm = s_mpos[hj - 1];
+            bool valid = mp != kNoWin;
+            bool st = valid && (pm == kNoWin || pm != mp);
+            starts |= uint32_t(st) << u;
+            acc_valid += valid;
+            if (a.valid_bits) {
+                uint64_t vm = __ballot(valid);
+                if (lane == 0) {
+                    int64_t j0 = t0 + u * kTileThreads + wid * 64;
+                    if (j0 < N) {
+                        a.valid_bits[j0 >> 5] = uint32_t(vm);
+                        if (j0 + 32 < N) a.valid_bits[(j0 >> 5) + 1] = uint32_t(vm >> 32);
+                    }
+                }
+            }
+        }
+
+        const uint32_t my = __popc(starts);
+        if (MODE == kModeRecords) {
+            // exclusive scan of the per-thread start counts (wave scan + wave totals)
+            uint32_t incl = my;
+            for (int off = 1; off < 64; off <<= 1) {
+                uint32_t y = __shfl_up(incl, off, 64);
+                if (lane >= off) incl += y;
+            }
+            if (lane == 63) s_wc[wid] = incl;
+            __syncthreads();
+            uint32_t before = 0, total = 0;
+            for (int i = 0; i < nwave; ++i) {
+                if (i < wid) before += s_wc[i];
+                total += s_wc[i];
+            }
+            if (tid == 0) {
+                uint32_t seg = tile & (kSeg - 1);
+                s_base = seg * a.seg_cap + atomicAdd(&a.seg_count[seg], total);
+            }
+            __syncthreads();
+            uint64_t slot = s_base + before + incl - my;
+            for (uint32_t bits = starts; bits; bits &= bits - 1) {
+                int u = __ffs(bits) - 1;
+                int hj = 1 + u * kTileThreads + tid;
+                uint16_t mp = s_mpos[hj];
+                int e = hj + 1;
+                while (e < hj + w && s_mpos[e] == mp) ++e;
+                uint32_t nk = uint32_t(e - hj);
+                uint64_t val = canonical_at(s_pk, uint64_t(mp) + kLeft - 1, uint32_t(m));
+                uint64_t block = xs64(val) & g.block_mask;
+                uint64_t pblk = ((block & g.shard_mask) << g.sub_log) | (block >> g.shard_shift);
+                uint64_t j = uint64_t(t0 - 1 + hj);
+                a.rec[slot++] = j | (uint64_t(nk - 1) << 32) | (pblk << 37);
+                acc_smers += nk + g.k - g.s;
+            }
+        } else {
+            // Plain path: this thread applies its own super-k-mers straight to HBM.
+            for (uint32_t bits = starts; bits; bits &= bits - 1) {
+                int u = __ffs(bits) - 1;
+                int hj = 1 + u * kTileThreads + tid;
+                uint16_t mp = s_mpos[hj];
+                int e = hj + 1;
+                while (e < hj + w && s_mpos[e] == mp) ++e;
+                uint32_t nk = uint32_t(e - hj);
+                uint64_t val = canonical_at(s_pk, uint64_t(mp) + kLeft - 1, uint32_t(m));
+                uint64_t block = xs64(val) & g.block_mask;
+                uint64_t pblk = ((block & g.shard_mask) << g.sub_log) | (block >> g.shard_shift);
+                uint64_t cbase = pblk << g.bse;
+                const uint64_t amask = (1ull << g.bse) - 1;
+                uint32_t A0, A1, A2, A3;
+                load_aligned(s_pk, uint64_t(hj + kLeft - 1), A0, A1, A2, A3);
+                uint32_t nbases = nk + g.k - 1;
+                if (MODE == kModePlainInsert) {
+                    // filter.rs:224-233
+                    for_each_smer(A0, A1, A2, A3, nbases, g.s, [&](uint32_t, uint64_t smer) {
+                        uint64_t h = xs64(smer);
+                        for (uint32_t t = 0; t < g.nh; ++t) {
+                            sat_add_global<BITS>(a.table, cbase + (h & amask), 1);
+                            if (t + 1 < g.nh) h = xs64(h);
+                        }
+                    });
+                    acc_smers += nk + g.k - g.s;
+                } else {
+                    // filter.rs:478-507 + 562-571
+                    WinMin<WINC> wm;
+                    uint64_t j0 = uint64_t(t0 - 1 + hj);
+                    for_each_smer(A0, A1, A2, A3, nbases, g.s, [&](uint32_t j, uint64_t smer) {
+                        uint64_t h = xs64(smer);
+                        uint32_t est = Ctr<BITS>::kMax;
+                        for (uint32_t t = 0; t < g.nh; ++t) {
+                            uint32_t v = load_ctr<BITS>(a.table, cbase + (h & amask));
+                            est = v < est ? v : est;
+                            if (t + 1 < g.nh) h = xs64(h);
+                        }
+                        uint32_t ab = wm.push(est, g.win);
+                        if (j + 1 >= g.win) {
+                            uint64_t q = j + 1 - g.win;
+                            if (a.out) reinterpret_cast<typename Ctr<BITS>::T *>(a.out)[j0 + q] =
+                                typename Ctr<BITS>::T(ab);
+                            acc_pos += ab > 0;
+                            acc_sum += ab;
+                        }
+                    });
                 }
             }
         }
     }
 
-    uint32_t my = __popc(starts);
-    uint64_t smers = 0, positive = 0, checksum = 0;
-    if (MODE == kModeRecords) {
-        s_cnt[tid] = my;
-        __syncthreads();
-        for (int d = 1; d < kTileThreads; d <<= 1) {  // inclusive scan (Hillis-Steele)
-            uint32_t v = tid >= d ? s_cnt[tid - d] : 0;
-            __syncthreads();
-            s_cnt[tid] += v;
-            __syncthreads();
-        }
-        if (tid == kTileThreads - 1) {
-            uint32_t seg = blockIdx.x & (kSeg - 1);
-            s_base = seg * a.seg_cap + atomicAdd(&a.seg_count[seg], s_cnt[tid]);
-        }
-        __syncthreads();
-        uint64_t slot = s_base + s_cnt[tid] - my;
-        for (uint32_t bits = starts; bits; bits &= bits - 1) {
-            int u = __ffs(bits) - 1;
-            int hj = 1 + u * kTileThreads + tid;
-            uint16_t mp = s_mpos[hj];
-            int e = hj + 1;
-            while (e < hj + w && s_mpos[e] == mp) ++e;
-            uint32_t nk = uint32_t(e - hj);
-            uint64_t val = canonical_at(s_pk, uint64_t(mp) + kLeft - 1, uint32_t(m));
-            uint64_t block = xs64(val) & g.block_mask;
-            uint64_t pblk = ((block & g.shard_mask) << g.sub_log) | (block >> g.shard_shift);
-            uint64_t j = uint64_t(t0 - 1 + hj);
-            a.rec[slot++] = j | (uint64_t(nk - 1) << 32) | (pblk << 37);
-            smers += nk + g.k - g.s;
-        }
-    } else {
-        // Plain path: this thread applies its own super-k-mers straight to HBM.
-        for (uint32_t bits = starts; bits; bits &= bits - 1) {
-            int u = __ffs(bits) - 1;
-            int hj = 1 + u * kTileThreads + tid;
-            uint16_t mp = s_mpos[hj];
-            int e = hj + 1;
-            while (e < hj + w && s_mpos[e] == mp) ++e;
-            uint32_t nk = uint32_t(e - hj);
-            uint64_t val = canonical_at(s_pk, uint64_t(mp) + kLeft - 1, uint32_t(m));
-            uint64_t block = xs64(val) & g.block_mask;
-            uint64_t pblk = ((block & g.shard_mask) << g.sub_log) | (block >> g.shard_shift);
-            uint64_t cbase = pblk << g.bse;
-            const uint64_t amask = (1ull << g.bse) - 1;
-            uint32_t A0, A1, A2, A3;
-            load_aligned(s_pk, uint64_t(hj + kLeft - 1), A0, A1, A2, A3);
-            uint32_t nbases = nk + g.k - 1;
-            if (MODE == kModePlainInsert) {
-                // filter.rs:224-233
-                for_each_smer(A0, A1, A2, A3, nbases, g.s, [&](uint32_t, uint64_t smer) {
-                    uint64_t h = xs64(smer);
-                    for (uint32_t t = 0; t < g.nh; ++t) {
-                        sat_add_global<BITS>(a.table, cbase + (h & amask), 1);
-                        if (t + 1 < g.nh) h = xs64(h);
-                    }
-                });
-                smers += nk + g.k - g.s;
-            } else {
-                // filter.rs:478-507 + 562-571
-                WinMin<WINC> wm;
-                uint64_t j0 = uint64_t(t0 - 1 + hj);
-                for_each_smer(A0, A1, A2, A3, nbases, g.s, [&](uint32_t j, uint64_t smer) {
-                    uint64_t h = xs64(smer);
-                    uint32_t est = Ctr<BITS>::kMax;
-                    for (uint32_t t = 0; t < g.nh; ++t) {
-                        uint32_t v = load_ctr<BITS>(a.table, cbase + (h & amask));
-                        est = v < est ? v : est;
-                        if (t + 1 < g.nh) h = xs64(h);
-                    }
-                    uint32_t ab = wm.push(est, g.win);
-                    if (j + 1 >= g.win) {
-                        uint64_t q = j + 1 - g.win;
-                        if (a.out) reinterpret_cast<typename Ctr<BITS>::T *>(a.out)[j0 + q] =
-                            typename Ctr<BITS>::T(ab);
-                        positive += ab > 0;
-                        checksum += ab;
-                    }
-                });
-            }
-        }
-    }
-
     // Block reductions of the statistics, added into one of kSeg spread slots.
-    uint64_t vals[3] = {nvalid, MODE == kModePlainQuery ? positive : smers,
-                        MODE == kModePlainQuery ? checksum : 0};
+    uint64_t vals[3] = {acc_valid, MODE == kModePlainQuery ? acc_pos : acc_smers,
+                        MODE == kModePlainQuery ? acc_sum : 0};
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         uint64_t v = vals[q];
@@ -800,15 +835,10 @@ __device__ __forceinline__ void stage_to_lds(uint32_t *lds, const uint32_t *src,
         for (uint32_t v0 = tid; v0 < nvec; v0 += nthr * U) {
             uint4 x[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                uint32_t vi = v0 + u * nthr;
-                if (vi < nvec) x[u] = sv[vi];
-            }
+            for (int u = 0; u < U; ++u) x[u] = sv[min(v0 + u * nthr, nvec - 1)];  // unconditional: all in flight
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                uint32_t vi = v0 + u * nthr;
-                if (vi < nvec) dv[vi] = x[u];
-            }
+            for (int u = 0; u < U; ++u)
+                if (v0 + u * nthr < nvec) dv[v0 + u * nthr] = x[u];
         }
     } else {
         for (uint32_t wi = tid; wi < nwords; wi += nthr) lds[wi] = src[wi];
@@ -914,10 +944,7 @@ __global__ __launch_bounds__(1024) void k_bin_insert(BinArgs a, DevGeo g) {
             for (uint32_t v0 = tid; v0 < nvec; v0 += nthr * kU) {
                 uint4 old[kU];
 #pragma unroll
-                for (int u = 0; u < kU; ++u) {
-                    uint32_t vi = v0 + u * nthr;
-                    if (vi < nvec) old[u] = tv[vi];
-                }
+                for (int u = 0; u < kU; ++u) old[u] = tv[min(v0 + u * nthr, nvec - 1)];  // all in flight
 #pragma unroll
                 for (int u = 0; u < kU; ++u) {
                     uint32_t vi = v0 + u * nthr;
@@ -1300,7 +1327,7 @@ class Workspace {
     hipStream_t stream = nullptr;
     DevBuf<uint32_t> pk, valid;
     DevBuf<uint64_t> rec, sorted, work, stats;
-    DevBuf<uint32_t> bin_count, bin_off, cursor, ccursor, coarse_hist, seg_count;
+    DevBuf<uint32_t> bin_count, bin_off, cursor, ccursor, coarse_hist, seg_count, tile_r0;
     uint64_t *final_sorted = nullptr;  // records grouped by bin after the partition passes
     DevBuf<uint8_t> h_bases_dev;   // staging for host batches
     DevBuf<uint64_t> h_offsets_dev;
@@ -1483,8 +1510,14 @@ Status launch_scan(Launch &L, const DeviceBatch &b, uint64_t seg_cap, uint32_t *
     unsigned tiles = unsigned((b.n_bases + kTile - 1) / kTile);
     if (!tiles) return Status::ok();
     bool wide = L.geo.win > 8;
-    dim3 grid(tiles), blk(kTileThreads);
     hipStream_t st = L.ws.stream;
+    SCB_HIP(L.ws.tile_r0.reserve(tiles));
+    hipLaunchKernelGGL(k_tile_r0, dim3((tiles + 255) / 256), dim3(256), 0, st, b.d_offsets, b.n_reads, tiles,
+                       L.ws.tile_r0.p);
+    a.tile_r0 = L.ws.tile_r0.p;
+    a.ntiles = tiles;
+    // persistent tiles: a few resident workgroups per CU, each prefetching its next tile
+    dim3 grid(tiles), blk(kTileThreads);
 #define SCB_SCAN(BITS)                                                                         \
     do {                                                                                       \
         if (wide) hipLaunchKernelGGL((k_scan<MODE, BITS, 32>), grid, blk, 0, st, a, L.dg);     \
