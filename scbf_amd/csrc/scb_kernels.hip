@@ -50,7 +50,7 @@ constexpr int kHashN = kTile + 64;
 constexpr int kWinN = kTile + 32;
 constexpr int kChunksMax = kSpan / 16;
 constexpr int kGPad = kSpan + kSpan / 16;  // padded XOR-prefix array length
-static_assert(2 * kGPad <= 4 * kHashN, "prefix arrays must fit the argmin arrays' space");
+constexpr int kGArr = 2 * kGPad > 2 * kHashN ? 2 * kGPad : 2 * kHashN;  // prefixes, then four u16 arrays
 constexpr uint16_t kNoWin = 0xFFFF;
 constexpr int kPkPadWords = 16;        // zero words after the packed batch
 
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
     __shared__ uint32_t s_H[kHashN];
     // Gf | Gr (padded: index li + li/16 keeps the 16-per-thread writes
     // bank-conflict free), then the four block prefix/suffix minimum arrays
-    __shared__ uint32_t s_G[4 * kHashN];
+    __shared__ uint32_t s_G[kGArr];
     __shared__ uint16_t s_mpos[kWinN];
     __shared__ uint32_t s_wx[2][kTileThreads / 64];
     __shared__ uint32_t s_of[2][kChunksMax];  // XOR prefix of all chunks before chunk c
@@ -340,9 +340,9 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
     uint16_t *s_bad16 = reinterpret_cast<uint16_t *>(s_bad);
     uint16_t *s_tg16 = reinterpret_cast<uint16_t *>(s_tg);
     uint32_t *s_Gf = s_G, *s_Gr = s_G + kGPad;
-    // (key16 << 16 | pos) minima give the leftmost argmin, (key16 << 16 | ~pos)
-    // the rightmost one
-    uint32_t *s_PL = s_G, *s_SL = s_G + kHashN, *s_PR = s_G + 2 * kHashN, *s_SR = s_G + 3 * kHashN;
+    // positions of the leftmost / rightmost minimum of every block prefix and suffix
+    uint16_t *s_PL = reinterpret_cast<uint16_t *>(s_G);
+    uint16_t *s_SL = s_PL + kHashN, *s_PR = s_PL + 2 * kHashN, *s_SR = s_PL + 3 * kHashN;
     const int NH = kTile + 2 * w - 1;
     uint64_t acc_valid = 0, acc_smers = 0, acc_pos = 0, acc_sum = 0;
 
@@ -464,14 +464,14 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
                     uint32_t key = s_H[hp] & 0xFFFF0000u;
                     cl = min(cl, key | uint32_t(hp));
                     cr = min(cr, key | uint32_t(0xFFFF - hp));
-                    s_PL[hp] = cl, s_PR[hp] = cr;
+                    s_PL[hp] = uint16_t(cl), s_PR[hp] = uint16_t(0xFFFF - (cr & 0xFFFF));
                 }
                 cl = cr = ~0u;
                 for (int hp = h1 - 1; hp >= h0; --hp) {
                     uint32_t key = s_H[hp] & 0xFFFF0000u;
                     cl = min(cl, key | uint32_t(hp));
                     cr = min(cr, key | uint32_t(0xFFFF - hp));
-                    s_SL[hp] = cl, s_SR[hp] = cr;
+                    s_SL[hp] = uint16_t(cl), s_SR[hp] = uint16_t(0xFFFF - (cr & 0xFFFF));
                 }
             }
         }
@@ -485,10 +485,12 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
             if (!bad) {
                 uint32_t tg = __popc(get_bits(s_tg, li, k));
                 int e = hj + w - 1;
-                if (2 * int(tg) > k) {  // leftmost minimum
-                    pos = uint16_t(min(s_SL[hj], s_PL[e]) & 0xFFFF);
-                } else {  // rightmost minimum
-                    pos = uint16_t(0xFFFF - (min(s_SR[hj], s_PR[e]) & 0xFFFF));
+                if (2 * int(tg) > k) {  // leftmost minimum (suffix candidate is the left one)
+                    uint32_t c1 = s_SL[hj], c2 = s_PL[e];
+                    pos = uint16_t((s_H[c1] >> 16) <= (s_H[c2] >> 16) ? c1 : c2);
+                } else {  // rightmost minimum (prefix candidate is the right one)
+                    uint32_t c1 = s_SR[hj], c2 = s_PR[e];
+                    pos = uint16_t((s_H[c2] >> 16) <= (s_H[c1] >> 16) ? c2 : c1);
                 }
             }
             s_mpos[hj] = pos;
