@@ -320,7 +320,7 @@ template <int MODE, int BITS, int WINC>
 __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
     __shared__ uint32_t s_bad[kSpanWords], s_tg[kSpanWords], s_brk[kSpanWords];
     __shared__ uint32_t s_pk[kPkWords];
-    __shared__ uint32_t s_H[kHashN];
+    __shared__ uint16_t s_K[kHashN];  // high 16 bits of each m-mer hash (the argmin key)
     // Gf | Gr (padded: index li + li/16 keeps the 16-per-thread writes
     // bank-conflict free), then the four block prefix/suffix minimum arrays
     __shared__ uint32_t s_G[kGArr];
@@ -449,7 +449,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
             uint32_t gr = s_Gr[pm] ^ s_Gr[pi] ^ s_of[1][lm >> 4] ^ s_of[1][li >> 4];
             uint32_t fw = rotl32(gf, uint32_t(lm - 1) & 31);
             uint32_t rc = rotr32(gr, uint32_t(li) & 31);
-            s_H[hp] = fw + rc;
+            s_K[hp] = uint16_t((fw + rc) >> 16);
         }
         __syncthreads();
 
@@ -461,14 +461,14 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
                 const int h0 = b * w, h1 = min(h0 + w, NH);
                 uint32_t cl = ~0u, cr = ~0u;
                 for (int hp = h0; hp < h1; ++hp) {
-                    uint32_t key = s_H[hp] & 0xFFFF0000u;
+                    uint32_t key = uint32_t(s_K[hp]) << 16;
                     cl = min(cl, key | uint32_t(hp));
                     cr = min(cr, key | uint32_t(0xFFFF - hp));
                     s_PL[hp] = uint16_t(cl), s_PR[hp] = uint16_t(0xFFFF - (cr & 0xFFFF));
                 }
                 cl = cr = ~0u;
                 for (int hp = h1 - 1; hp >= h0; --hp) {
-                    uint32_t key = s_H[hp] & 0xFFFF0000u;
+                    uint32_t key = uint32_t(s_K[hp]) << 16;
                     cl = min(cl, key | uint32_t(hp));
                     cr = min(cr, key | uint32_t(0xFFFF - hp));
                     s_SL[hp] = uint16_t(cl), s_SR[hp] = uint16_t(0xFFFF - (cr & 0xFFFF));
@@ -487,10 +487,10 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
                 int e = hj + w - 1;
                 if (2 * int(tg) > k) {  // leftmost minimum (suffix candidate is the left one)
                     uint32_t c1 = s_SL[hj], c2 = s_PL[e];
-                    pos = uint16_t((s_H[c1] >> 16) <= (s_H[c2] >> 16) ? c1 : c2);
+                    pos = uint16_t(s_K[c1] <= s_K[c2] ? c1 : c2);
                 } else {  // rightmost minimum (prefix candidate is the right one)
                     uint32_t c1 = s_SR[hj], c2 = s_PR[e];
-                    pos = uint16_t((s_H[c2] >> 16) <= (s_H[c1] >> 16) ? c2 : c1);
+                    pos = uint16_t(s_K[c2] <= s_K[c1] ? c2 : c1);
                 }
             }
             s_mpos[hj] = pos;
