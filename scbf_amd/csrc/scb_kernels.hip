@@ -48,6 +48,7 @@ constexpr int kSpanWords = kSpan / 32 + 2;
 constexpr int kPkWords = kSpan / 16 + 8;
 constexpr int kHashN = kTile + 64;
 constexpr int kWinN = kTile + 32;
+static_assert(kWinN <= kHashN, "window positions reuse a hash-position array");
 constexpr int kChunksMax = kSpan / 16;
 constexpr int kGPad = kSpan + kSpan / 16;  // padded XOR-prefix array length
 constexpr int kGArr = 2 * kGPad > 2 * kHashN ? 2 * kGPad : 2 * kHashN;  // prefixes, then four u16 arrays
@@ -324,7 +325,6 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
     // Gf | Gr (padded: index li + li/16 keeps the 16-per-thread writes
     // bank-conflict free), then the four block prefix/suffix minimum arrays
     __shared__ uint32_t s_G[kGArr];
-    __shared__ uint16_t s_mpos[kWinN];
     __shared__ uint32_t s_wx[2][kTileThreads / 64];
     __shared__ uint32_t s_of[2][kChunksMax];  // XOR prefix of all chunks before chunk c
     __shared__ uint32_t s_wc[kTileThreads / 64];
@@ -343,6 +343,9 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
     // positions of the leftmost / rightmost minimum of every block prefix and suffix
     uint16_t *s_PL = reinterpret_cast<uint16_t *>(s_G);
     uint16_t *s_SL = s_PL + kHashN, *s_PR = s_PL + 2 * kHashN, *s_SR = s_PL + 3 * kHashN;
+    // minimizer position per window overwrites s_SL in place: window hj reads
+    // s_SL only at index hj before writing it
+    uint16_t *s_mpos = s_SL;
     const int NH = kTile + 2 * w - 1;
     uint64_t acc_valid = 0, acc_smers = 0, acc_pos = 0, acc_sum = 0;
 
