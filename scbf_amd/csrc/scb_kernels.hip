@@ -201,14 +201,21 @@ __device__ __forceinline__ uint32_t load_ctr(const void *base, uint64_t idx) {
 // Streams the bases of one super-k-mer (from 4 pre-aligned words holding its
 // first 64 bases) and calls f(j, canonical s-mer) for each of its s-mers in
 // position order (filter.rs:224-226, 423-426 with canonical_kmer 622-627).
+// The first s-mer is read directly from the words; the rest roll one base at a
+// time.
 template <class F>
 __device__ __forceinline__ void for_each_smer(uint32_t A0, uint32_t A1, uint32_t A2, uint32_t A3,
                                               uint32_t nbases, uint32_t s, F &&f) {
     const uint64_t mask = kmask(s);
     const uint32_t top = 2 * (s - 1);
-    uint64_t fw = 0, rc = 0;
-    uint32_t cur = A0;
-    for (uint32_t i = 0; i < nbases; ++i) {
+    uint64_t fw = ((uint64_t(A1) << 32) | A0) & mask;
+    uint64_t rc = revcomp(fw, s);
+    f(0u, fw < rc ? fw : rc);
+    // word stream positioned at base s (s is uniform across the wave)
+    if (s >= 16) A0 = A1, A1 = A2, A2 = A3;
+    if (s >= 32) A0 = A1, A1 = A2;
+    uint32_t cur = A0 >> (2 * (s & 15));
+    for (uint32_t i = s; i < nbases; ++i) {
         uint32_t c = cur & 3;
         cur >>= 2;
         if ((i & 15) == 15) {
@@ -218,7 +225,7 @@ __device__ __forceinline__ void for_each_smer(uint32_t A0, uint32_t A1, uint32_t
         }
         fw = (fw >> 2) | (uint64_t(c) << top);
         rc = ((rc << 2) | (c ^ 2)) & mask;
-        if (i + 1 >= s) f(i + 1 - s, fw < rc ? fw : rc);
+        f(i + 1 - s, fw < rc ? fw : rc);
     }
 }
 
