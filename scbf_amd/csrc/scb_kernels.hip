@@ -653,13 +653,31 @@ __global__ __launch_bounds__(1024) void k_bin_plan(const uint32_t *bin_count, ui
                                                    uint64_t *stats) {
     __shared__ uint64_t s_rec[1024], s_items[1024];
     const int tid = threadIdx.x;
-    uint64_t chunk = (nbins + 1023) / 1024;
-    uint64_t b0 = tid * chunk, b1 = min(nbins, b0 + chunk);
+    const uint64_t chunk = (nbins + 1023) / 1024;
+    const uint64_t b0 = tid * chunk, b1 = min(nbins, b0 + chunk);
+    // counts of this thread's contiguous run, loaded 4 at a time (8 loads in flight)
+    const bool vec = (chunk & 3) == 0 && nbins % 1024 == 0;
     uint64_t rsum = 0, isum = 0;
-    for (uint64_t b = b0; b < b1; ++b) {
-        uint32_t c = bin_count[b];
-        rsum += c;
-        isum += (c + per_item - 1) / per_item;
+    if (vec) {
+        const uint4 *cv = reinterpret_cast<const uint4 *>(bin_count + b0);
+        for (uint64_t q0 = 0; q0 < chunk / 4; q0 += 8) {
+            uint4 x[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) x[u] = cv[min<uint64_t>(q0 + u, chunk / 4 - 1)];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (q0 + u >= chunk / 4) break;
+                uint32_t c[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) rsum += c[e], isum += (c[e] + per_item - 1) / per_item;
+            }
+        }
+    } else {
+        for (uint64_t b = b0; b < b1; ++b) {
+            uint32_t c = bin_count[b];
+            rsum += c;
+            isum += (c + per_item - 1) / per_item;
+        }
     }
     s_rec[tid] = rsum, s_items[tid] = isum;
     __syncthreads();
@@ -671,7 +689,7 @@ __global__ __launch_bounds__(1024) void k_bin_plan(const uint32_t *bin_count, ui
     }
     uint64_t ro = s_rec[tid] - rsum, io = s_items[tid] - isum;
     for (uint64_t b = b0; b < b1; ++b) {
-        uint32_t c = bin_count[b];
+        uint32_t c = bin_count[b];  // L2-resident re-read
         bin_off[b] = uint32_t(ro);
         cursor[b] = uint32_t(ro);
         uint32_t parts = (c + per_item - 1) / per_item;
@@ -707,6 +725,7 @@ struct SegIn {
 // coarse bucket of the tile's first record (tile_based) or 0.
 struct KeySpec {
     uint32_t bin_shift, key_shift, fine_bits, tile_based;
+    uint32_t copies;  // counters per key (spreads contended coarse counters); 1 for bins
 };
 
 __device__ __forceinline__ uint32_t rec_bin(uint64_t r, uint32_t bin_shift) {
@@ -750,26 +769,26 @@ __global__ __launch_bounds__(kPartThreads) void k_seg_hist(SegIn in, KeySpec ks,
             }
         }
         __syncthreads();
+        const uint32_t copy = uint32_t(u % ks.copies);
         for (uint32_t i = tid; i < kPartRange; i += kPartThreads) {
-            if (s_h[i]) atomicAdd(&hist[lo + i], s_h[i]);
+            if (s_h[i]) atomicAdd(&hist[(lo + i) * ks.copies + copy], s_h[i]);
             if (kmers && s_k[i]) atomicAdd(&kmers[lo + i], s_k[i]);
         }
         __syncthreads();
     }
 }
 
-// Exclusive scan of a small histogram (n <= 2048) into cursors; the total
-// becomes the record count.
+// Exclusive scan of a small histogram (n <= 16384) into cursors; the total
+// becomes the record count. Each thread scans a contiguous run of n/1024.
 __global__ __launch_bounds__(1024) void k_small_scan(const uint32_t *hist, uint32_t n, uint32_t *cursor,
                                                      uint64_t *stats, uint32_t *total_out) {
-    __shared__ uint32_t s[2048];
-    const uint32_t tid = threadIdx.x;
-    for (uint32_t i = tid; i < 2048; i += 1024) s[i] = i < n ? hist[i] : 0;
-    __syncthreads();
-    // two elements per thread: Hillis-Steele over 1024 pair sums
-    uint32_t a0 = s[2 * tid], a1 = s[2 * tid + 1];
     __shared__ uint32_t p[1024];
-    p[tid] = a0 + a1;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t per = (n + 1023) / 1024;
+    const uint32_t b0 = tid * per, b1 = min(n, b0 + per);
+    uint32_t sum = 0;
+    for (uint32_t i = b0; i < b1; ++i) sum += hist[i];
+    p[tid] = sum;
     __syncthreads();
     for (uint32_t d = 1; d < 1024; d <<= 1) {
         uint32_t v = tid >= d ? p[tid - d] : 0;
@@ -777,9 +796,11 @@ __global__ __launch_bounds__(1024) void k_small_scan(const uint32_t *hist, uint3
         p[tid] += v;
         __syncthreads();
     }
-    uint32_t ex = p[tid] - a0 - a1;
-    if (2 * tid < n) cursor[2 * tid] = ex;
-    if (2 * tid + 1 < n) cursor[2 * tid + 1] = ex + a0;
+    uint32_t run = p[tid] - sum;
+    for (uint32_t i = b0; i < b1; ++i) {
+        cursor[i] = run;
+        run += hist[i];
+    }
     if (tid == 1023) {
         stats[kStRecords] = p[1023];
         *total_out = p[1023];
@@ -821,9 +842,10 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(SegIn in, KeySpec ks
             }
         }
         __syncthreads();
+        const uint32_t copy = uint32_t(u % ks.copies);
         for (uint32_t kk = tid; kk < kPartRange; kk += kPartThreads) {
             uint32_t c = s_hist[kk];
-            if (c) s_hist[kk] = atomicAdd(&cursor[kk + lo], c);
+            if (c) s_hist[kk] = atomicAdd(&cursor[(kk + lo) * ks.copies + copy], c);
         }
         __syncthreads();
 #pragma unroll
@@ -1575,7 +1597,7 @@ Status batched_front(Launch &L, const DeviceBatch &b, uint32_t bin_log, uint32_t
     Status s = launch_scan<kModeRecords>(L, b, seg_cap, valid_bits, nullptr);
     if (!s.good()) return s;
     mark(ws, 1);
-    const int pgrid = cu_count(L.device) * 8;
+    const int pgrid = cu_count(L.device) * 16;
     const dim3 pg(pgrid), pb(kPartThreads);
     SegIn segs{ws.rec.p, ws.seg_count.p, kSeg, seg_cap};
     if (nbins > kPartRange) {
@@ -1583,23 +1605,24 @@ Status batched_front(Launch &L, const DeviceBatch &b, uint32_t bin_log, uint32_t
         const uint32_t lb = uint32_t(__builtin_ctzll(nbins));
         const uint32_t fine_bits = std::max<uint32_t>(7, lb > 11 ? lb - 11 : 0);
         const uint32_t ncoarse = uint32_t(nbins >> fine_bits);
-        SCB_HIP(ws.coarse_hist.reserve(ncoarse));
-        SCB_HIP(ws.ccursor.reserve(ncoarse));
-        fill(ws.coarse_hist.p, ncoarse, 0, st);
-        KeySpec coarse{bin_shift, fine_bits, 0, 0};
+        constexpr uint32_t kCopies = 8;  // coarse counters are hot: spread each over 8 copies
+        SCB_HIP(ws.coarse_hist.reserve(ncoarse * kCopies));
+        SCB_HIP(ws.ccursor.reserve(ncoarse * kCopies));
+        fill(ws.coarse_hist.p, ncoarse * kCopies, 0, st);
+        KeySpec coarse{bin_shift, fine_bits, 0, 0, kCopies};
         hipLaunchKernelGGL(k_seg_hist, pg, pb, 0, st, segs, coarse, ws.coarse_hist.p, (uint32_t *)nullptr);
-        hipLaunchKernelGGL(k_small_scan, dim3(1), dim3(1024), 0, st, ws.coarse_hist.p, ncoarse, ws.ccursor.p,
-                           ws.stats.p, ws.seg_count.p + kSeg);
+        hipLaunchKernelGGL(k_small_scan, dim3(1), dim3(1024), 0, st, ws.coarse_hist.p, ncoarse * kCopies,
+                           ws.ccursor.p, ws.stats.p, ws.seg_count.p + kSeg);
         hipLaunchKernelGGL(k_partition, pg, pb, 0, st, segs, coarse, ws.ccursor.p, ws.sorted.p);
         SegIn comp{ws.sorted.p, ws.seg_count.p + kSeg, 1, cap};
-        KeySpec fine{bin_shift, 0, fine_bits, 1};
+        KeySpec fine{bin_shift, 0, fine_bits, 1, 1};
         hipLaunchKernelGGL(k_seg_hist, pg, pb, 0, st, comp, fine, ws.bin_count.p, bin_kmers);
         hipLaunchKernelGGL(k_bin_plan, dim3(1), dim3(1024), 0, st, ws.bin_count.p, nbins, ws.bin_off.p,
                            ws.cursor.p, ws.work.p, per_item, ws.stats.p);
         hipLaunchKernelGGL(k_partition, pg, pb, 0, st, comp, fine, ws.cursor.p, ws.rec.p);
         ws.final_sorted = ws.rec.p;
     } else {
-        KeySpec one{bin_shift, 0, 0, 0};
+        KeySpec one{bin_shift, 0, 0, 0, 1};
         hipLaunchKernelGGL(k_seg_hist, pg, pb, 0, st, segs, one, ws.bin_count.p, bin_kmers);
         hipLaunchKernelGGL(k_bin_plan, dim3(1), dim3(1024), 0, st, ws.bin_count.p, nbins, ws.bin_off.p,
                            ws.cursor.p, ws.work.p, per_item, ws.stats.p);
