@@ -115,6 +115,35 @@ def cpu_baseline(ins_b, ins_o, q_b, q_o, target_s=15.0):
             "host_cpus": os.cpu_count(), "cpu_model": model}
 
 
+PMC_PROFILE = os.path.join(ROOT, "profiles", "pmc_default_geometry.md")
+
+
+def pmc_figures():
+    """Per-launch HBM traffic and VALU figures of the kernels, read from the JSON block of the
+    committed rocprofv3 --pmc summary (FETCH_SIZE / WRITE_SIZE scaled by a copy kernel of known
+    size). Counters cannot be read live without the profiler, so these come from that file."""
+    try:
+        with open(PMC_PROFILE) as f:
+            text = f.read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return json.loads(block)
+    except (OSError, IndexError, ValueError):
+        return None
+
+
+def copy_probe(mib=1024, reps=3):
+    """Device-to-device copy of a known size: the calibration kernel for FETCH_SIZE/WRITE_SIZE."""
+    import torch
+    n = mib << 20
+    src = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(7)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,11 +152,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 1 GiB device copy (calibrates the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
 
     import torch
     import scbf_amd as scb
@@ -201,7 +235,9 @@ def main():
             log(f"batched vs plain bit-exact: {bit_exact}")
         del idx_p
 
-    # roofline: algorithmic bytes (SURVEY 8(d) read-modify-write model)
+    # roofline: algorithmic bytes, read-modify-write model of the compulsory traffic:
+    # insert = bases + 2*64 B per distinct counter line touched; query = bases + 64 B per
+    # distinct line read + W bytes per abundance + 1 bit of validity per k-mer.
     W = 2
     L_touch = lines_touched(counters_b)
     n_sk_i = (ir.inserted_smers - ir.inserted_kmers) // (31 - 25)
@@ -225,6 +261,12 @@ def main():
             "pipeline_frac": round((b_insert_pipe + b_query_pipe) / (pipe_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
             "bytes_per_kmer": round((b_insert_pipe + b_query_pipe) / kmers, 2),
             "lines_touched": L_touch}
+    pmc = pmc_figures()
+    if pmc and dom in pmc.get("kernels_by_stage", {}):
+        kp = pmc["kernels"][pmc["kernels_by_stage"][dom]]
+        roof["traffic"] = int(kp["hbm_bytes"])
+        roof["traffic_source"] = os.path.relpath(PMC_PROFILE, ROOT)
+        roof["valu_busy"] = kp.get("valu_busy")
 
     cpu = None
     if not args.no_cpu:

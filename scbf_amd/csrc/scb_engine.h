@@ -105,10 +105,16 @@ class Engine {
 
    private:
     Engine() = default;
+    // A new table is not cleared when it is allocated: the first batched insert
+    // writes every counter word of the table (merging its counts into zero
+    // instead of reading the table), and any other first use clears it.
+    Status ensure_zeroed() const;
     Geometry geo_;
     Tuning tune_;
     void *d_table_ = nullptr;
     int device_ = 0;
+    mutable std::mutex zero_mu_;
+    mutable bool zero_pending_ = false;
     std::unique_ptr<Workspace> acquire() const;
     void release(std::unique_ptr<Workspace> ws) const;
 };

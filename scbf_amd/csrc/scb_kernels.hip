@@ -919,7 +919,26 @@ struct BinArgs {
     void *out;
     uint32_t bin_log;      // slots per bin = 2^bin_log
     uint32_t per_item;
+    uint32_t fresh;        // insert into a table whose contents are undefined (see k_zero_cold)
 };
+
+// Before the first insert into an uncleared table: zero the bins that the
+// insert kernel will not write whole, i.e. bins without records and bins split
+// over several work items (those merge with atomics into the table).
+__global__ __launch_bounds__(256) void k_zero_cold(const uint32_t *bin_count, uint64_t nbins, uint32_t per_item,
+                                                   uint32_t *table, uint32_t words_per_bin) {
+    for (uint64_t b = blockIdx.x; b < nbins; b += gridDim.x) {
+        uint32_t c = bin_count[b];
+        if (c != 0 && c <= per_item) continue;
+        uint32_t *p = table + b * words_per_bin;
+        if ((words_per_bin & 3) == 0) {
+            uint4 *v = reinterpret_cast<uint4 *>(p);
+            for (uint32_t i = threadIdx.x; i < words_per_bin / 4; i += blockDim.x) v[i] = make_uint4(0, 0, 0, 0);
+        } else {
+            for (uint32_t i = threadIdx.x; i < words_per_bin; i += blockDim.x) p[i] = 0;
+        }
+    }
+}
 
 // filter.rs:199-237 for every super-k-mer of one bin: n_hashes saturating +1
 // per s-mer, counted in LDS (u32 per slot) and merged once as
@@ -977,14 +996,19 @@ __global__ __launch_bounds__(1024) void k_bin_insert(BinArgs a, DevGeo g) {
             const uint32_t nvec = nwords >> 2;
             for (uint32_t v0 = tid; v0 < nvec; v0 += nthr * kU) {
                 uint4 old[kU];
+                if (a.fresh) {  // the table holds no counts yet: write every word
 #pragma unroll
-                for (int u = 0; u < kU; ++u) old[u] = tv[min(v0 + u * nthr, nvec - 1)];  // all in flight
+                    for (int u = 0; u < kU; ++u) old[u] = make_uint4(0, 0, 0, 0);
+                } else {
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) old[u] = tv[min(v0 + u * nthr, nvec - 1)];  // all in flight
+                }
 #pragma unroll
                 for (int u = 0; u < kU; ++u) {
                     uint32_t vi = v0 + u * nthr;
                     if (vi >= nvec) continue;
                     uint32_t c[4 * per];
-                    uint32_t any = 0;
+                    uint32_t any = a.fresh;
 #pragma unroll
                     for (uint32_t q = 0; q < per; ++q) {
                         uint4 x = cv[vi * per + q];
@@ -1003,12 +1027,12 @@ __global__ __launch_bounds__(1024) void k_bin_insert(BinArgs a, DevGeo g) {
         } else {
             for (uint32_t wi = tid; wi < nwords; wi += nthr) {
                 uint32_t c[per];
-                uint32_t any = 0;
+                uint32_t any = !multi && a.fresh;
 #pragma unroll
                 for (uint32_t q = 0; q < per; ++q) c[q] = s_cnt[wi * per + q], any |= c[q];
                 if (!any) continue;
                 if (!multi) {
-                    tw[wi] = sat_merge_word<BITS>(tw[wi], c);
+                    tw[wi] = sat_merge_word<BITS>(a.fresh ? 0u : tw[wi], c);
                 } else {
 #pragma unroll
                     for (uint32_t q = 0; q < per; ++q)
@@ -1445,14 +1469,73 @@ int device_count() {
     return n;
 }
 
+// Counter tables of freed engines are kept for reuse by the next engine of
+// the same size on the same device (a builder per batch of work would
+// otherwise pay hipMalloc/hipFree of the whole table every time). At most
+// kTableCacheMax tables are held; an allocation that fails drops the cache and
+// retries.
+namespace {
+constexpr size_t kTableCacheMax = 2;
+struct TableCache {
+    std::mutex mu;
+    struct Entry {
+        int device;
+        uint64_t bytes;
+        void *p;
+    };
+    std::vector<Entry> free;
+};
+TableCache &table_cache() {
+    static TableCache *c = new TableCache;  // never destroyed, like the workspace pool
+    return *c;
+}
+hipError_t table_alloc(int device, uint64_t bytes, void **out) {
+    TableCache &c = table_cache();
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        for (size_t i = c.free.size(); i-- > 0;) {
+            if (c.free[i].device == device && c.free[i].bytes == bytes) {
+                *out = c.free[i].p;
+                c.free.erase(c.free.begin() + i);
+                return hipSuccess;
+            }
+        }
+    }
+    hipError_t e = hipMalloc(out, bytes);
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+        (void)hipGetLastError();
+        std::vector<TableCache::Entry> drop;
+        {
+            std::lock_guard<std::mutex> lk(c.mu);
+            drop.swap(c.free);
+        }
+        for (auto &d : drop) (void)hipFree(d.p);
+        e = hipMalloc(out, bytes);
+    }
+    return e;
+}
+void table_release(int device, uint64_t bytes, void *p) {
+    TableCache &c = table_cache();
+    void *evict = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        c.free.push_back({device, bytes, p});
+        if (c.free.size() > kTableCacheMax) {
+            evict = c.free.front().p;
+            c.free.erase(c.free.begin());
+        }
+    }
+    if (evict) (void)hipFree(evict);
+}
+}  // namespace
+
 Status Engine::create(const Geometry &g, std::unique_ptr<Engine> &out) {
     if (device_count() <= 0) return Status{8, "device error: no HIP device is visible"};
     std::unique_ptr<Engine> e(new Engine);
     e->geo_ = g;
     SCB_HIP(hipGetDevice(&e->device_));
-    SCB_HIP(hipMalloc(&e->d_table_, std::max<uint64_t>(g.table_bytes(), 4)));
-    SCB_HIP(hipMemset(e->d_table_, 0, g.table_bytes()));
-    SCB_HIP(hipDeviceSynchronize());
+    SCB_HIP(table_alloc(e->device_, std::max<uint64_t>(g.table_bytes(), 4), &e->d_table_));
+    e->zero_pending_ = true;
     // keep query bins within LDS at every width (u32 counters: 2^15 slots = 128 KiB)
     e->tune_.insert_bin_log = std::min<uint32_t>(15, g.cse);
     e->tune_.query_bin_log = std::min<uint32_t>(g.bits == 8 ? 15 : 15, g.cse);
@@ -1461,7 +1544,18 @@ Status Engine::create(const Geometry &g, std::unique_ptr<Engine> &out) {
 }
 
 Engine::~Engine() {
-    if (d_table_) (void)hipFree(d_table_);
+    // every call on the engine has synchronized its stream before returning
+    if (d_table_) table_release(device_, std::max<uint64_t>(geo_.table_bytes(), 4), d_table_);
+}
+
+Status Engine::ensure_zeroed() const {
+    std::lock_guard<std::mutex> lk(zero_mu_);
+    if (!zero_pending_) return Status::ok();
+    SCB_HIP(hipSetDevice(device_));
+    SCB_HIP(hipMemset(d_table_, 0, std::max<uint64_t>(geo_.table_bytes(), 4)));
+    SCB_HIP(hipDeviceSynchronize());
+    zero_pending_ = false;
+    return Status::ok();
 }
 
 // Scratch workspaces are pooled per device for the whole process, so a new
@@ -1505,6 +1599,8 @@ void Engine::release(std::unique_ptr<Workspace> ws) const {
 
 Status Engine::download(void *host, uint64_t bytes) const {
     if (bytes != geo_.table_bytes()) return Status{6, "internal state error: table size mismatch"};
+    Status z = ensure_zeroed();
+    if (!z.good()) return z;
     SCB_HIP(hipMemcpy(host, d_table_, bytes, hipMemcpyDeviceToHost));
     return Status::ok();
 }
@@ -1512,6 +1608,8 @@ Status Engine::download(void *host, uint64_t bytes) const {
 Status Engine::upload(const void *host, uint64_t bytes) {
     if (bytes != geo_.table_bytes()) return Status{6, "internal state error: table size mismatch"};
     SCB_HIP(hipMemcpy(d_table_, host, bytes, hipMemcpyHostToDevice));
+    std::lock_guard<std::mutex> lk(zero_mu_);
+    zero_pending_ = false;
     return Status::ok();
 }
 
@@ -1673,11 +1771,21 @@ Status Engine::insert(const DeviceBatch &b, Method method, InsertResult &r) {
     if (!s.good()) return s;
     if (b.n_bases < geo_.k || b.n_reads == 0) return Status::ok();
     SCB_HIP(hipSetDevice(device_));
+    bool batched = method == kBatched && batched_ok(geo_, tune_.insert_bin_log);
+    bool fresh = false;
+    {
+        std::lock_guard<std::mutex> lk(zero_mu_);
+        fresh = zero_pending_;
+    }
+    if (fresh && !batched) {
+        s = ensure_zeroed();
+        if (!s.good()) return s;
+        fresh = false;
+    }
     auto ws = acquire();
     if (!ws) return Status{8, "device error: cannot create a workspace"};
     Launch L{geo_, dev_geo(geo_), *ws, d_table_, device_};
     hipStream_t st = ws->stream;
-    bool batched = method == kBatched && batched_ok(geo_, tune_.insert_bin_log);
     if (!batched) {
         fill(reinterpret_cast<uint32_t *>(ws->stats.p), 2 * kStatsLen, 0, st);
         mark(*ws, 0);
@@ -1698,6 +1806,13 @@ Status Engine::insert(const DeviceBatch &b, Method method, InsertResult &r) {
             a.table = d_table_;
             a.bin_log = bin_log;
             a.per_item = tune_.max_records_per_item;
+            a.fresh = fresh;
+            if (fresh) {
+                const uint32_t words_per_bin = uint32_t(((uint64_t(1) << bin_log) * geo_.bytes) / 4);
+                hipLaunchKernelGGL(k_zero_cold, dim3(unsigned(std::min<uint64_t>(nbins, 8192))), dim3(256), 0, st,
+                                   ws->bin_count.p, nbins, a.per_item, reinterpret_cast<uint32_t *>(d_table_),
+                                   words_per_bin);
+            }
             size_t lds = (size_t(1) << bin_log) * 4;
             int thr = int(tune_.insert_threads);
 #define SCB_INS(BITS)                                                                           \
@@ -1721,6 +1836,10 @@ Status Engine::insert(const DeviceBatch &b, Method method, InsertResult &r) {
         }
     }
     if (s.good()) s = finish(*ws);
+    if (s.good() && fresh) {
+        std::lock_guard<std::mutex> lk(zero_mu_);
+        zero_pending_ = false;
+    }
     if (s.good()) {
         collect_times(*ws, batched);
         r.inserted_kmers = ws->h_stats[kStValid];
@@ -1735,6 +1854,8 @@ Status Engine::query(const DeviceBatch &b, Method method, void *d_values, uint32
                      QueryResult &r) const {
     r = QueryResult{};
     Status s = check_batch(geo_, b);
+    if (!s.good()) return s;
+    s = ensure_zeroed();
     if (!s.good()) return s;
     if (b.n_bases == 0 || b.n_reads == 0) return Status::ok();
     SCB_HIP(hipSetDevice(device_));
@@ -1806,6 +1927,8 @@ Status Engine::query(const DeviceBatch &b, Method method, void *d_values, uint32
 Status Engine::spectrum(const DeviceBatch &b, SpectrumResult &r) const {
     r = SpectrumResult{};
     Status s = check_batch(geo_, b);
+    if (!s.good()) return s;
+    s = ensure_zeroed();
     if (!s.good()) return s;
     if (b.n_bases < geo_.k || b.n_reads == 0) return Status::ok();
     if (geo_.nb > (uint64_t(1) << 27))
@@ -1917,6 +2040,8 @@ Status Engine::spectrum(const DeviceBatch &b, SpectrumResult &r) const {
 
 Status Engine::counter_histogram(std::vector<std::pair<uint64_t, uint64_t>> &rows) const {
     rows.clear();
+    Status z = ensure_zeroed();
+    if (!z.good()) return z;
     SCB_HIP(hipSetDevice(device_));
     auto ws = acquire();
     if (!ws) return Status{8, "device error: cannot create a workspace"};

@@ -117,7 +117,8 @@ def test_counter_histogram(gpu, oracle, bits):
 
 
 def test_c1_small_parity(gpu, oracle):
-    """configs[0] shape, reduced to 20k reads so the oracle stays fast."""
+    """1 Mbp genome, error-free 150 bp reads with random strand (the small CPU-runnable case),
+    reduced to 20k reads so the oracle stays fast."""
     scb = gpu
     k, m, s, h, bits, cse, bse = 31, 21, 25, 4, 16, 24, 9
     g = synth.genome(1_000_000, 11)
@@ -137,3 +138,38 @@ def test_c1_small_parity(gpu, oracle):
         assert np.array_equal(valid, wok) and np.array_equal(vals[wok], wv[wok])
         # no underestimation: every inserted k-mer has estimate >= 1
         assert (vals[wok] >= 1).all()
+
+
+@pytest.mark.parametrize("method", METHODS, ids=["batched", "plain"])
+def test_reused_table_memory_starts_from_zero(gpu, oracle, method):
+    """Counter tables are recycled between builders of the same geometry and are
+    not cleared at allocation: the first insert must define every counter, and a
+    table that is read before any insert must read as zero."""
+    scb = gpu
+    k, m, s, h, bits, cse, bse = 31, 21, 25, 4, 16, 20, 9
+    g = synth.genome(50_000, 31)
+    big_b, big_o = synth.fixed_reads(g, 4000, 150, 32)
+    small_b, small_o = synth.fixed_reads(g, 50, 150, 33)
+    for _ in range(2):  # leave dirty tables of this size behind
+        b = scb.SuperCountingBloomBuilder(_cfg(scb, k, m, s, h, bits, cse, bse))
+        b.add_sequences((big_b, big_o))
+        assert b.counters().any()
+        del b
+    # read before any insert
+    b = scb.SuperCountingBloomBuilder(_cfg(scb, k, m, s, h, bits, cse, bse))
+    assert not b.counters().any()
+    idx = b.into_index()
+    vals, valid, rep = idx.estimate_sequences(small_b, small_o)
+    assert rep.positive_estimates == 0 and not vals[valid].any()
+    del idx
+    # first insert is a small batch that touches few bins
+    b = scb.SuperCountingBloomBuilder(_cfg(scb, k, m, s, h, bits, cse, bse))
+    b.set_method(method)
+    b.add_sequences((small_b, small_o))
+    of = oracle.Filter(oracle.make_cfg(k, m, s, h, bits, cse, bse))
+    oracle.insert_records(of, small_b, small_o)
+    assert np.array_equal(b.counters().astype(np.uint32), of.counters())
+    # an insert that produces no k-mers leaves the table pending; the next read clears it
+    b = scb.SuperCountingBloomBuilder(_cfg(scb, k, m, s, h, bits, cse, bse))
+    b.add_sequences((np.frombuffer(b"ACGT", dtype=np.uint8), np.array([0, 4], dtype=np.uint64)))
+    assert not b.counters().any()
