@@ -144,6 +144,109 @@ def copy_probe(mib=1024, reps=3):
     print(json.dumps({"copy_probe_bytes": n, "reps": reps}), flush=True)
 
 
+def serve_bench(scb, idx, genome, qps_steps, seconds, seed=9):
+    """Open-loop serving of the C2 index: Poisson arrivals at each QPS step; request
+    sizes log-uniform (1-10,000 records of 31-20,000 bp; 70% genome substrings on a
+    random strand, 30% random); 80% abundance and 20% spectrum requests. Latency is
+    scheduled arrival -> completion (CLOCK_MONOTONIC), so submit lateness counts."""
+    import threading
+    from scbf_amd import synth
+    pool = synth.serving_requests(genome, 96, seed)
+    pinned = []
+    for b, o in pool:  # request bytes live in page-locked memory like a serving front end's
+        hb = scb.host_array(b.size, np.uint8)
+        hb[:] = b
+        pinned.append((hb, o))
+    W = np.dtype({8: np.uint8, 16: np.uint16, 32: np.uint32}[idx.config().counter_bits])
+    outs = [(scb.host_array(max(1, int(o[-1])), W), scb.host_array(max(1, (int(o[-1]) + 31) // 32), np.uint32))
+            for _, o in pool]
+    busy = [False] * len(pool)
+    rng = np.random.default_rng([seed, 1])
+    steps = []
+    with scb.SuperCountingBloomServer(idx) as srv:
+        # warm-up: one request of each kind
+        srv.wait(srv.submit_abundances(*pinned[0], *outs[0]))
+        srv.wait(srv.submit_spectrum(*pinned[0]))
+        for qps in qps_steps:
+            arrivals = {}
+            lat, kmers_done, refused = [], [0], [0]
+            lock = threading.Lock()
+            n_sub = [0]
+            stop = [False]
+
+            def collect():
+                while True:
+                    r = srv.wait_any(0.05)
+                    if r is None:
+                        with lock:
+                            if stop[0] and len(lat) + refused[0] >= n_sub[0]:
+                                return
+                        continue
+                    with lock:
+                        t_arr, pi, kind, owns = arrivals.pop(r.ticket)
+                        lat.append((r.done_ns - t_arr) / 1e6)
+                        kmers_done[0] += r.report.valid_kmers if kind == 0 else r.queried
+                        if owns:
+                            busy[pi] = False
+
+            th = threading.Thread(target=collect)
+            th.start()
+            t0 = time.monotonic_ns()
+            t_next = t0
+            end = t0 + int(seconds * 1e9)
+            while True:
+                t_next += int(rng.exponential(1e9 / qps))
+                if t_next >= end:
+                    break
+                now = time.monotonic_ns()
+                if t_next > now:
+                    time.sleep((t_next - now) / 1e9)
+                pi = int(rng.integers(0, len(pool)))
+                kind = 0 if rng.random() < 0.8 else 1
+                owns = False
+                try:
+                    with lock:
+                        if kind == 0:
+                            if busy[pi]:
+                                nb = int(pool[pi][1][-1])
+                                vo, bo = np.empty(max(1, nb), W), np.empty(max(1, (nb + 31) // 32), np.uint32)
+                            else:
+                                vo, bo = outs[pi]
+                                busy[pi] = owns = True
+                            t = srv.submit_abundances(*pinned[pi], vo, bo)
+                        else:
+                            t = srv.submit_spectrum(*pinned[pi])
+                        arrivals[t] = (t_next, pi, kind, owns)
+                        n_sub[0] += 1
+                except scb.SuperCountingBloomError:
+                    with lock:
+                        refused[0] += 1
+                        n_sub[0] += 1
+                        if owns:
+                            busy[pi] = False
+            with lock:
+                stop[0] = True
+            th.join()
+            wall = (time.monotonic_ns() - t0) / 1e9
+            a = np.array(lat)
+            steps.append({"qps": qps, "requests": int(a.size), "refused": refused[0],
+                          "p50_ms": round(float(np.percentile(a, 50)), 3) if a.size else None,
+                          "p99_ms": round(float(np.percentile(a, 99)), 3) if a.size else None,
+                          "max_ms": round(float(a.max()), 3) if a.size else None,
+                          "served_G_kmers_per_s": round(kmers_done[0] / wall / 1e9, 4)})
+            log(f"served qps={qps}: {steps[-1]}")
+        # spot check: served abundances equal the direct call
+        b, o = pool[1]
+        s = srv.wait(srv.submit_abundances(b, o))
+        vals, valid, rep = idx.estimate_sequences(b, o)
+        ok = bool(np.array_equal(s.valid, valid) and np.array_equal(s.values[:b.size].astype(np.uint64), vals)
+                  and s.report == rep)
+    mean_bp = float(np.mean([int(o[-1]) for _, o in pool]))
+    return {"steps": steps, "seconds_per_step": seconds, "request_pool": len(pool),
+            "mean_request_bases": round(mean_bp), "mix": "80% abundances, 20% spectrum",
+            "served_equals_direct": ok}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,6 +255,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-qps", default="100,400")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -209,6 +315,11 @@ def main():
                                        out_bits.data_ptr())
     counters_b = idx_b.counters()
     vals_b, bits_b = out_vals.clone(), out_bits.clone()
+    served = None
+    if not args.no_serve:
+        from scbf_amd import synth
+        served = serve_bench(scb, idx_b, synth.genome(5_000_000, 1),
+                             [float(q) for q in args.serve_qps.split(",")], args.serve_seconds)
     del idx_b
 
     plain = None
@@ -289,6 +400,9 @@ def main():
         "batched_over_plain": round(value / plain["value"], 2) if plain else None,
         "bit_exact_batched_vs_plain": bit_exact,
         "roofline": roof,
+        "served": served,
+        "served_p50_ms": served["steps"][0]["p50_ms"] if served else None,
+        "served_p99_ms": served["steps"][0]["p99_ms"] if served else None,
         "cpu_baseline": cpu,
         "host": platform.node(),
     }

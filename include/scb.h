@@ -45,7 +45,8 @@ enum {
     SCB_ERR_WORKER_PANIC = 5,
     SCB_ERR_INTERNAL_STATE = 6,
     SCB_ERR_IO = 7,
-    SCB_ERR_DEVICE = 8
+    SCB_ERR_DEVICE = 8,
+    SCB_ERR_BUSY = 9 /* serving loop: the request queue is full (admission control) */
 };
 
 /* Insert/query execution method (extension). BATCHED: super-k-mers binned by
@@ -242,6 +243,62 @@ int scb_write_spectrum(const uint64_t *values, const uint64_t *counts, size_t n_
  * record (keep_non_actg); a nonzero return stops the walk and is returned. */
 typedef int (*scb_record_visitor)(void *user, const char *name, const uint8_t *seq, size_t len);
 int scb_for_each_fastx_record(const char *path, scb_record_visitor visit, void *user);
+
+/* ---- serving loop --------------------------------------------------------
+ * No counterpart in the reference. A frozen index stays resident in HBM and
+ * answers concurrent requests: worker threads coalesce queued abundance
+ * requests into one device launch (up to max_batch_bases / max_batch_requests),
+ * run spectrum requests one per launch, and refuse new requests with
+ * SCB_ERR_BUSY once max_pending are queued. Each request's results equal
+ * scb_index_estimate_sequences / scb_index_spectrum on the same records
+ * (api.rs:250-262, 273-327; fastx.rs:133-170 per request). */
+typedef struct scb_server scb_server;
+
+enum { SCB_REQ_ABUNDANCES = 0, SCB_REQ_SPECTRUM = 1 };
+
+typedef struct scb_serve_config {
+    uint64_t max_batch_bases;    /* bases coalesced into one launch (default 2^27) */
+    uint64_t max_batch_requests; /* requests coalesced into one launch (default 1024) */
+    uint64_t hbm_budget_bytes;   /* device memory for staging and pipeline scratch of all
+                                    workers (default 16 GiB); max_batch_bases is lowered to fit */
+    uint32_t workers;            /* launches in flight at once (default 2) */
+    uint32_t max_pending;        /* queued requests before submit answers SCB_ERR_BUSY (4096) */
+} scb_serve_config;
+
+typedef struct scb_served {
+    int status;                   /* SCB_OK or the error code of this request */
+    uint32_t kind;                /* SCB_REQ_ABUNDANCES or SCB_REQ_SPECTRUM */
+    scb_query_report report;      /* abundances: as scb_index_estimate_sequences */
+    uint64_t n_rows, queried, distinct;  /* spectrum: as scb_index_spectrum */
+    uint64_t submit_ns, start_ns, done_ns;  /* CLOCK_MONOTONIC nanoseconds */
+    uint64_t batch_requests;      /* requests served by the same launch */
+    char message[192];            /* error text when status != SCB_OK */
+} scb_served;
+
+void scb_serve_config_default(scb_serve_config *cfg);
+/* The index must outlive the server. */
+int scb_server_new(const scb_index *idx, const scb_serve_config *cfg, scb_server **out);
+/* Finishes the queued requests, then stops the workers. */
+void scb_server_free(scb_server *srv);
+/* values: total_bases counters of counter_bits/8 bytes, indexed by position over
+ * the concatenated records, 0 where no valid k-mer window starts; valid_bits:
+ * (total_bases + 31) / 32 words, bit p % 32 of word p / 32 set for a valid
+ * window at p. Inputs and outputs must stay valid until the ticket is waited
+ * for. Records longer than max_batch_bases - 64 are refused. */
+int scb_server_submit_abundances(scb_server *srv, const uint8_t *bases, const uint64_t *offsets,
+                                 size_t n, void *values, uint32_t *valid_bits, uint64_t *ticket);
+/* Spectrum rows (value, #distinct keys) as scb_index_spectrum. */
+int scb_server_submit_spectrum(scb_server *srv, const uint8_t *bases, const uint64_t *offsets,
+                               size_t n, uint64_t *values, uint64_t *counts, size_t cap,
+                               uint64_t *ticket);
+/* Blocks until the request has finished; each ticket is waited for once. */
+int scb_server_wait(scb_server *srv, uint64_t ticket, scb_served *out);
+/* Waits up to timeout_us for any finished request not yet waited for; *ticket
+ * is 0 when none finished in time. */
+int scb_server_wait_any(scb_server *srv, uint64_t timeout_us, uint64_t *ticket, scb_served *out);
+/* Page-locked host memory for request buffers (faster host<->HBM copies). */
+void *scb_host_alloc(size_t bytes);
+void scb_host_free(void *p);
 
 /* ---- low-level primitives (filter.rs:622-635) ----------------------------- */
 uint64_t scb_xorshift_u64(uint64_t x);

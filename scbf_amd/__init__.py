@@ -1,12 +1,13 @@
 """MI355X-native SuperCountingBloom engine: hand-written gfx950 HIP kernels
 behind a C ABI (include/scb.h, scbf_amd/libscb.so), with this Python mirror of
 the super-counting-bloom crate API on top."""
-from .api import (METHOD_BATCHED, METHOD_PLAIN, AddReport, ApproxSpectrum, BuildIndexReport,
-                  CounterBits, FilterOnlySpectrumEstimate, QueryFastaReport,
-                  SequenceAbundanceSummary, StreamEstimateReport, StreamEstimateStats,
-                  SuperCountingBloomBuilder, SuperCountingBloomConfig, SuperCountingBloomError,
-                  SuperCountingBloomIndex, SuperCountingBloomReport, SuperCountingBloomStats,
+from .api import (METHOD_BATCHED, METHOD_PLAIN, REQ_ABUNDANCES, REQ_SPECTRUM, AddReport,
+                  ApproxSpectrum, BuildIndexReport, CounterBits, FilterOnlySpectrumEstimate,
+                  QueryFastaReport, SequenceAbundanceSummary, ServeConfig, Served,
+                  StreamEstimateReport, StreamEstimateStats, SuperCountingBloomBuilder,
+                  SuperCountingBloomConfig, SuperCountingBloomError, SuperCountingBloomIndex,
+                  SuperCountingBloomReport, SuperCountingBloomServer, SuperCountingBloomStats,
                   canonical_kmer, concat_records, device_count, estimate_spectrum_from_fastx,
-                  estimate_spectrum_from_fastx_pair, for_each_fastx_record,
+                  estimate_spectrum_from_fastx_pair, for_each_fastx_record, host_array,
                   stream_estimates_from_fastx, stream_estimates_from_fastx_pair, write_spectrum,
                   xorshift_u64)

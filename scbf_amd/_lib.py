@@ -55,6 +55,18 @@ class ScbSummary(C.Structure):
                 ("mean_abundance", C.c_double), ("median_abundance", C.c_double)]
 
 
+class ScbServeConfig(C.Structure):
+    _fields_ = [("max_batch_bases", C.c_uint64), ("max_batch_requests", C.c_uint64),
+                ("hbm_budget_bytes", C.c_uint64), ("workers", C.c_uint32), ("max_pending", C.c_uint32)]
+
+
+class ScbServed(C.Structure):
+    _fields_ = [("status", C.c_int), ("kind", C.c_uint32), ("report", ScbQueryReport),
+                ("n_rows", C.c_uint64), ("queried", C.c_uint64), ("distinct", C.c_uint64),
+                ("submit_ns", C.c_uint64), ("start_ns", C.c_uint64), ("done_ns", C.c_uint64),
+                ("batch_requests", C.c_uint64), ("message", C.c_char * 192)]
+
+
 VISITOR = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_char_p, u8p, C.c_size_t)
 
 # name -> (restype, argtypes)
@@ -111,6 +123,17 @@ SIGNATURES = {
                                                        C.POINTER(ScbStreamStats)]),
     "scb_write_spectrum": (C.c_int, [u64p, u64p, C.c_size_t, C.c_char_p]),
     "scb_for_each_fastx_record": (C.c_int, [C.c_char_p, VISITOR, C.c_void_p]),
+    "scb_serve_config_default": (None, [C.POINTER(ScbServeConfig)]),
+    "scb_server_new": (C.c_int, [C.c_void_p, C.POINTER(ScbServeConfig), C.POINTER(C.c_void_p)]),
+    "scb_server_free": (None, [C.c_void_p]),
+    "scb_server_submit_abundances": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
+                                               C.c_void_p, u64p]),
+    "scb_server_submit_spectrum": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
+                                             C.c_void_p, C.c_size_t, u64p]),
+    "scb_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(ScbServed)]),
+    "scb_server_wait_any": (C.c_int, [C.c_void_p, C.c_uint64, u64p, C.POINTER(ScbServed)]),
+    "scb_host_alloc": (C.c_void_p, [C.c_size_t]),
+    "scb_host_free": (None, [C.c_void_p]),
     "scb_xorshift_u64": (C.c_uint64, [C.c_uint64]),
     "scb_canonical_kmer": (C.c_uint64, [u8p, C.c_size_t, C.c_size_t, C.c_size_t]),
 }

@@ -10,15 +10,16 @@ from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
-from ._lib import (VISITOR, ScbAddReport, ScbBuildReport, ScbConfig, ScbQueryReport,
-                   ScbSpectrumStats, ScbStreamStats, ScbSummary, last_error, lib, u8p, u32p,
-                   u64p)
+from ._lib import (VISITOR, ScbAddReport, ScbBuildReport, ScbConfig, ScbQueryReport, ScbServeConfig,
+                   ScbServed, ScbSpectrumStats, ScbStreamStats, ScbSummary, last_error, lib, u8p,
+                   u32p, u64p)
 
 ApproxSpectrum = List[Tuple[int, int]]
 
-# error.rs:3-11 (+ Device, the HIP runtime failure kind of this engine)
+# error.rs:3-11 (+ Device, the HIP runtime failure kind of this engine, and Busy,
+# the admission refusal of the serving loop)
 ERROR_KINDS = {1: "InvalidConfig", 2: "FastxOpen", 3: "InvalidIndexFormat", 4: "ChannelClosed",
-               5: "WorkerPanic", 6: "InternalState", 7: "Io", 8: "Device"}
+               5: "WorkerPanic", 6: "InternalState", 7: "Io", 8: "Device", 9: "Busy"}
 
 METHOD_BATCHED = 0
 METHOD_PLAIN = 1
@@ -505,6 +506,157 @@ def for_each_fastx_record(path, visit):  # api.rs:893-915
     if err:
         raise err[0]
     _check(rc)
+
+
+# ---- serving loop (no counterpart in the reference) -------------------------
+REQ_ABUNDANCES = 0
+REQ_SPECTRUM = 1
+
+
+@dataclass
+class ServeConfig:
+    """Coalescing limits, HBM budget and admission limit of a SuperCountingBloomServer."""
+    max_batch_bases: int = 1 << 27
+    max_batch_requests: int = 1024
+    hbm_budget_bytes: int = 16 << 30
+    workers: int = 2
+    max_pending: int = 4096
+
+    def _c(self):
+        return ScbServeConfig(self.max_batch_bases, self.max_batch_requests, self.hbm_budget_bytes,
+                              self.workers, self.max_pending)
+
+
+@dataclass
+class Served:
+    """One finished request: an abundance request carries values (counter width,
+    position-indexed, 0 where no valid window starts), the validity bitmap and the
+    report of scb_index_estimate_sequences; a spectrum request carries rows,
+    queried and distinct. Times are CLOCK_MONOTONIC nanoseconds."""
+    ticket: int
+    kind: int
+    report: Optional[QueryFastaReport] = None
+    values: Optional[np.ndarray] = None
+    valid_bits: Optional[np.ndarray] = None
+    total_bases: int = 0
+    rows: Optional[ApproxSpectrum] = None
+    queried: int = 0
+    distinct: int = 0
+    submit_ns: int = 0
+    start_ns: int = 0
+    done_ns: int = 0
+    batch_requests: int = 0
+
+    @property
+    def valid(self) -> np.ndarray:
+        bits = np.unpackbits(self.valid_bits.view(np.uint8), bitorder="little")
+        return bits[:self.total_bases].astype(bool)
+
+
+def host_array(n: int, dtype) -> np.ndarray:
+    """numpy array over page-locked host memory (freed with the array)."""
+    dt = np.dtype(dtype)
+    nbytes = max(1, n * dt.itemsize)
+    p = lib().scb_host_alloc(nbytes)
+    if not p:
+        raise MemoryError(f"scb_host_alloc({nbytes}) failed")
+    buf = (C.c_uint8 * nbytes).from_address(p)
+    arr = np.frombuffer(buf, dtype=dt, count=n)
+    import weakref
+    weakref.finalize(buf, lib().scb_host_free, C.c_void_p(p))
+    return arr
+
+
+class SuperCountingBloomServer:
+    """Serves concurrent abundance and spectrum requests from an index resident in
+    HBM (include/scb.h serving loop). Inputs and outputs of a request are kept
+    alive here until it is waited for."""
+
+    def __init__(self, index: "SuperCountingBloomIndex", cfg: Optional[ServeConfig] = None):
+        self._index = index
+        self._h = C.c_void_p()
+        c = (cfg or ServeConfig())._c()
+        _check(lib().scb_server_new(index._h, C.byref(c), C.byref(self._h)))
+        self._dtype = {8: np.uint8, 16: np.uint16, 32: np.uint32}[index.config().counter_bits]
+        self._live = {}
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().scb_server_free(self._h)
+            self._h = C.c_void_p()
+            self._live.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit_abundances(self, bases: np.ndarray, offsets: np.ndarray, values: Optional[np.ndarray] = None,
+                          valid_bits: Optional[np.ndarray] = None) -> int:
+        bases = np.ascontiguousarray(bases, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        nb = int(offsets[-1])
+        if values is None:
+            values = np.empty(max(nb, 1), dtype=self._dtype)
+        if valid_bits is None:
+            valid_bits = np.empty(max((nb + 31) // 32, 1), dtype=np.uint32)
+        if values.dtype != self._dtype or values.size < nb or valid_bits.dtype != np.uint32 \
+                or valid_bits.size < (nb + 31) // 32:
+            raise ValueError("output buffers are too small or of the wrong dtype")
+        t = C.c_uint64()
+        _check(lib().scb_server_submit_abundances(self._h, bases.ctypes.data, offsets.ctypes.data,
+                                                  offsets.size - 1, values.ctypes.data, valid_bits.ctypes.data,
+                                                  C.byref(t)))
+        self._live[t.value] = (REQ_ABUNDANCES, bases, offsets, values, valid_bits, nb)
+        return t.value
+
+    def submit_spectrum(self, bases: np.ndarray, offsets: np.ndarray, cap: int = 1 << 17) -> int:
+        bases = np.ascontiguousarray(bases, dtype=np.uint8)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        rv = np.zeros(cap, dtype=np.uint64)
+        rc = np.zeros(cap, dtype=np.uint64)
+        t = C.c_uint64()
+        _check(lib().scb_server_submit_spectrum(self._h, bases.ctypes.data, offsets.ctypes.data,
+                                                offsets.size - 1, rv.ctypes.data, rc.ctypes.data, cap,
+                                                C.byref(t)))
+        self._live[t.value] = (REQ_SPECTRUM, bases, offsets, rv, rc, cap)
+        return t.value
+
+    def _result(self, ticket, r: ScbServed) -> Served:
+        kind, _b, _o, x, y, z = self._live.pop(ticket)
+        out = Served(ticket, kind, submit_ns=r.submit_ns, start_ns=r.start_ns, done_ns=r.done_ns,
+                     batch_requests=r.batch_requests)
+        if kind == REQ_ABUNDANCES:
+            out.report, out.values, out.valid_bits, out.total_bases = _qrep(r.report), x[:z], y, z
+        else:
+            n = min(z, r.n_rows)
+            out.rows = [(int(a), int(b)) for a, b in zip(x[:n], y[:n])]
+            out.queried, out.distinct = r.queried, r.distinct
+        return out
+
+    def wait(self, ticket: int) -> Served:
+        r = ScbServed()
+        rc = lib().scb_server_wait(self._h, ticket, C.byref(r))
+        if rc != 0 and ticket in self._live:
+            self._live.pop(ticket)
+        _check(rc)
+        return self._result(ticket, r)
+
+    def wait_any(self, timeout_s: float) -> Optional[Served]:
+        r = ScbServed()
+        t = C.c_uint64()
+        rc = lib().scb_server_wait_any(self._h, int(timeout_s * 1e6), C.byref(t), C.byref(r))
+        if rc != 0 and t.value in self._live:
+            self._live.pop(t.value)
+        _check(rc)
+        return self._result(t.value, r) if t.value else None
 
 
 def xorshift_u64(x: int) -> int:  # filter.rs:629-635

@@ -13,12 +13,15 @@ LIB = os.path.join(HERE, "libscb.so")
 ROOT = os.path.dirname(HERE)
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ROCM = os.path.dirname(os.path.dirname(os.path.realpath(HIPCC)))
 ARCH = os.environ.get("SCB_OFFLOAD_ARCH", "gfx950")
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result",
     f"-I{os.path.join(ROOT, 'include')}",
 ]
+# Host-only sources that call the HIP runtime API.
+HOST = ["-x", "c++", "-D__HIP_PLATFORM_AMD__", f"-I{os.path.join(ROCM, 'include')}"]
 # Packed-FP32 instructions are disabled for all device code.
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
@@ -36,9 +39,9 @@ def build(verbose=False):
     objs = []
     tmpdir = os.path.join(HERE, "_build")
     os.makedirs(tmpdir, exist_ok=True)
-    for src, dev in (("scb_kernels.hip", True), ("scb_api.cpp", False)):
+    for src, dev in (("scb_kernels.hip", True), ("scb_api.cpp", False), ("scb_serve.cpp", False)):
         obj = os.path.join(tmpdir, src + ".o")
-        cmd = [HIPCC] + COMMON + (DEVICE if dev else ["-x", "c++"]) + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC] + COMMON + (DEVICE if dev else HOST) + ["-c", os.path.join(CSRC, src), "-o", obj]
         out = _run(cmd)
         if verbose and out:
             print(out)

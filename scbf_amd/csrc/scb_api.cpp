@@ -20,19 +20,26 @@
 
 #include "../../include/scb.h"
 #include "scb_engine.h"
+#include "scb_handles.h"
 
 using scb::Engine;
 using scb::Status;
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
 
+namespace scb_host {
 int fail(int code, const std::string &msg) {
     g_last_error = msg;
     return code;
 }
 int fail(const Status &s) { return fail(s.code, s.msg); }
+}  // namespace scb_host
+
+namespace {
+
+using scb_host::fail;
 
 int invalid_config(const std::string &m) { return fail(SCB_ERR_INVALID_CONFIG, "invalid config: " + m); }
 
@@ -246,20 +253,6 @@ int for_each_batch(const char *path, bool split, size_t k, Fn &&fn) {
 }
 
 }  // namespace
-
-struct scb_builder {
-    scb_config cfg;
-    std::unique_ptr<Engine> engine;
-    uint64_t inserted_kmers = 0, inserted_smers = 0;
-    scb::Method method = scb::kBatched;
-};
-
-struct scb_index {
-    scb_config cfg;
-    std::unique_ptr<Engine> engine;
-    uint64_t inserted_kmers = 0;
-    scb::Method method = scb::kBatched;
-};
 
 namespace {
 

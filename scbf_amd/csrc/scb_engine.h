@@ -79,6 +79,7 @@ class Engine {
     ~Engine();
 
     const Geometry &geometry() const { return geo_; }
+    int device() const { return device_; }
     Tuning &tuning() { return tune_; }
 
     Status insert(const DeviceBatch &b, Method method, InsertResult &r);
@@ -126,6 +127,17 @@ struct StageTimes {
     bool batched = false;
 };
 StageTimes last_stage_times();
+
+// Stream-ordered helpers for host code that drives its own HIP streams
+// (stream is a hipStream_t).
+Status fill_u32_async(uint32_t *d_p, uint64_t n, uint32_t value, void *stream);
+// Statistics of query outputs per segment of positions: segs holds nseg sorted,
+// disjoint [begin, end) position pairs; out receives, per segment, the number
+// of valid windows, of valid windows with a positive estimate and the wrapping
+// sum of the estimates (3 u64 each).
+Status segment_stats_async(uint32_t counter_bits, const void *d_values, const uint32_t *d_valid_bits,
+                           const uint64_t *d_segs, uint32_t nseg, uint64_t n_positions, uint64_t *d_out,
+                           void *stream);
 
 Status make_geometry(uint64_t k, uint64_t m, uint64_t s, uint64_t nh, uint32_t bits, uint32_t cse,
                      uint32_t bse, Geometry &g);

@@ -1,0 +1,29 @@
+// Internal: the opaque handle types behind include/scb.h and the error
+// reporting shared by the C ABI translation units (scb_api.cpp, scb_serve.cpp).
+#pragma once
+
+#include <memory>
+#include <string>
+
+#include "../../include/scb.h"
+#include "scb_engine.h"
+
+struct scb_builder {
+    scb_config cfg;
+    std::unique_ptr<scb::Engine> engine;
+    uint64_t inserted_kmers = 0, inserted_smers = 0;
+    scb::Method method = scb::kBatched;
+};
+
+struct scb_index {
+    scb_config cfg;
+    std::unique_ptr<scb::Engine> engine;
+    uint64_t inserted_kmers = 0;
+    scb::Method method = scb::kBatched;
+};
+
+namespace scb_host {
+// Records msg as the calling thread's last error (scb_last_error) and returns code.
+int fail(int code, const std::string &msg);
+int fail(const scb::Status &s);
+}  // namespace scb_host

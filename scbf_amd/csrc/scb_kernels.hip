@@ -2231,6 +2231,76 @@ Status Engine::spectrum_host(const uint8_t *bases, const uint64_t *offsets, uint
 
 StageTimes last_stage_times() { return g_stage_times; }
 
+namespace {
+constexpr uint32_t kStatChunk = 4096;  // positions per workgroup
+
+// filter.rs:552-556 per segment: count, positive count and wrapping sum of
+// the valid windows' estimates.
+template <int BITS>
+__global__ __launch_bounds__(256) void k_segment_stats(const void *values, const uint32_t *valid_bits,
+                                                       const uint64_t *segs, uint32_t nseg, uint64_t npos,
+                                                       unsigned long long *out) {
+    using T = typename Ctr<BITS>::T;
+    __shared__ uint64_t s_red[3][4];
+    const uint64_t c0 = uint64_t(blockIdx.x) * kStatChunk, c1 = min(npos, c0 + kStatChunk);
+    // first segment ending after c0
+    uint32_t lo = 0, hi = nseg;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (segs[2 * mid + 1] <= c0) lo = mid + 1; else hi = mid;
+    }
+    for (uint32_t sg = lo; sg < nseg && segs[2 * sg] < c1; ++sg) {
+        const uint64_t b = max(c0, segs[2 * sg]), e = min(c1, segs[2 * sg + 1]);
+        uint64_t n = 0, pos = 0, sum = 0;
+        for (uint64_t p = b + threadIdx.x; p < e; p += blockDim.x) {
+            if ((valid_bits[p >> 5] >> (p & 31)) & 1) {
+                uint64_t v = reinterpret_cast<const T *>(values)[p];
+                ++n;
+                pos += v > 0;
+                sum += v;
+            }
+        }
+        uint64_t vals[3] = {n, pos, sum};
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            uint64_t v = vals[q];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+            if ((threadIdx.x & 63) == 0) s_red[q][threadIdx.x >> 6] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < 3) {
+            uint64_t t = 0;
+            for (uint32_t i = 0; i < blockDim.x / 64; ++i) t += s_red[threadIdx.x][i];
+            if (t) atomicAdd(&out[3 * sg + threadIdx.x], (unsigned long long)t);
+        }
+        __syncthreads();
+    }
+}
+}  // namespace
+
+Status fill_u32_async(uint32_t *d_p, uint64_t n, uint32_t value, void *stream) {
+    fill(d_p, n, value, reinterpret_cast<hipStream_t>(stream));
+    SCB_HIP(hipGetLastError());
+    return Status::ok();
+}
+
+Status segment_stats_async(uint32_t counter_bits, const void *d_values, const uint32_t *d_valid_bits,
+                           const uint64_t *d_segs, uint32_t nseg, uint64_t n_positions, uint64_t *d_out,
+                           void *stream) {
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    fill(reinterpret_cast<uint32_t *>(d_out), 6 * uint64_t(nseg), 0, st);
+    if (!nseg || !n_positions) return Status::ok();
+    dim3 grid(unsigned((n_positions + kStatChunk - 1) / kStatChunk)), blk(256);
+    auto *o = reinterpret_cast<unsigned long long *>(d_out);
+    switch (counter_bits) {
+        case 8: hipLaunchKernelGGL(k_segment_stats<8>, grid, blk, 0, st, d_values, d_valid_bits, d_segs, nseg, n_positions, o); break;
+        case 16: hipLaunchKernelGGL(k_segment_stats<16>, grid, blk, 0, st, d_values, d_valid_bits, d_segs, nseg, n_positions, o); break;
+        default: hipLaunchKernelGGL(k_segment_stats<32>, grid, blk, 0, st, d_values, d_valid_bits, d_segs, nseg, n_positions, o); break;
+    }
+    SCB_HIP(hipGetLastError());
+    return Status::ok();
+}
+
 Status make_geometry(uint64_t k, uint64_t m, uint64_t s, uint64_t nh, uint32_t bits, uint32_t cse,
                      uint32_t bse, Geometry &g) {
     g = Geometry{};

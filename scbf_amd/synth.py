@@ -107,3 +107,40 @@ def random_records(n, lo, hi, seed, n_rate=0.0, lower_rate=0.0):
     offs = np.zeros(n + 1, dtype=np.uint64)
     offs[1:] = np.cumsum(lens)
     return b, offs
+
+
+def serving_requests(g, count, seed, n_range=(1, 10_000), len_range=(31, 20_000), genome_frac=0.7,
+                     max_bases=None):
+    """Requests of the served workload: each has n ~ log-uniform(n_range) records
+    of length ~ log-uniform(len_range); a record is a genome substring on a random
+    strand with probability genome_frac, else uniform random ACGT. Returns a list
+    of (bases u8, offsets u64[n+1]); max_bases caps one request's total."""
+    rng = np.random.default_rng([seed, 6])
+    grc = _COMP[g[::-1]]
+    out = []
+    for _ in range(count):
+        n = int(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1] + 1))))
+        n = max(n_range[0], min(n, n_range[1]))
+        lens = np.exp(rng.uniform(np.log(len_range[0]), np.log(len_range[1] + 1), size=n)).astype(np.int64)
+        lens = np.clip(lens, len_range[0], min(len_range[1], g.size))
+        if max_bases is not None:
+            keep = np.cumsum(lens) <= max_bases
+            keep[0] = True
+            lens = lens[keep]
+            n = lens.size
+        from_g = rng.random(n) < genome_frac
+        strand = rng.random(n) < 0.5
+        starts = rng.integers(0, g.size - lens + 1)
+        parts = []
+        for i in range(n):
+            L = int(lens[i])
+            if from_g[i]:
+                src = grc if strand[i] else g
+                parts.append(src[starts[i]:starts[i] + L])
+            else:
+                parts.append(ACGT[rng.integers(0, 4, size=L, dtype=np.uint8)])
+        bases = np.ascontiguousarray(np.concatenate(parts))
+        offsets = np.zeros(n + 1, dtype=np.uint64)
+        offsets[1:] = np.cumsum(lens)
+        out.append((bases, offsets))
+    return out
