@@ -335,6 +335,8 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
     __shared__ uint32_t s_wx[2][kTileThreads / 64];
     __shared__ uint32_t s_of[2][kChunksMax];  // XOR prefix of all chunks before chunk c
     __shared__ uint32_t s_wc[kTileThreads / 64];
+    // bit b set: window b + 1 ends the super-k-mer before it (invalid or a new start)
+    __shared__ uint32_t s_bnd[kTile / 32 + 2];
     __shared__ uint64_t s_red[3][kTileThreads / 64];
     __shared__ uint64_t s_base;
     __shared__ uint32_t s_r0;
@@ -510,6 +512,17 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
         // owned windows hj in [1, kTile]: validity bitmap, super-k-mer starts
         constexpr int kPer = kTile / kTileThreads;
         uint32_t starts = 0;  // bit u set if window 1 + u*256 + tid starts a super-k-mer
+        if (MODE == kModeRecords && wid == 0) {
+            // boundaries of the windows after the tile (hj in [kTile + 1, kTile + w))
+            const int hj = kTile + 1 + lane;
+            bool b = false;
+            if (hj < kTile + w) {
+                uint16_t mp = s_mpos[hj], pm = s_mpos[hj - 1];
+                b = mp == kNoWin || pm != mp;
+            }
+            uint64_t bm = __ballot(b);
+            if (lane == 0) s_bnd[kTile / 32] = uint32_t(bm), s_bnd[kTile / 32 + 1] = uint32_t(bm >> 32);
+        }
 #pragma unroll
         for (int u = 0; u < kPer; ++u) {
             int hj = 1 + u * kTileThreads + tid;
@@ -518,6 +531,13 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
             bool st = valid && (pm == kNoWin || pm != mp);
             starts |= uint32_t(st) << u;
             acc_valid += valid;
+            if (MODE == kModeRecords) {
+                uint64_t bm = __ballot(st || !valid);
+                if (lane == 0) {
+                    const int b0 = (u * kTileThreads + wid * 64) >> 5;
+                    s_bnd[b0] = uint32_t(bm), s_bnd[b0 + 1] = uint32_t(bm >> 32);
+                }
+            }
             if (a.valid_bits) {
                 uint64_t vm = __ballot(valid);
                 if (lane == 0) {
@@ -549,20 +569,28 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
                 uint32_t seg = tile & (kSeg - 1);
                 s_base = seg * a.seg_cap + atomicAdd(&a.seg_count[seg], total);
             }
+            // compact the start windows into a list (s_PL is dead after phase 4) so
+            // that every thread emits the same number of records, in slot order
+            uint16_t *s_list = s_PL;
+            {
+                uint32_t at = before + incl - my;
+                for (uint32_t bits = starts; bits; bits &= bits - 1)
+                    s_list[at++] = uint16_t(1 + (__ffs(bits) - 1) * kTileThreads + tid);
+            }
             __syncthreads();
-            uint64_t slot = s_base + before + incl - my;
-            for (uint32_t bits = starts; bits; bits &= bits - 1) {
-                int u = __ffs(bits) - 1;
-                int hj = 1 + u * kTileThreads + tid;
-                uint16_t mp = s_mpos[hj];
-                int e = hj + 1;
-                while (e < hj + w && s_mpos[e] == mp) ++e;
-                uint32_t nk = uint32_t(e - hj);
+            const uint64_t base = s_base;
+            for (uint32_t i = tid; i < total; i += kTileThreads) {
+                const int hj = s_list[i];
+                const uint16_t mp = s_mpos[hj];
+                // k-mers of the super-k-mer: up to the next boundary window (at most w)
+                const uint32_t wi = uint32_t(hj) >> 5;
+                const uint64_t bw = (uint64_t(s_bnd[wi + 1]) << 32 | s_bnd[wi]) >> (hj & 31);
+                const uint32_t nk = bw ? min(uint32_t(__builtin_ctzll(bw)) + 1, uint32_t(w)) : uint32_t(w);
                 uint64_t val = canonical_at(s_pk, uint64_t(mp) + kLeft - 1, uint32_t(m));
                 uint64_t block = xs64(val) & g.block_mask;
                 uint64_t pblk = ((block & g.shard_mask) << g.sub_log) | (block >> g.shard_shift);
                 uint64_t j = uint64_t(t0 - 1 + hj);
-                a.rec[slot++] = j | (uint64_t(nk - 1) << 32) | (pblk << 37);
+                a.rec[base + i] = j | (uint64_t(nk - 1) << 32) | (pblk << 37);
                 acc_smers += nk + g.k - g.s;
             }
         } else {
@@ -1086,6 +1114,7 @@ __global__ __launch_bounds__(512) void k_bin_query(BinArgs a, DevGeo g) {
             uint32_t A0, A1, A2, A3;
             load_aligned(a.pk, j0, A0, A1, A2, A3);
             WinMin<WINC> wm;
+            T *out = reinterpret_cast<T *>(a.out);
             for_each_smer(A0, A1, A2, A3, nk + g.k - 1, g.s, [&](uint32_t j, uint64_t smer) {
                 uint64_t h = xs64(smer);
                 uint32_t est = Ctr<BITS>::kMax;
@@ -1097,7 +1126,7 @@ __global__ __launch_bounds__(512) void k_bin_query(BinArgs a, DevGeo g) {
                 uint32_t ab = wm.push(est, g.win);
                 if (j + 1 >= g.win) {
                     uint64_t q = j + 1 - g.win;
-                    if (a.out) reinterpret_cast<T *>(a.out)[j0 + q] = T(ab);
+                    if (out) out[j0 + q] = T(ab);
                     ++queried;
                     positive += ab > 0;
                     checksum += ab;
