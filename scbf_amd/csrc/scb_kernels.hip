@@ -937,6 +937,54 @@ __device__ __forceinline__ uint32_t sat_merge_word(uint32_t old, const uint32_t 
     }
 }
 
+// Work items hold at most this many records (their order lives in LDS as u16).
+constexpr uint32_t kMaxItemRecords = 4096;
+
+// Orders the records [rb, rb + n) of a work item by k-mer count, longest
+// first, into s_perm (indices relative to rb). A wave's s-mer loop runs as
+// long as its longest record; with the lanes of a wave on records of equal
+// length no lane idles. Lanes of equal length are matched with five ballots
+// and counted by one leader each, so the LDS atomics never collide.
+__device__ __forceinline__ void order_by_length(const uint64_t *sorted, uint32_t rb, uint32_t n, uint16_t *s_perm,
+                                                uint32_t *s_cls, uint32_t tid, uint32_t nthr) {
+    const uint32_t lane = tid & 63;
+    const uint64_t lt = (uint64_t(1) << lane) - 1;
+    if (tid < 32) s_cls[tid] = 0;
+    __syncthreads();
+#pragma unroll 1
+    for (int pass = 0; pass < 2; ++pass) {
+        for (uint32_t i0 = 0; i0 < n; i0 += nthr) {
+            const uint32_t i = i0 + tid;
+            const bool act = i < n;
+            const uint32_t key = act ? 31 - uint32_t((sorted[rb + i] >> 32) & 31) : 0;  // longest first
+            uint64_t m = __ballot(act);
+#pragma unroll
+            for (int bit = 0; bit < 5; ++bit) {
+                const uint64_t b = __ballot((key >> bit) & 1);
+                m &= ((key >> bit) & 1) ? b : ~b;
+            }
+            const uint32_t rank = uint32_t(__popcll(m & lt));
+            const uint32_t leader = act ? uint32_t(__builtin_ctzll(m)) : 0;
+            uint32_t base = 0;
+            if (act && rank == 0) base = atomicAdd(&s_cls[key], uint32_t(__popcll(m)));
+            base = __shfl(base, int(leader), 64);
+            if (pass == 1 && act) s_perm[base + rank] = uint16_t(i);
+        }
+        __syncthreads();
+        if (pass == 0) {
+            if (tid == 0) {
+                uint32_t run = 0;
+                for (int c = 0; c < 32; ++c) {
+                    const uint32_t v = s_cls[c];
+                    s_cls[c] = run;
+                    run += v;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 struct BinArgs {
     const uint64_t *sorted;
     const uint32_t *bin_off, *bin_count;
@@ -976,6 +1024,8 @@ template <int BITS>
 __global__ __launch_bounds__(1024) void k_bin_insert(BinArgs a, DevGeo g) {
     extern __shared__ __align__(16) uint32_t s_cnt[];
     __shared__ uint64_t s_item;
+    __shared__ uint16_t s_perm[kMaxItemRecords];
+    __shared__ uint32_t s_cls[32];
     const uint32_t tid = threadIdx.x, nthr = blockDim.x;
     const uint32_t bin_slots = 1u << a.bin_log;
     const uint32_t lshift = a.bin_log - g.bse;
@@ -995,9 +1045,9 @@ __global__ __launch_bounds__(1024) void k_bin_insert(BinArgs a, DevGeo g) {
         uint32_t re = min(a.bin_off[bin] + cnt, rb + a.per_item);
         bool multi = cnt > a.per_item;
         zero_lds(s_cnt, bin_slots, tid, nthr);
-        __syncthreads();
-        for (uint32_t r = rb + tid; r < re; r += nthr) {
-            uint64_t rec = a.sorted[r];
+        order_by_length(a.sorted, rb, re - rb, s_perm, s_cls, tid, nthr);  // (synchronizes)
+        for (uint32_t i = tid; i < re - rb; i += nthr) {
+            uint64_t rec = a.sorted[rb + s_perm[i]];
             uint64_t j = rec & 0xFFFFFFFFull;
             uint32_t nk = uint32_t((rec >> 32) & 31) + 1;
             uint32_t lbase = uint32_t((rec >> 37) & ((1u << lshift) - 1)) << g.bse;
@@ -1082,6 +1132,8 @@ __global__ __launch_bounds__(512) void k_bin_query(BinArgs a, DevGeo g) {
     extern __shared__ __align__(16) uint32_t s_tab[];
     __shared__ uint64_t s_item;
     __shared__ uint64_t s_red[3][8];
+    __shared__ uint16_t s_perm[kMaxItemRecords];
+    __shared__ uint32_t s_cls[32];
     using T = typename Ctr<BITS>::T;
     const uint32_t tid = threadIdx.x, nthr = blockDim.x, lane = tid & 63, wid = tid >> 6;
     const uint32_t bin_slots = 1u << a.bin_log;
@@ -1105,9 +1157,9 @@ __global__ __launch_bounds__(512) void k_bin_query(BinArgs a, DevGeo g) {
         uint32_t re = min(a.bin_off[bin] + cnt, rb + a.per_item);
         const uint32_t *tw = reinterpret_cast<const uint32_t *>(a.table) + bin * uint64_t(nwords);
         stage_to_lds<8>(s_tab, tw, nwords, tid, nthr);
-        __syncthreads();
-        for (uint32_t r = rb + tid; r < re; r += nthr) {
-            uint64_t rec = a.sorted[r];
+        order_by_length(a.sorted, rb, re - rb, s_perm, s_cls, tid, nthr);  // (synchronizes)
+        for (uint32_t i = tid; i < re - rb; i += nthr) {
+            uint64_t rec = a.sorted[rb + s_perm[i]];
             uint64_t j0 = rec & 0xFFFFFFFFull;
             uint32_t nk = uint32_t((rec >> 32) & 31) + 1;
             uint32_t lbase = uint32_t((rec >> 37) & ((1u << lshift) - 1)) << g.bse;
@@ -1558,6 +1610,8 @@ void table_release(int device, uint64_t bytes, void *p) {
 }
 }  // namespace
 
+static_assert(kMaxItemRecords <= 65536, "record order indices are u16");
+
 Status Engine::create(const Geometry &g, std::unique_ptr<Engine> &out) {
     if (device_count() <= 0) return Status{8, "device error: no HIP device is visible"};
     std::unique_ptr<Engine> e(new Engine);
@@ -1823,7 +1877,7 @@ Status Engine::insert(const DeviceBatch &b, Method method, InsertResult &r) {
     } else {
         uint64_t nbins = 0;
         uint32_t bin_log = tune_.insert_bin_log;
-        s = batched_front(L, b, bin_log, tune_.max_records_per_item, nullptr, nbins);
+        s = batched_front(L, b, bin_log, std::max<uint32_t>(1, std::min<uint32_t>(tune_.max_records_per_item, kMaxItemRecords)), nullptr, nbins);
         if (s.good()) {
             BinArgs a{};
             a.sorted = ws->final_sorted;
@@ -1834,7 +1888,7 @@ Status Engine::insert(const DeviceBatch &b, Method method, InsertResult &r) {
             a.pk = ws->pk.p;
             a.table = d_table_;
             a.bin_log = bin_log;
-            a.per_item = tune_.max_records_per_item;
+            a.per_item = std::max<uint32_t>(1, std::min<uint32_t>(tune_.max_records_per_item, kMaxItemRecords));
             a.fresh = fresh;
             if (fresh) {
                 const uint32_t words_per_bin = uint32_t(((uint64_t(1) << bin_log) * geo_.bytes) / 4);
@@ -1905,7 +1959,7 @@ Status Engine::query(const DeviceBatch &b, Method method, void *d_values, uint32
     } else {
         uint64_t nbins = 0;
         uint32_t bin_log = tune_.query_bin_log;
-        s = batched_front(L, b, bin_log, tune_.max_records_per_item, d_valid_bits, nbins);
+        s = batched_front(L, b, bin_log, std::max<uint32_t>(1, std::min<uint32_t>(tune_.max_records_per_item, kMaxItemRecords)), d_valid_bits, nbins);
         if (s.good()) {
             BinArgs a{};
             a.sorted = ws->final_sorted;
@@ -1917,7 +1971,7 @@ Status Engine::query(const DeviceBatch &b, Method method, void *d_values, uint32
             a.table = d_table_;
             a.out = d_values;
             a.bin_log = bin_log;
-            a.per_item = tune_.max_records_per_item;
+            a.per_item = std::max<uint32_t>(1, std::min<uint32_t>(tune_.max_records_per_item, kMaxItemRecords));
             size_t lds = (size_t(1) << bin_log) * geo_.bytes;
             int thr = int(tune_.query_threads);
             bool wide = geo_.win > 8;
@@ -1970,7 +2024,7 @@ Status Engine::spectrum(const DeviceBatch &b, SpectrumResult &r) const {
     hipStream_t st = ws.stream;
     const uint32_t bin_log = geo_.bse <= 14 ? std::min<uint32_t>(14, geo_.cse) : geo_.bse;
     const uint32_t stage = (geo_.bse <= 14 && (uint64_t(1) << bin_log) * geo_.bytes >= 4) ? 1 : 0;
-    const uint32_t per_item = tune_.max_records_per_item;
+    const uint32_t per_item = std::max<uint32_t>(1, std::min<uint32_t>(tune_.max_records_per_item, kMaxItemRecords));
     uint64_t nbins = uint64_t(1) << (geo_.cse - bin_log);
     const uint32_t dense_n = geo_.bits == 8 ? 256 : (geo_.bits == 16 ? 65536 : kHistSmall);
     do {
