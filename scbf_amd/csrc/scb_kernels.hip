@@ -675,36 +675,28 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
 // Bin planning: exclusive scan of per-bin record counts and the work list.
 // One workgroup of 1024 threads.
 // ----------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_bin_plan(const uint32_t *bin_count, uint64_t nbins,
-                                                   uint32_t *bin_off, uint32_t *cursor,
-                                                   uint64_t *work, uint32_t per_item,
-                                                   uint64_t *stats) {
+__global__ __launch_bounds__(1024) void k_bin_plan(const uint32_t *__restrict__ bin_count, uint64_t nbins,
+                                                   uint32_t *__restrict__ bin_off, uint32_t *__restrict__ cursor,
+                                                   uint64_t *__restrict__ work, uint32_t per_item,
+                                                   uint64_t *__restrict__ stats) {
+    constexpr int kRegBins = 32;  // runs of up to 32 bins per thread stay in registers
     __shared__ uint64_t s_rec[1024], s_items[1024];
     const int tid = threadIdx.x;
     const uint64_t chunk = (nbins + 1023) / 1024;
     const uint64_t b0 = tid * chunk, b1 = min(nbins, b0 + chunk);
-    // counts of this thread's contiguous run, loaded 4 at a time (8 loads in flight)
-    const bool vec = (chunk & 3) == 0 && nbins % 1024 == 0;
+    const bool in_regs = chunk <= kRegBins;
+    uint32_t c[kRegBins];
     uint64_t rsum = 0, isum = 0;
-    if (vec) {
-        const uint4 *cv = reinterpret_cast<const uint4 *>(bin_count + b0);
-        for (uint64_t q0 = 0; q0 < chunk / 4; q0 += 8) {
-            uint4 x[8];
+    if (in_regs) {
 #pragma unroll
-            for (int u = 0; u < 8; ++u) x[u] = cv[min<uint64_t>(q0 + u, chunk / 4 - 1)];
+        for (int i = 0; i < kRegBins; ++i) c[i] = b0 + i < b1 ? bin_count[b0 + i] : 0;  // all loads in flight
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                if (q0 + u >= chunk / 4) break;
-                uint32_t c[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
-#pragma unroll
-                for (int e = 0; e < 4; ++e) rsum += c[e], isum += (c[e] + per_item - 1) / per_item;
-            }
-        }
+        for (int i = 0; i < kRegBins; ++i) rsum += c[i], isum += (c[i] + per_item - 1) / per_item;
     } else {
         for (uint64_t b = b0; b < b1; ++b) {
-            uint32_t c = bin_count[b];
-            rsum += c;
-            isum += (c + per_item - 1) / per_item;
+            uint32_t x = bin_count[b];
+            rsum += x;
+            isum += (x + per_item - 1) / per_item;
         }
     }
     s_rec[tid] = rsum, s_items[tid] = isum;
@@ -716,13 +708,27 @@ __global__ __launch_bounds__(1024) void k_bin_plan(const uint32_t *bin_count, ui
         __syncthreads();
     }
     uint64_t ro = s_rec[tid] - rsum, io = s_items[tid] - isum;
-    for (uint64_t b = b0; b < b1; ++b) {
-        uint32_t c = bin_count[b];  // L2-resident re-read
-        bin_off[b] = uint32_t(ro);
-        cursor[b] = uint32_t(ro);
-        uint32_t parts = (c + per_item - 1) / per_item;
-        for (uint32_t p = 0; p < parts; ++p) work[io++] = (b << 32) | p;
-        ro += c;
+    if (in_regs) {
+#pragma unroll
+        for (int i = 0; i < kRegBins; ++i) {
+            const uint64_t b = b0 + i;
+            if (b < b1) {
+                bin_off[b] = uint32_t(ro);
+                cursor[b] = uint32_t(ro);
+                const uint32_t parts = (c[i] + per_item - 1) / per_item;
+                for (uint32_t p = 0; p < parts; ++p) work[io++] = (b << 32) | p;
+                ro += c[i];
+            }
+        }
+    } else {
+        for (uint64_t b = b0; b < b1; ++b) {
+            uint32_t x = bin_count[b];
+            bin_off[b] = uint32_t(ro);
+            cursor[b] = uint32_t(ro);
+            uint32_t parts = (x + per_item - 1) / per_item;
+            for (uint32_t p = 0; p < parts; ++p) work[io++] = (b << 32) | p;
+            ro += x;
+        }
     }
     if (tid == 1023) {
         stats[kStNWork] = s_items[1023];
@@ -766,18 +772,46 @@ __device__ __forceinline__ uint32_t tile_lo(const uint64_t *tile, const KeySpec 
     return (b0 >> ks.fine_bits) << ks.fine_bits;
 }
 
+// Numbers the non-empty tiles of all segments densely: s_pre[s] is the first
+// tile of segment s (s_pre[nseg] the total), so that a grid-stride loop visits
+// only tiles holding records. Run by the whole workgroup (nseg <= 64).
+__device__ __forceinline__ uint32_t dense_tiles(const SegIn &in, uint32_t *s_pre, uint32_t tid) {
+    if (tid < 64) {
+        const uint32_t n = tid < in.nseg ? uint32_t(min<uint64_t>(in.counts[tid], in.seg_cap)) : 0;
+        uint32_t incl = (n + kPartTile - 1) / kPartTile;
+        for (int off = 1; off < 64; off <<= 1) {
+            uint32_t y = __shfl_up(incl, off, 64);
+            if (tid >= uint32_t(off)) incl += y;
+        }
+        s_pre[tid + 1] = incl;
+        if (tid == 0) s_pre[0] = 0;
+    }
+    __syncthreads();
+    return s_pre[in.nseg];
+}
+
+// Segment of dense tile u (s_pre ascending): the last s with s_pre[s] <= u.
+__device__ __forceinline__ uint32_t tile_segment(const uint32_t *s_pre, uint32_t nseg, uint32_t u) {
+    uint32_t lo = 0, hi = nseg;  // s_pre[lo] <= u < s_pre[hi]
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (s_pre[mid] <= u) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
 // Histogram of keys (records per key, and k-mers per key when kmers != null).
 __global__ __launch_bounds__(kPartThreads) void k_seg_hist(SegIn in, KeySpec ks, uint32_t *hist,
                                                            uint32_t *kmers) {
     __shared__ uint32_t s_h[kPartRange], s_k[kPartRange];
     __shared__ uint32_t s_lo;
+    __shared__ uint32_t s_pre[65];
     const uint32_t tid = threadIdx.x;
-    const uint64_t tps = (in.seg_cap + kPartTile - 1) / kPartTile;
-    for (uint64_t u = blockIdx.x; u < in.nseg * tps; u += gridDim.x) {
-        const uint32_t seg = uint32_t(u / tps);
-        const uint64_t t0 = (u % tps) * kPartTile;
-        const uint64_t n = in.counts[seg];
-        if (t0 >= n) continue;  // uniform across the workgroup
+    const uint32_t total = dense_tiles(in, s_pre, tid);
+    for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+        const uint32_t seg = tile_segment(s_pre, in.nseg, u);
+        const uint64_t t0 = uint64_t(u - s_pre[seg]) * kPartTile;
+        const uint64_t n = min<uint64_t>(in.counts[seg], in.seg_cap);
         const uint64_t *tile = in.src + seg * in.seg_cap + t0;
         const uint32_t cnt = uint32_t(min<uint64_t>(kPartTile, n - t0));
         for (uint32_t i = tid; i < kPartRange; i += kPartThreads) s_h[i] = 0, s_k[i] = 0;
@@ -792,7 +826,7 @@ __global__ __launch_bounds__(kPartThreads) void k_seg_hist(SegIn in, KeySpec ks,
                 atomicAdd(&s_h[key - lo], 1u);
                 if (kmers) atomicAdd(&s_k[key - lo], nk);
             } else {
-                atomicAdd(&hist[key], 1u);
+                atomicAdd(&hist[key * ks.copies + u % ks.copies], 1u);
                 if (kmers) atomicAdd(&kmers[key], nk);
             }
         }
@@ -839,13 +873,13 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(SegIn in, KeySpec ks
                                                             uint64_t *dst) {
     __shared__ uint32_t s_hist[kPartRange];
     __shared__ uint32_t s_lo;
+    __shared__ uint32_t s_pre[65];
     const uint32_t tid = threadIdx.x;
-    const uint64_t tps = (in.seg_cap + kPartTile - 1) / kPartTile;
-    for (uint64_t u = blockIdx.x; u < in.nseg * tps; u += gridDim.x) {
-        const uint32_t seg = uint32_t(u / tps);
-        const uint64_t t0 = (u % tps) * kPartTile;
-        const uint64_t n = in.counts[seg];
-        if (t0 >= n) continue;  // uniform across the workgroup
+    const uint32_t total = dense_tiles(in, s_pre, tid);
+    for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+        const uint32_t seg = tile_segment(s_pre, in.nseg, u);
+        const uint64_t t0 = uint64_t(u - s_pre[seg]) * kPartTile;
+        const uint64_t n = min<uint64_t>(in.counts[seg], in.seg_cap);
         const uint64_t *tile = in.src + seg * in.seg_cap + t0;
         const uint32_t cnt = uint32_t(min<uint64_t>(kPartTile, n - t0));
         for (uint32_t i = tid; i < kPartRange; i += kPartThreads) s_hist[i] = 0;
@@ -865,7 +899,7 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(SegIn in, KeySpec ks
                     key[j] = kk;
                     rank[j] = atomicAdd(&s_hist[kk], 1u);
                 } else {  // outside the tile's range: reserve one slot directly
-                    dst[atomicAdd(&cursor[kk + lo], 1u)] = r[j];
+                    dst[atomicAdd(&cursor[(kk + lo) * ks.copies + u % ks.copies], 1u)] = r[j];
                 }
             }
         }
