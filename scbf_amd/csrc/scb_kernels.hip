@@ -840,8 +840,8 @@ __global__ __launch_bounds__(kPartThreads) void k_seg_hist(SegIn in, KeySpec ks,
     }
 }
 
-// Exclusive scan of a small histogram (n <= 16384) into cursors; the total
-// becomes the record count. Each thread scans a contiguous run of n/1024.
+// Exclusive scan of a histogram into cursors (one workgroup; each thread scans
+// a contiguous run of n/1024 entries); the total becomes the record count.
 __global__ __launch_bounds__(1024) void k_small_scan(const uint32_t *hist, uint32_t n, uint32_t *cursor,
                                                      uint64_t *stats, uint32_t *total_out) {
     __shared__ uint32_t p[1024];
@@ -1820,7 +1820,7 @@ Status batched_front(Launch &L, const DeviceBatch &b, uint32_t bin_log, uint32_t
         const uint32_t lb = uint32_t(__builtin_ctzll(nbins));
         const uint32_t fine_bits = std::max<uint32_t>(7, lb > 11 ? lb - 11 : 0);
         const uint32_t ncoarse = uint32_t(nbins >> fine_bits);
-        constexpr uint32_t kCopies = 8;  // coarse counters are hot: spread each over 8 copies
+        constexpr uint32_t kCopies = 32;  // coarse counters are hot: spread each over 32 copies
         SCB_HIP(ws.coarse_hist.reserve(ncoarse * kCopies));
         SCB_HIP(ws.ccursor.reserve(ncoarse * kCopies));
         fill(ws.coarse_hist.p, ncoarse * kCopies, 0, st);

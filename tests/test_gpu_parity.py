@@ -173,3 +173,57 @@ def test_reused_table_memory_starts_from_zero(gpu, oracle, method):
     b = scb.SuperCountingBloomBuilder(_cfg(scb, k, m, s, h, bits, cse, bse))
     b.add_sequences((np.frombuffer(b"ACGT", dtype=np.uint8), np.array([0, 4], dtype=np.uint64)))
     assert not b.counters().any()
+
+
+def _parity_roundtrip(scb, oracle, geo, ib, io, qb, qo, method):
+    k, m, s, h, bits, cse, bse = geo
+    of = oracle.Filter(oracle.make_cfg(k, m, s, h, bits, cse, bse))
+    ok_kmers, ok_smers = oracle.insert_records(of, ib, io)
+    b = scb.SuperCountingBloomBuilder(_cfg(scb, k, m, s, h, bits, cse, bse))
+    b.set_method(method)
+    rep = b.add_sequences((ib, io))
+    assert (rep.inserted_kmers, rep.inserted_smers) == (ok_kmers, ok_smers)
+    got = b.counters().astype(np.uint32)
+    want = of.counters()
+    assert np.array_equal(got, want), f"counters differ at {np.flatnonzero(got != want)[:10]}"
+    idx = b.into_index()
+    idx.set_method(method)
+    vals, valid, qrep = idx.estimate_sequences(qb, qo)
+    wv, wok = oracle.estimate_records(of, qb, qo, k)
+    assert np.array_equal(valid, wok)
+    assert np.array_equal(vals[wok], wv[wok])
+    assert qrep.estimate_checksum == int(wv[wok].sum()) & 0xFFFFFFFFFFFFFFFF
+
+
+@pytest.mark.parametrize("cse", [22, 18], ids=["sparse", "hot_bins"])
+@pytest.mark.parametrize("method", METHODS, ids=["batched", "plain"])
+def test_long_reads_with_repeats_parity(gpu, oracle, cse, method):
+    """Long noisy reads over a repeat-rich genome, 8-bit counters (the long-read
+    workload at reduced size): records span many scan tiles, indels shift
+    minimizers, the repeat saturates counters, and at cse=18 every bin holds more
+    records than one work item."""
+    scb = gpu
+    print(f"long reads cse={cse} method={method}", flush=True)
+    g = synth.insert_repeats(synth.genome(200_000, 51), 300, 200, 52)
+    ib, io = synth.variable_reads(g, 1_500_000, 8000, 0.7, 500, 60_000, 53, sub=0.03, ins=0.01, dele=0.01)
+    qb, qo = synth.variable_reads(g, 300_000, 8000, 0.7, 500, 60_000, 54, sub=0.03, ins=0.01, dele=0.01)
+    _parity_roundtrip(scb, oracle, (31, 21, 25, 4, 8, cse, 9), ib, io, qb, qo, method)
+
+
+@pytest.mark.parametrize("method", METHODS, ids=["batched", "plain"])
+def test_weighted_mixture_u32_parity(gpu, oracle, method):
+    """Abundance-weighted mixture of genomes, k21 m15 s17 h3, 32-bit counters,
+    1024-slot blocks (the metagenome workload at reduced size)."""
+    scb = gpu
+    rng = np.random.default_rng(61)
+    weights = np.exp(rng.normal(0.0, 2.0, size=6))
+    reads_b, reads_o = [], [np.zeros(1, dtype=np.uint64)]
+    for i, wgt in enumerate(weights / weights.sum()):
+        g = synth.genome(40_000, 62 + i)
+        n = max(1, int(wgt * 20_000))
+        b, o = synth.fixed_reads(g, n, 150, 70 + i, sub_rate=0.005)
+        reads_b.append(b)
+        reads_o.append(o[1:] + reads_o[-1][-1])
+    ib = np.concatenate(reads_b)
+    io = np.concatenate(reads_o).astype(np.uint64)
+    _parity_roundtrip(scb, oracle, (21, 15, 17, 3, 32, 22, 10), ib, io, ib, io, method)
