@@ -91,6 +91,36 @@ __device__ __forceinline__ uint64_t xs64(uint64_t x) {
     return x ^ (x >> 31);
 }
 
+// Low 15 bits of xs64(x), enough for a slot address inside a block of at most
+// 2^15 slots. The second multiply only has to be exact in bits 0..45 of its
+// product (bit b < 15 of x ^ (x >> 31) reads bits b and b + 31): the low word
+// comes from one 32x32->64 multiply and bits 32..45 from two 14-bit partial
+// products, which the full-rate 24-bit multiplier computes exactly.
+__device__ __forceinline__ uint32_t xs64_low15(uint64_t x) {
+    constexpr uint64_t kC2 = 0x94D049BB133111EBull;
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x ^= x >> 27;
+    const uint32_t yl = uint32_t(x), yh = uint32_t(x >> 32);
+    const uint64_t p = uint64_t(yl) * uint32_t(kC2);
+    const uint32_t hi = uint32_t(p >> 32) + __umul24(yh & 0x3FFFu, uint32_t(kC2) & 0x3FFFu) +
+                        __umul24(yl & 0x3FFFu, uint32_t(kC2 >> 32) & 0x3FFFu);
+    const uint64_t z = (uint64_t(hi) << 32) | uint32_t(p);
+    return uint32_t(z ^ (z >> 31));
+}
+
+// Calls f(address) for the n_hashes chain values xs64(x), xs64(xs64(x)), ...
+// of one canonical s-mer (filter.rs:224-233, 478-481); amask < 2^15.
+template <class F>
+__device__ __forceinline__ void for_each_address(uint64_t smer, uint32_t nh, uint32_t amask, F &&f) {
+    uint64_t h = smer;
+    for (uint32_t t = 1; t < nh; ++t) {
+        h = xs64(h);
+        f(uint32_t(h) & amask);
+    }
+    f(xs64_low15(h) & amask);
+}
+
 // Reverse complement of a len-base 2-bit word (packed-seq read_revcomp_kmer).
 __device__ __forceinline__ uint64_t revcomp(uint64_t x, uint32_t len) {
     uint32_t lo = uint32_t(x), hi = uint32_t(x >> 32);
@@ -247,16 +277,29 @@ __device__ __forceinline__ void load_aligned(const uint32_t *words, uint64_t b, 
 // is associative, so a shift register gives the deque's result).
 template <int WINC>
 struct WinMin {
-    uint32_t e[WINC];
+    // m[i] = minimum of the last i + 1 estimates; each push is WINC - 1 mins
+    uint32_t m[WINC];
+    __device__ __forceinline__ WinMin() {
+#pragma unroll
+        for (int i = 0; i < WINC; ++i) m[i] = 0xFFFFFFFFu;
+    }
+    __device__ __forceinline__ void add(uint32_t est) {
+#pragma unroll
+        for (int i = WINC - 1; i > 0; --i) m[i] = m[i - 1] < est ? m[i - 1] : est;
+        m[0] = est;
+    }
+    // window of `win` <= WINC estimates
     __device__ __forceinline__ uint32_t push(uint32_t est, uint32_t win) {
+        add(est);
+        uint32_t r = m[0];
 #pragma unroll
-        for (int i = 0; i < WINC - 1; ++i) e[i] = e[i + 1];
-        e[WINC - 1] = est;
-        uint32_t m = est;
-#pragma unroll
-        for (int i = 1; i < WINC; ++i)
-            if (uint32_t(i) < win) m = e[WINC - 1 - i] < m ? e[WINC - 1 - i] : m;
-        return m;
+        for (int i = 1; i < WINC; ++i) r = uint32_t(i) + 1 == win ? m[i] : r;
+        return r;
+    }
+    // window of exactly WINC estimates
+    __device__ __forceinline__ uint32_t push_exact(uint32_t est) {
+        add(est);
+        return m[WINC - 1];
     }
 };
 
@@ -1088,11 +1131,7 @@ __global__ __launch_bounds__(1024) void k_bin_insert(BinArgs a, DevGeo g) {
             uint32_t A0, A1, A2, A3;
             load_aligned(a.pk, j, A0, A1, A2, A3);
             for_each_smer(A0, A1, A2, A3, nk + g.k - 1, g.s, [&](uint32_t, uint64_t smer) {
-                uint64_t h = xs64(smer);
-                for (uint32_t t = 0; t < g.nh; ++t) {
-                    atomicAdd(&s_cnt[lbase + (uint32_t(h) & amask)], 1u);
-                    if (t + 1 < g.nh) h = xs64(h);
-                }
+                for_each_address(smer, g.nh, amask, [&](uint32_t ad) { atomicAdd(&s_cnt[lbase + ad], 1u); });
             });
         }
         __syncthreads();
@@ -1161,7 +1200,7 @@ __global__ __launch_bounds__(1024) void k_bin_insert(BinArgs a, DevGeo g) {
 // filter.rs:462-508 (+ 562-571) for every super-k-mer of one bin: counters of
 // the bin staged in LDS at native width, min over hashes per s-mer, sliding
 // min over k-s+1 s-mers per k-mer, abundances written at their window index.
-template <int BITS, int WINC>
+template <int BITS, int WINC, bool EXACT>
 __global__ __launch_bounds__(512) void k_bin_query(BinArgs a, DevGeo g) {
     extern __shared__ __align__(16) uint32_t s_tab[];
     __shared__ uint64_t s_item;
@@ -1201,23 +1240,27 @@ __global__ __launch_bounds__(512) void k_bin_query(BinArgs a, DevGeo g) {
             load_aligned(a.pk, j0, A0, A1, A2, A3);
             WinMin<WINC> wm;
             T *out = reinterpret_cast<T *>(a.out);
+            // per-record sums: at most 32 k-mers, so 8/16-bit estimates sum in 32 bits
+            uint32_t rec_pos = 0, rec_sum = 0;
+            uint64_t rec_sum64 = 0;
             for_each_smer(A0, A1, A2, A3, nk + g.k - 1, g.s, [&](uint32_t j, uint64_t smer) {
-                uint64_t h = xs64(smer);
                 uint32_t est = Ctr<BITS>::kMax;
-                for (uint32_t t = 0; t < g.nh; ++t) {
-                    uint32_t v = lt[lbase + (uint32_t(h) & amask)];
+                for_each_address(smer, g.nh, amask, [&](uint32_t ad) {
+                    uint32_t v = lt[lbase + ad];
                     est = v < est ? v : est;
-                    if (t + 1 < g.nh) h = xs64(h);
-                }
-                uint32_t ab = wm.push(est, g.win);
+                });
+                const uint32_t ab = EXACT ? wm.push_exact(est) : wm.push(est, g.win);
                 if (j + 1 >= g.win) {
                     uint64_t q = j + 1 - g.win;
                     if (out) out[j0 + q] = T(ab);
-                    ++queried;
-                    positive += ab > 0;
-                    checksum += ab;
+                    rec_pos += ab > 0;
+                    if (BITS < 32) rec_sum += ab;
+                    else rec_sum64 += ab;
                 }
             });
+            queried += nk;
+            positive += rec_pos;
+            checksum += BITS < 32 ? uint64_t(rec_sum) : rec_sum64;
         }
         __syncthreads();
     }
@@ -2008,19 +2051,31 @@ Status Engine::query(const DeviceBatch &b, Method method, void *d_values, uint32
             a.per_item = std::max<uint32_t>(1, std::min<uint32_t>(tune_.max_records_per_item, kMaxItemRecords));
             size_t lds = (size_t(1) << bin_log) * geo_.bytes;
             int thr = int(tune_.query_threads);
-            bool wide = geo_.win > 8;
-#define SCB_QRY(BITS, WC)                                                                       \
-    do {                                                                                        \
-        s = set_lds(k_bin_query<BITS, WC>, lds);                                                \
-        if (!s.good()) break;                                                                   \
-        int grid = persistent_grid(k_bin_query<BITS, WC>, thr, lds, device_);                   \
-        hipLaunchKernelGGL((k_bin_query<BITS, WC>), dim3(grid), dim3(thr), lds, st, a, L.dg);   \
+#define SCB_QRY(BITS, WC, EX)                                                                          \
+    do {                                                                                               \
+        s = set_lds(k_bin_query<BITS, WC, EX>, lds);                                                   \
+        if (!s.good()) break;                                                                          \
+        int grid = persistent_grid(k_bin_query<BITS, WC, EX>, thr, lds, device_);                      \
+        hipLaunchKernelGGL((k_bin_query<BITS, WC, EX>), dim3(grid), dim3(thr), lds, st, a, L.dg);      \
     } while (0)
+#define SCB_QRY_W(BITS)                                                                                \
+    switch (geo_.win) {  /* fimpera window k - s + 1: exact up to 8, else up to 32 */                  \
+        case 1: SCB_QRY(BITS, 1, true); break;                                                         \
+        case 2: SCB_QRY(BITS, 2, true); break;                                                         \
+        case 3: SCB_QRY(BITS, 3, true); break;                                                         \
+        case 4: SCB_QRY(BITS, 4, true); break;                                                         \
+        case 5: SCB_QRY(BITS, 5, true); break;                                                         \
+        case 6: SCB_QRY(BITS, 6, true); break;                                                         \
+        case 7: SCB_QRY(BITS, 7, true); break;                                                         \
+        case 8: SCB_QRY(BITS, 8, true); break;                                                         \
+        default: SCB_QRY(BITS, 32, false); break;                                                      \
+    }
             switch (geo_.bits) {
-                case 8: if (wide) SCB_QRY(8, 32); else SCB_QRY(8, 8); break;
-                case 16: if (wide) SCB_QRY(16, 32); else SCB_QRY(16, 8); break;
-                default: if (wide) SCB_QRY(32, 32); else SCB_QRY(32, 8); break;
+                case 8: SCB_QRY_W(8); break;
+                case 16: SCB_QRY_W(16); break;
+                default: SCB_QRY_W(32); break;
             }
+#undef SCB_QRY_W
 #undef SCB_QRY
             if (s.good()) {
                 hipError_t e = hipGetLastError();

@@ -23,6 +23,8 @@ GEOMETRIES = [
     ("k21m15s17_u32", 21, 15, 17, 3, 32, 20, 10),
     ("k31m11s15_u16_widewin", 31, 11, 15, 2, 16, 18, 8),  # fimpera window 17 (> 8)
     ("k31m21s25_u8_sat", 31, 21, 25, 4, 8, 14, 9),  # dense: saturates u8
+    ("k31m21s25_u16_bse15", 31, 21, 25, 4, 16, 20, 15),  # 15-bit slot addresses (largest staged block)
+    ("k15m9s11_u32_h1", 15, 9, 11, 1, 32, 18, 12),  # one hash per s-mer
 ]
 
 
