@@ -66,7 +66,6 @@ struct Tuning {
     uint32_t insert_bin_log = 15;   // slots per insert bin (u32 LDS counts)
     uint32_t query_bin_log = 15;    // slots per query bin (native-width LDS copy)
     uint32_t max_records_per_item = 4096;  // hot-bin split size (at most kMaxItemRecords)
-    uint32_t insert_threads = 1024, query_threads = 512;
 };
 
 class Workspace;  // device scratch buffers + stream, one per concurrent call

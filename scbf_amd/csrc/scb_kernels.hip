@@ -971,14 +971,17 @@ __device__ __forceinline__ void stage_to_lds(uint32_t *lds, const uint32_t *src,
         const uint4 *sv = reinterpret_cast<const uint4 *>(src);
         uint4 *dv = reinterpret_cast<uint4 *>(lds);
         const uint32_t nvec = nwords >> 2;
-        for (uint32_t v0 = tid; v0 < nvec; v0 += nthr * U) {
+        uint32_t v0 = tid;
+        // whole rounds carry no conditions, so the compiler keeps all U loads
+        // in flight instead of sinking each into its guarded store
+        for (; v0 + (U - 1) * nthr < nvec; v0 += nthr * U) {
             uint4 x[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) x[u] = sv[min(v0 + u * nthr, nvec - 1)];  // unconditional: all in flight
+            for (int u = 0; u < U; ++u) x[u] = sv[v0 + u * nthr];
 #pragma unroll
-            for (int u = 0; u < U; ++u)
-                if (v0 + u * nthr < nvec) dv[v0 + u * nthr] = x[u];
+            for (int u = 0; u < U; ++u) dv[v0 + u * nthr] = x[u];
         }
+        for (; v0 < nvec; v0 += nthr) dv[v0] = sv[v0];
     } else {
         for (uint32_t wi = tid; wi < nwords; wi += nthr) lds[wi] = src[wi];
     }
@@ -1016,34 +1019,45 @@ __device__ __forceinline__ uint32_t sat_merge_word(uint32_t old, const uint32_t 
 
 // Work items hold at most this many records (their order lives in LDS as u16).
 constexpr uint32_t kMaxItemRecords = 4096;
+// Workgroup sizes of the bin kernels (their launch bounds).
+constexpr int kInsertThreads = 1024, kQueryThreads = 512;
 
 // Orders the records [rb, rb + n) of a work item by k-mer count, longest
 // first, into s_perm (indices relative to rb). A wave's s-mer loop runs as
 // long as its longest record; with the lanes of a wave on records of equal
 // length no lane idles. Lanes of equal length are matched with five ballots
-// and counted by one leader each, so the LDS atomics never collide.
+// and counted by one leader each, so the LDS atomics never collide. R rounds
+// of nthr threads cover the largest item; all R keys are loaded up front.
+template <int R>
 __device__ __forceinline__ void order_by_length(const uint64_t *sorted, uint32_t rb, uint32_t n, uint16_t *s_perm,
                                                 uint32_t *s_cls, uint32_t tid, uint32_t nthr) {
     const uint32_t lane = tid & 63;
     const uint64_t lt = (uint64_t(1) << lane) - 1;
     if (tid < 32) s_cls[tid] = 0;
+    uint32_t key[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {  // longest first; clamped index keeps the load in bounds
+        const uint32_t i = min(r * nthr + tid, n ? n - 1 : 0);
+        key[r] = n ? 31 - uint32_t((sorted[rb + i] >> 32) & 31) : 0;
+    }
     __syncthreads();
-#pragma unroll 1
+#pragma unroll
     for (int pass = 0; pass < 2; ++pass) {
-        for (uint32_t i0 = 0; i0 < n; i0 += nthr) {
-            const uint32_t i = i0 + tid;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (uint32_t(r) * nthr >= n) break;  // uniform
+            const uint32_t i = r * nthr + tid;
             const bool act = i < n;
-            const uint32_t key = act ? 31 - uint32_t((sorted[rb + i] >> 32) & 31) : 0;  // longest first
             uint64_t m = __ballot(act);
 #pragma unroll
             for (int bit = 0; bit < 5; ++bit) {
-                const uint64_t b = __ballot((key >> bit) & 1);
-                m &= ((key >> bit) & 1) ? b : ~b;
+                const uint64_t b = __ballot((key[r] >> bit) & 1);
+                m &= ((key[r] >> bit) & 1) ? b : ~b;
             }
             const uint32_t rank = uint32_t(__popcll(m & lt));
             const uint32_t leader = act ? uint32_t(__builtin_ctzll(m)) : 0;
             uint32_t base = 0;
-            if (act && rank == 0) base = atomicAdd(&s_cls[key], uint32_t(__popcll(m)));
+            if (act && rank == 0) base = atomicAdd(&s_cls[key[r]], uint32_t(__popcll(m)));
             base = __shfl(base, int(leader), 64);
             if (pass == 1 && act) s_perm[base + rank] = uint16_t(i);
         }
@@ -1059,6 +1073,48 @@ __device__ __forceinline__ void order_by_length(const uint64_t *sorted, uint32_t
             }
             __syncthreads();
         }
+    }
+}
+
+// The five packed words holding bases [b & ~15, (b & ~15) + 80) and the four
+// words of 16 bases each starting exactly at base b.
+__device__ __forceinline__ void load_words5(const uint32_t *pk, uint64_t b, uint32_t (&W)[5]) {
+    const uint64_t wi = b >> 4;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) W[q] = pk[wi + q];
+}
+__device__ __forceinline__ void align_words(const uint32_t (&W)[5], uint64_t b, uint32_t &A0, uint32_t &A1,
+                                            uint32_t &A2, uint32_t &A3) {
+    const uint32_t sh = 2 * uint32_t(b & 15);
+    A0 = __builtin_amdgcn_alignbit(W[1], W[0], sh);
+    A1 = __builtin_amdgcn_alignbit(W[2], W[1], sh);
+    A2 = __builtin_amdgcn_alignbit(W[3], W[2], sh);
+    A3 = __builtin_amdgcn_alignbit(W[4], W[3], sh);
+}
+
+// Walks the records [rb, rb + n) of a work item in the order of s_perm, one
+// per thread per round, calling f(record, packed words). The next record and
+// its packed words are in flight while the current one is processed: a record
+// and its bases are two dependent global loads, which would otherwise stall
+// every round. Indices past the end are clamped, so every load is in bounds.
+template <class F>
+__device__ __forceinline__ void walk_records(const uint64_t *sorted, const uint32_t *pk, uint32_t rb, uint32_t n,
+                                             const uint16_t *s_perm, uint32_t tid, uint32_t nthr, F &&f) {
+    if (tid >= n) return;
+    const uint32_t last = n - 1;
+    uint64_t rc = sorted[rb + s_perm[tid]];
+    uint64_t rn = sorted[rb + s_perm[min(tid + nthr, last)]];
+    uint32_t W[5];
+    load_words5(pk, rc & 0xFFFFFFFFull, W);
+    for (uint32_t i = tid; i < n; i += nthr) {
+        uint32_t V[5];
+        load_words5(pk, rn & 0xFFFFFFFFull, V);
+        const uint64_t rnn = sorted[rb + s_perm[min(i + 2 * nthr, last)]];
+        f(rc, W);
+        rc = rn;
+        rn = rnn;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) W[q] = V[q];
     }
 }
 
@@ -1098,7 +1154,7 @@ __global__ __launch_bounds__(256) void k_zero_cold(const uint32_t *bin_count, ui
 // min(MAX, old + count), which equals applying the +1s one by one (saturating
 // increments commute).
 template <int BITS>
-__global__ __launch_bounds__(1024) void k_bin_insert(BinArgs a, DevGeo g) {
+__global__ __launch_bounds__(kInsertThreads) void k_bin_insert(BinArgs a, DevGeo g) {
     extern __shared__ __align__(16) uint32_t s_cnt[];
     __shared__ uint64_t s_item;
     __shared__ uint16_t s_perm[kMaxItemRecords];
@@ -1122,18 +1178,17 @@ __global__ __launch_bounds__(1024) void k_bin_insert(BinArgs a, DevGeo g) {
         uint32_t re = min(a.bin_off[bin] + cnt, rb + a.per_item);
         bool multi = cnt > a.per_item;
         zero_lds(s_cnt, bin_slots, tid, nthr);
-        order_by_length(a.sorted, rb, re - rb, s_perm, s_cls, tid, nthr);  // (synchronizes)
-        for (uint32_t i = tid; i < re - rb; i += nthr) {
-            uint64_t rec = a.sorted[rb + s_perm[i]];
-            uint64_t j = rec & 0xFFFFFFFFull;
-            uint32_t nk = uint32_t((rec >> 32) & 31) + 1;
-            uint32_t lbase = uint32_t((rec >> 37) & ((1u << lshift) - 1)) << g.bse;
+        order_by_length<kMaxItemRecords / kInsertThreads>(a.sorted, rb, re - rb, s_perm, s_cls, tid, nthr);  // (synchronizes)
+        walk_records(a.sorted, a.pk, rb, re - rb, s_perm, tid, nthr, [&](uint64_t rec, const uint32_t (&W)[5]) {
+            const uint64_t j = rec & 0xFFFFFFFFull;
+            const uint32_t nk = uint32_t((rec >> 32) & 31) + 1;
+            const uint32_t lbase = uint32_t((rec >> 37) & ((1u << lshift) - 1)) << g.bse;
             uint32_t A0, A1, A2, A3;
-            load_aligned(a.pk, j, A0, A1, A2, A3);
+            align_words(W, j, A0, A1, A2, A3);
             for_each_smer(A0, A1, A2, A3, nk + g.k - 1, g.s, [&](uint32_t, uint64_t smer) {
                 for_each_address(smer, g.nh, amask, [&](uint32_t ad) { atomicAdd(&s_cnt[lbase + ad], 1u); });
             });
-        }
+        });
         __syncthreads();
         // merge: 32-bit table words of this bin
         constexpr uint32_t per = 32 / BITS;
@@ -1201,7 +1256,7 @@ __global__ __launch_bounds__(1024) void k_bin_insert(BinArgs a, DevGeo g) {
 // the bin staged in LDS at native width, min over hashes per s-mer, sliding
 // min over k-s+1 s-mers per k-mer, abundances written at their window index.
 template <int BITS, int WINC, bool EXACT>
-__global__ __launch_bounds__(512) void k_bin_query(BinArgs a, DevGeo g) {
+__global__ __launch_bounds__(kQueryThreads) void k_bin_query(BinArgs a, DevGeo g) {
     extern __shared__ __align__(16) uint32_t s_tab[];
     __shared__ uint64_t s_item;
     __shared__ uint64_t s_red[3][8];
@@ -1230,14 +1285,13 @@ __global__ __launch_bounds__(512) void k_bin_query(BinArgs a, DevGeo g) {
         uint32_t re = min(a.bin_off[bin] + cnt, rb + a.per_item);
         const uint32_t *tw = reinterpret_cast<const uint32_t *>(a.table) + bin * uint64_t(nwords);
         stage_to_lds<8>(s_tab, tw, nwords, tid, nthr);
-        order_by_length(a.sorted, rb, re - rb, s_perm, s_cls, tid, nthr);  // (synchronizes)
-        for (uint32_t i = tid; i < re - rb; i += nthr) {
-            uint64_t rec = a.sorted[rb + s_perm[i]];
-            uint64_t j0 = rec & 0xFFFFFFFFull;
-            uint32_t nk = uint32_t((rec >> 32) & 31) + 1;
-            uint32_t lbase = uint32_t((rec >> 37) & ((1u << lshift) - 1)) << g.bse;
+        order_by_length<kMaxItemRecords / kQueryThreads>(a.sorted, rb, re - rb, s_perm, s_cls, tid, nthr);  // (synchronizes)
+        walk_records(a.sorted, a.pk, rb, re - rb, s_perm, tid, nthr, [&](uint64_t rec, const uint32_t (&W)[5]) {
+            const uint64_t j0 = rec & 0xFFFFFFFFull;
+            const uint32_t nk = uint32_t((rec >> 32) & 31) + 1;
+            const uint32_t lbase = uint32_t((rec >> 37) & ((1u << lshift) - 1)) << g.bse;
             uint32_t A0, A1, A2, A3;
-            load_aligned(a.pk, j0, A0, A1, A2, A3);
+            align_words(W, j0, A0, A1, A2, A3);
             WinMin<WINC> wm;
             T *out = reinterpret_cast<T *>(a.out);
             // per-record sums: at most 32 k-mers, so 8/16-bit estimates sum in 32 bits
@@ -1261,7 +1315,7 @@ __global__ __launch_bounds__(512) void k_bin_query(BinArgs a, DevGeo g) {
             queried += nk;
             positive += rec_pos;
             checksum += BITS < 32 ? uint64_t(rec_sum) : rec_sum64;
-        }
+        });
         __syncthreads();
     }
     uint64_t vals[3] = {queried, positive, checksum};
@@ -1974,7 +2028,7 @@ Status Engine::insert(const DeviceBatch &b, Method method, InsertResult &r) {
                                    words_per_bin);
             }
             size_t lds = (size_t(1) << bin_log) * 4;
-            int thr = int(tune_.insert_threads);
+            const int thr = kInsertThreads;
 #define SCB_INS(BITS)                                                                           \
     do {                                                                                        \
         s = set_lds(k_bin_insert<BITS>, lds);                                                   \
@@ -2050,7 +2104,7 @@ Status Engine::query(const DeviceBatch &b, Method method, void *d_values, uint32
             a.bin_log = bin_log;
             a.per_item = std::max<uint32_t>(1, std::min<uint32_t>(tune_.max_records_per_item, kMaxItemRecords));
             size_t lds = (size_t(1) << bin_log) * geo_.bytes;
-            int thr = int(tune_.query_threads);
+            const int thr = kQueryThreads;
 #define SCB_QRY(BITS, WC, EX)                                                                          \
     do {                                                                                               \
         s = set_lds(k_bin_query<BITS, WC, EX>, lds);                                                   \
