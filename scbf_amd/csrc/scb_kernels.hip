@@ -1259,7 +1259,7 @@ template <int BITS, int WINC, bool EXACT>
 __global__ __launch_bounds__(kQueryThreads) void k_bin_query(BinArgs a, DevGeo g) {
     extern __shared__ __align__(16) uint32_t s_tab[];
     __shared__ uint64_t s_item;
-    __shared__ uint64_t s_red[3][8];
+    __shared__ uint64_t s_red[3][kQueryThreads / 64];
     __shared__ uint16_t s_perm[kMaxItemRecords];
     __shared__ uint32_t s_cls[32];
     using T = typename Ctr<BITS>::T;
