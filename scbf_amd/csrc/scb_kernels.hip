@@ -533,22 +533,18 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
         __syncthreads();
 
         // (4) minimizer position of every window j = t0-1+hj, hj in [0, kTile + w)
+        // Branch-free: both tie rules read the same shape of data, so the rule
+        // only selects the arrays and the direction of the final comparison.
         for (int hj = tid; hj < kTile + w; hj += kTileThreads) {
-            int li = hj + kLeft - 1;
-            uint32_t bad = get_bits(s_bad, li, k) | get_bits(s_brk, li + 1, k - 1);
-            uint16_t pos = kNoWin;
-            if (!bad) {
-                uint32_t tg = __popc(get_bits(s_tg, li, k));
-                int e = hj + w - 1;
-                if (2 * int(tg) > k) {  // leftmost minimum (suffix candidate is the left one)
-                    uint32_t c1 = s_SL[hj], c2 = s_PL[e];
-                    pos = uint16_t(s_K[c1] <= s_K[c2] ? c1 : c2);
-                } else {  // rightmost minimum (prefix candidate is the right one)
-                    uint32_t c1 = s_SR[hj], c2 = s_PR[e];
-                    pos = uint16_t(s_K[c2] <= s_K[c1] ? c2 : c1);
-                }
-            }
-            s_mpos[hj] = pos;
+            const int li = hj + kLeft - 1;
+            const uint32_t bad = get_bits(s_bad, li, k) | get_bits(s_brk, li + 1, k - 1);
+            const bool left = 2 * int(__popc(get_bits(s_tg, li, k))) > k;  // leftmost minimum on ties
+            const int e = hj + w - 1;
+            const uint32_t c1 = (left ? s_SL : s_SR)[hj], c2 = (left ? s_PL : s_PR)[e];
+            const uint32_t k1 = s_K[c1], k2 = s_K[c2];
+            // leftmost: the suffix candidate c1 (left of c2) wins ties; rightmost: c2 wins ties
+            const uint32_t pos = left ? (k1 <= k2 ? c1 : c2) : (k2 <= k1 ? c2 : c1);
+            s_mpos[hj] = bad ? kNoWin : uint16_t(pos);
         }
         __syncthreads();
 
