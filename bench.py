@@ -256,6 +256,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-host", action="store_true")
     ap.add_argument("--serve-qps", default="100,400")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--copy-probe", action="store_true",
@@ -379,6 +380,25 @@ def main():
         roof["traffic_source"] = os.path.relpath(PMC_PROFILE, ROOT)
         roof["valu_busy"] = kp.get("valu_busy")
 
+    # host-buffer path (PCIe-inclusive): the crate-style calls on numpy arrays in host memory,
+    # abundances returned as u64 + validity bytes per position
+    host_api = None
+    if not args.no_host:
+        torch.cuda.synchronize()
+        th0 = time.perf_counter()
+        hb = scb.SuperCountingBloomBuilder(cfg)
+        hrep = hb.add_sequences((ins_b, ins_o))
+        hidx = hb.into_index()
+        th1 = time.perf_counter()
+        _, _, hq = hidx.estimate_sequences(q_b, q_o)
+        th2 = time.perf_counter()
+        host_api = {"value": round((hrep.inserted_kmers + hq.valid_kmers) / (th2 - th0) / 1e9, 3),
+                    "unit": "G k-mers/s", "insert_s": round(th1 - th0, 4), "query_s": round(th2 - th1, 4),
+                    "note": "scb_builder_add_sequences + scb_index_estimate_sequences on host arrays "
+                            "(PCIe transfers and u64/u8 result expansion included)"}
+        log(f"host-buffer path: {host_api}")
+        del hidx
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(ins_b, ins_o, q_b, q_o)
@@ -401,6 +421,7 @@ def main():
         "bit_exact_batched_vs_plain": bit_exact,
         "roofline": roof,
         "served": served,
+        "host_buffers": host_api,
         "served_p50_ms": served["steps"][0]["p50_ms"] if served else None,
         "served_p99_ms": served["steps"][0]["p99_ms"] if served else None,
         "cpu_baseline": cpu,

@@ -258,17 +258,41 @@ namespace {
 
 uint64_t sat_add(uint64_t a, uint64_t b) { return a + b < a ? UINT64_MAX : a + b; }
 
+// Records with at least one valid k-mer (api.rs:638-640), counted by host
+// threads over record ranges.
+uint64_t count_indexed(const uint8_t *bases, const uint64_t *offsets, size_t n, size_t k) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned nt = offsets[n] - offsets[0] < (uint64_t(1) << 22) ? 1u : std::min(16u, hw);
+    std::vector<uint64_t> part(nt, 0);
+    auto run = [&](unsigned t) {
+        const size_t i0 = n * t / nt, i1 = n * (t + 1) / nt;
+        uint64_t c = 0;
+        for (size_t i = i0; i < i1; ++i)
+            c += valid_runs(bases + offsets[i], size_t(offsets[i + 1] - offsets[i]), k).second > 0;
+        part[t] = c;
+    };
+    std::vector<std::thread> ts;
+    for (unsigned t = 1; t < nt; ++t) ts.emplace_back(run, t);
+    run(0);
+    for (auto &t : ts) t.join();
+    uint64_t c = 0;
+    for (uint64_t v : part) c += v;
+    return c;
+}
+
 int builder_insert_host(scb_builder *b, const uint8_t *bases, const uint64_t *offsets, size_t n,
                         scb_add_report *rep) {
     scb::InsertResult r;
+    // the host count overlaps the device insert
+    uint64_t indexed = 0;
+    std::thread counter;
+    if (rep) counter = std::thread([&] { indexed = count_indexed(bases, offsets, n, size_t(b->cfg.k)); });
     Status s = b->engine->insert_host(bases, offsets, n, b->method, r);
+    if (counter.joinable()) counter.join();
     if (!s.good()) return fail(s);
     if (rep) {
         rep->records_processed += n;
-        for (size_t i = 0; i < n; ++i) {
-            auto vr = valid_runs(bases + offsets[i], size_t(offsets[i + 1] - offsets[i]), size_t(b->cfg.k));
-            if (vr.second > 0) ++rep->records_indexed;  // api.rs:638-640
-        }
+        rep->records_indexed += indexed;
         rep->inserted_kmers += r.inserted_kmers;
         rep->inserted_smers += r.inserted_smers;
     }

@@ -31,6 +31,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 
 #include "scb_engine.h"
 
@@ -1586,6 +1587,26 @@ struct DevBuf {
     }
 };
 
+// Page-locked host buffer that only grows (host side of batch copies).
+template <class T>
+struct HostBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    hipError_t reserve(size_t want) {
+        if (want <= n) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&p), std::max<size_t>(want, 1) * sizeof(T),
+                                     hipHostMallocDefault);
+        if (e == hipSuccess) n = want;
+        return e;
+    }
+    ~HostBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
 }  // namespace
 
 class Workspace {
@@ -1598,6 +1619,10 @@ class Workspace {
     DevBuf<uint8_t> h_bases_dev;   // staging for host batches
     DevBuf<uint64_t> h_offsets_dev;
     DevBuf<uint8_t> out_dev;
+    DevBuf<uint32_t> out_bits;
+    HostBuf<uint8_t> h_vals;       // query results on their way back
+    HostBuf<uint32_t> h_bits;
+    HostBuf<uint64_t> h_offs;
     DevBuf<unsigned long long> hist_dense, hist_big_counts;
     DevBuf<uint32_t> hist_big_keys;
     DevBuf<uint32_t> spec_bin_kmers;
@@ -2352,103 +2377,132 @@ Status Engine::insert_host(const uint8_t *bases, const uint64_t *offsets, uint64
                            InsertResult &r) {
     r = InsertResult{};
     SCB_HIP(hipSetDevice(device_));
-    uint64_t limit = std::min<uint64_t>(kMaxBatchBases, uint64_t(1) << 30);
-    DevBuf<uint8_t> db;
-    DevBuf<uint64_t> doff;
-    std::vector<uint64_t> loc;
-    return for_each_chunk(offsets, n, limit, [&](uint64_t r0, uint64_t r1) -> Status {
-        uint64_t nb = offsets[r1] - offsets[r0];
-        SCB_HIP(db.reserve(nb + 4));
-        SCB_HIP(doff.reserve(r1 - r0 + 1));
-        loc.resize(r1 - r0 + 1);
-        for (uint64_t i = r0; i <= r1; ++i) loc[i - r0] = offsets[i] - offsets[r0];
-        SCB_HIP(hipMemcpy(db.p, bases + offsets[r0], nb, hipMemcpyHostToDevice));
-        SCB_HIP(hipMemcpy(doff.p, loc.data(), loc.size() * 8, hipMemcpyHostToDevice));
-        DeviceBatch b{db.p, doff.p, r1 - r0, nb};
+    auto st = acquire();  // staging buffers, kept with the pooled workspace
+    if (!st) return Status{8, "device error: cannot create a workspace"};
+    const uint64_t limit = std::min<uint64_t>(kMaxBatchBases, uint64_t(1) << 30);
+    Status s = for_each_chunk(offsets, n, limit, [&](uint64_t r0, uint64_t r1) -> Status {
+        const uint64_t nb = offsets[r1] - offsets[r0];
+        SCB_HIP(st->h_bases_dev.reserve(nb + 4));
+        SCB_HIP(st->h_offsets_dev.reserve(r1 - r0 + 1));
+        SCB_HIP(st->h_offs.reserve(r1 - r0 + 1));
+        for (uint64_t i = r0; i <= r1; ++i) st->h_offs.p[i - r0] = offsets[i] - offsets[r0];
+        SCB_HIP(hipMemcpy(st->h_bases_dev.p, bases + offsets[r0], nb, hipMemcpyHostToDevice));
+        SCB_HIP(hipMemcpy(st->h_offsets_dev.p, st->h_offs.p, (r1 - r0 + 1) * 8, hipMemcpyHostToDevice));
+        DeviceBatch b{st->h_bases_dev.p, st->h_offsets_dev.p, r1 - r0, nb};
         InsertResult part;
-        Status s = insert(b, method, part);
-        if (!s.good()) return s;
+        Status ps = insert(b, method, part);
+        if (!ps.good()) return ps;
         r.inserted_kmers += part.inserted_kmers;
         r.inserted_smers += part.inserted_smers;
         r.super_kmers += part.super_kmers;
         return Status::ok();
     });
+    release(std::move(st));
+    return s;
 }
+
+namespace {
+// Runs fn(i0, i1) over [0, n) in up to 32 host threads (chunks of a multiple of
+// 32 positions, so no two threads share a validity word).
+template <class Fn>
+void parallel_ranges(uint64_t n, Fn &&fn) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned nt = n < (uint64_t(1) << 20) ? 1u : std::min(32u, hw);
+    if (nt == 1) {
+        fn(uint64_t(0), n);
+        return;
+    }
+    const uint64_t step = ((n + nt - 1) / nt + 31) & ~uint64_t(31);
+    std::vector<std::thread> ts;
+    for (unsigned t = 0; t < nt; ++t) {
+        const uint64_t i0 = std::min(n, t * step), i1 = std::min(n, i0 + step);
+        if (i0 < i1) ts.emplace_back([&fn, i0, i1] { fn(i0, i1); });
+    }
+    for (auto &t : ts) t.join();
+}
+}  // namespace
 
 Status Engine::query_host(const uint8_t *bases, const uint64_t *offsets, uint64_t n, Method method,
                           uint64_t *values, uint8_t *valid, QueryResult &r) const {
     r = QueryResult{};
     SCB_HIP(hipSetDevice(device_));
-    uint64_t limit = std::min<uint64_t>(kMaxBatchBases, uint64_t(1) << 29);
-    DevBuf<uint8_t> db, dv;
-    DevBuf<uint64_t> doff;
-    DevBuf<uint32_t> dbits;
-    std::vector<uint64_t> loc;
-    std::vector<uint8_t> hv;
-    std::vector<uint32_t> hb;
+    auto st = acquire();  // staging buffers, kept with the pooled workspace
+    if (!st) return Status{8, "device error: cannot create a workspace"};
+    const uint64_t limit = std::min<uint64_t>(kMaxBatchBases, uint64_t(1) << 29);
     const uint32_t W = geo_.bytes;
-    return for_each_chunk(offsets, n, limit, [&](uint64_t r0, uint64_t r1) -> Status {
-        uint64_t nb = offsets[r1] - offsets[r0];
-        SCB_HIP(db.reserve(nb + 4));
-        SCB_HIP(doff.reserve(r1 - r0 + 1));
-        bool want = values || valid;
+    Status s = for_each_chunk(offsets, n, limit, [&](uint64_t r0, uint64_t r1) -> Status {
+        const uint64_t nb = offsets[r1] - offsets[r0];
+        SCB_HIP(st->h_bases_dev.reserve(nb + 4));
+        SCB_HIP(st->h_offsets_dev.reserve(r1 - r0 + 1));
+        SCB_HIP(st->h_offs.reserve(r1 - r0 + 1));
+        const bool want = values || valid;
         if (want) {
-            SCB_HIP(dv.reserve(nb * W + 4));
-            SCB_HIP(dbits.reserve((nb + 31) / 32 + 1));
+            SCB_HIP(st->out_dev.reserve(nb * W + 4));
+            SCB_HIP(st->out_bits.reserve((nb + 31) / 32 + 1));
+            SCB_HIP(st->h_vals.reserve(nb * W + 4));
+            SCB_HIP(st->h_bits.reserve((nb + 31) / 32 + 1));
         }
-        loc.resize(r1 - r0 + 1);
-        for (uint64_t i = r0; i <= r1; ++i) loc[i - r0] = offsets[i] - offsets[r0];
-        SCB_HIP(hipMemcpy(db.p, bases + offsets[r0], nb, hipMemcpyHostToDevice));
-        SCB_HIP(hipMemcpy(doff.p, loc.data(), loc.size() * 8, hipMemcpyHostToDevice));
-        DeviceBatch b{db.p, doff.p, r1 - r0, nb};
+        for (uint64_t i = r0; i <= r1; ++i) st->h_offs.p[i - r0] = offsets[i] - offsets[r0];
+        SCB_HIP(hipMemcpy(st->h_bases_dev.p, bases + offsets[r0], nb, hipMemcpyHostToDevice));
+        SCB_HIP(hipMemcpy(st->h_offsets_dev.p, st->h_offs.p, (r1 - r0 + 1) * 8, hipMemcpyHostToDevice));
+        DeviceBatch b{st->h_bases_dev.p, st->h_offsets_dev.p, r1 - r0, nb};
         QueryResult part;
-        Status s = query(b, method, want ? dv.p : nullptr, want ? dbits.p : nullptr, part);
-        if (!s.good()) return s;
+        Status qs = query(b, method, want ? st->out_dev.p : nullptr, want ? st->out_bits.p : nullptr, part);
+        if (!qs.good()) return qs;
         r.queried += part.queried;
         r.positive += part.positive;
         r.checksum += part.checksum;
         r.super_kmers += part.super_kmers;
         if (want) {
-            hv.resize(nb * W);
-            hb.resize((nb + 31) / 32);
-            SCB_HIP(hipMemcpy(hv.data(), dv.p, nb * W, hipMemcpyDeviceToHost));
-            SCB_HIP(hipMemcpy(hb.data(), dbits.p, hb.size() * 4, hipMemcpyDeviceToHost));
-            uint64_t o = offsets[r0];
-            for (uint64_t i = 0; i < nb; ++i) {
-                bool ok = (hb[i >> 5] >> (i & 31)) & 1;
-                if (valid) valid[o + i] = ok;
-                if (values) {
-                    uint64_t v = 0;
-                    if (ok) {
-                        if (W == 1) v = hv[i];
-                        else if (W == 2) v = reinterpret_cast<const uint16_t *>(hv.data())[i];
-                        else v = reinterpret_cast<const uint32_t *>(hv.data())[i];
+            SCB_HIP(hipMemcpy(st->h_vals.p, st->out_dev.p, nb * W, hipMemcpyDeviceToHost));
+            SCB_HIP(hipMemcpy(st->h_bits.p, st->out_bits.p, (nb + 31) / 32 * 4, hipMemcpyDeviceToHost));
+            const uint64_t o = offsets[r0];
+            const uint8_t *hv = st->h_vals.p;
+            const uint32_t *hb = st->h_bits.p;
+            // position-indexed u64 values (0 where no valid window) and validity bytes
+            parallel_ranges(nb, [&](uint64_t i0, uint64_t i1) {
+                for (uint64_t i = i0; i < i1; ++i) {
+                    const bool ok = (hb[i >> 5] >> (i & 31)) & 1;
+                    if (valid) valid[o + i] = ok;
+                    if (values) {
+                        uint64_t v = 0;
+                        if (ok) {
+                            if (W == 1) v = hv[i];
+                            else if (W == 2) v = reinterpret_cast<const uint16_t *>(hv)[i];
+                            else v = reinterpret_cast<const uint32_t *>(hv)[i];
+                        }
+                        values[o + i] = v;
                     }
-                    values[o + i] = v;
                 }
-            }
+            });
         }
         return Status::ok();
     });
+    release(std::move(st));
+    return s;
 }
 
 Status Engine::spectrum_host(const uint8_t *bases, const uint64_t *offsets, uint64_t n,
                              SpectrumResult &r) const {
     r = SpectrumResult{};
     SCB_HIP(hipSetDevice(device_));
-    uint64_t nb = offsets[n] - offsets[0];
+    const uint64_t nb = offsets[n] - offsets[0];
     if (nb >= kMaxBatchBases)
         return Status{6, "internal state error: spectrum query exceeds 2^32 bases in one call"};
-    DevBuf<uint8_t> db;
-    DevBuf<uint64_t> doff;
-    SCB_HIP(db.reserve(nb + 4));
-    SCB_HIP(doff.reserve(n + 1));
-    std::vector<uint64_t> loc(n + 1);
-    for (uint64_t i = 0; i <= n; ++i) loc[i] = offsets[i] - offsets[0];
-    SCB_HIP(hipMemcpy(db.p, bases + offsets[0], nb, hipMemcpyHostToDevice));
-    SCB_HIP(hipMemcpy(doff.p, loc.data(), loc.size() * 8, hipMemcpyHostToDevice));
-    DeviceBatch b{db.p, doff.p, n, nb};
-    return spectrum(b, r);
+    auto st = acquire();
+    if (!st) return Status{8, "device error: cannot create a workspace"};
+    Status s = [&]() -> Status {
+        SCB_HIP(st->h_bases_dev.reserve(nb + 4));
+        SCB_HIP(st->h_offsets_dev.reserve(n + 1));
+        SCB_HIP(st->h_offs.reserve(n + 1));
+        for (uint64_t i = 0; i <= n; ++i) st->h_offs.p[i] = offsets[i] - offsets[0];
+        SCB_HIP(hipMemcpy(st->h_bases_dev.p, bases + offsets[0], nb, hipMemcpyHostToDevice));
+        SCB_HIP(hipMemcpy(st->h_offsets_dev.p, st->h_offs.p, (n + 1) * 8, hipMemcpyHostToDevice));
+        DeviceBatch b{st->h_bases_dev.p, st->h_offsets_dev.p, n, nb};
+        return spectrum(b, r);
+    }();
+    release(std::move(st));
+    return s;
 }
 
 StageTimes last_stage_times() { return g_stage_times; }
