@@ -17,7 +17,7 @@
 //   k_scan      per 2048-window tile: ASCII -> codes, ACGT-run and read-boundary
 //               masks, canonical ntHash of every m-mer, minimizer of every k-mer
 //               window, super-k-mer cut, block hash -> rec + bin histogram
-//   k_bin_plan  one workgroup: bin offsets + work list (hot bins split)
+//   k_plan_sums / k_plan_write  bin offsets + work list (hot bins split)
 //   k_partition rec -> records grouped by bin (tile-aggregated counting sort,
 //               one or two passes)
 //   k_bin_insert / k_bin_query  persistent work queue, one bin per item, counters
@@ -715,65 +715,78 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
 // Bin planning: exclusive scan of per-bin record counts and the work list.
 // One workgroup of 1024 threads.
 // ----------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_bin_plan(const uint32_t *__restrict__ bin_count, uint64_t nbins,
-                                                   uint32_t *__restrict__ bin_off, uint32_t *__restrict__ cursor,
-                                                   uint64_t *__restrict__ work, uint32_t per_item,
-                                                   uint64_t *__restrict__ stats) {
-    constexpr int kRegBins = 32;  // runs of up to 32 bins per thread stay in registers
-    __shared__ uint64_t s_rec[1024], s_items[1024];
-    const int tid = threadIdx.x;
-    const uint64_t chunk = (nbins + 1023) / 1024;
-    const uint64_t b0 = tid * chunk, b1 = min(nbins, b0 + chunk);
-    const bool in_regs = chunk <= kRegBins;
-    uint32_t c[kRegBins];
-    uint64_t rsum = 0, isum = 0;
-    if (in_regs) {
-#pragma unroll
-        for (int i = 0; i < kRegBins; ++i) c[i] = b0 + i < b1 ? bin_count[b0 + i] : 0;  // all loads in flight
-#pragma unroll
-        for (int i = 0; i < kRegBins; ++i) rsum += c[i], isum += (c[i] + per_item - 1) / per_item;
-    } else {
-        for (uint64_t b = b0; b < b1; ++b) {
-            uint32_t x = bin_count[b];
-            rsum += x;
-            isum += (x + per_item - 1) / per_item;
-        }
+// Bin plan in two launches of nbins/1024 workgroups, one bin per thread: block
+// sums of records and work items, then each block adds the sums of the blocks
+// before it to its own exclusive scan and writes offsets, cursors and its part
+// of the work list (a bin above per_item records becomes several items).
+constexpr int kPlanThreads = 1024;
+
+__device__ __forceinline__ uint32_t items_of(uint32_t c, uint32_t per_item, uint32_t per_shift) {
+    return per_shift < 32 ? (c + per_item - 1) >> per_shift : (c + per_item - 1) / per_item;
+}
+
+// Inclusive scan over the workgroup of two values; returns the totals.
+__device__ __forceinline__ void block_scan2(uint64_t &a, uint64_t &b, uint64_t *s_a, uint64_t *s_b, uint32_t tid,
+                                           uint64_t &tot_a, uint64_t &tot_b) {
+    const uint32_t lane = tid & 63, wid = tid >> 6;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t ya = __shfl_up(a, off, 64), yb = __shfl_up(b, off, 64);
+        if (lane >= uint32_t(off)) a += ya, b += yb;
     }
-    s_rec[tid] = rsum, s_items[tid] = isum;
+    if (lane == 63) s_a[wid] = a, s_b[wid] = b;
     __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {
-        uint64_t vr = tid >= d ? s_rec[tid - d] : 0, vi = tid >= d ? s_items[tid - d] : 0;
-        __syncthreads();
-        s_rec[tid] += vr, s_items[tid] += vi;
-        __syncthreads();
+    uint64_t pa = 0, pb = 0;
+    tot_a = tot_b = 0;
+    for (uint32_t w = 0; w < kPlanThreads / 64; ++w) {
+        if (w < wid) pa += s_a[w], pb += s_b[w];
+        tot_a += s_a[w], tot_b += s_b[w];
     }
-    uint64_t ro = s_rec[tid] - rsum, io = s_items[tid] - isum;
-    if (in_regs) {
-#pragma unroll
-        for (int i = 0; i < kRegBins; ++i) {
-            const uint64_t b = b0 + i;
-            if (b < b1) {
-                bin_off[b] = uint32_t(ro);
-                cursor[b] = uint32_t(ro);
-                const uint32_t parts = (c[i] + per_item - 1) / per_item;
-                for (uint32_t p = 0; p < parts; ++p) work[io++] = (b << 32) | p;
-                ro += c[i];
-            }
-        }
-    } else {
-        for (uint64_t b = b0; b < b1; ++b) {
-            uint32_t x = bin_count[b];
-            bin_off[b] = uint32_t(ro);
-            cursor[b] = uint32_t(ro);
-            uint32_t parts = (x + per_item - 1) / per_item;
-            for (uint32_t p = 0; p < parts; ++p) work[io++] = (b << 32) | p;
-            ro += x;
-        }
+    a += pa, b += pb;
+}
+
+__global__ __launch_bounds__(kPlanThreads) void k_plan_sums(const uint32_t *__restrict__ bin_count, uint64_t nbins,
+                                                            uint32_t per_item, uint32_t per_shift,
+                                                            uint64_t *__restrict__ blk) {
+    __shared__ uint64_t s_a[kPlanThreads / 64], s_b[kPlanThreads / 64];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t bin = uint64_t(blockIdx.x) * kPlanThreads + tid;
+    const uint32_t c = bin < nbins ? bin_count[bin] : 0;
+    uint64_t a = c, b = items_of(c, per_item, per_shift), ta, tb;
+    block_scan2(a, b, s_a, s_b, tid, ta, tb);
+    if (tid == 0) blk[2 * blockIdx.x] = ta, blk[2 * blockIdx.x + 1] = tb;
+}
+
+__global__ __launch_bounds__(kPlanThreads) void k_plan_write(const uint32_t *__restrict__ bin_count, uint64_t nbins,
+                                                             uint32_t per_item, uint32_t per_shift,
+                                                             const uint64_t *__restrict__ blk,
+                                                             uint32_t *__restrict__ bin_off,
+                                                             uint32_t *__restrict__ cursor,
+                                                             uint64_t *__restrict__ work,
+                                                             uint64_t *__restrict__ stats) {
+    __shared__ uint64_t s_a[kPlanThreads / 64], s_b[kPlanThreads / 64];
+    __shared__ uint64_t s_base[2];
+    const uint32_t tid = threadIdx.x;
+    if (tid < 64) {  // records and items of the blocks before this one
+        uint64_t ra = 0, rb = 0;
+        for (uint32_t q = tid; q < blockIdx.x; q += 64) ra += blk[2 * q], rb += blk[2 * q + 1];
+        for (int off = 32; off > 0; off >>= 1) ra += __shfl_down(ra, off, 64), rb += __shfl_down(rb, off, 64);
+        if (tid == 0) s_base[0] = ra, s_base[1] = rb;
     }
-    if (tid == 1023) {
-        stats[kStNWork] = s_items[1023];
+    const uint64_t bin = uint64_t(blockIdx.x) * kPlanThreads + tid;
+    const uint32_t c = bin < nbins ? bin_count[bin] : 0;
+    const uint32_t parts = items_of(c, per_item, per_shift);
+    uint64_t a = c, b = parts, ta, tb;
+    block_scan2(a, b, s_a, s_b, tid, ta, tb);  // (synchronizes: s_base is visible)
+    const uint64_t ro = s_base[0] + a - c, io = s_base[1] + b - parts;
+    if (bin < nbins) {
+        bin_off[bin] = uint32_t(ro);
+        cursor[bin] = uint32_t(ro);
+        for (uint32_t p = 0; p < parts; ++p) work[io + p] = (bin << 32) | p;
+    }
+    if (blockIdx.x == gridDim.x - 1 && tid == kPlanThreads - 1) {
+        stats[kStNWork] = s_base[1] + tb;
         stats[kStWorkHead] = 0;
-        stats[kStRecords] = s_rec[1023];
+        stats[kStRecords] = s_base[0] + ta;
     }
 }
 
@@ -1613,7 +1626,7 @@ class Workspace {
    public:
     hipStream_t stream = nullptr;
     DevBuf<uint32_t> pk, valid;
-    DevBuf<uint64_t> rec, sorted, work, stats;
+    DevBuf<uint64_t> rec, sorted, work, stats, plan_blk;
     DevBuf<uint32_t> bin_count, bin_off, cursor, ccursor, coarse_hist, seg_count, tile_r0;
     uint64_t *final_sorted = nullptr;  // records grouped by bin after the partition passes
     DevBuf<uint8_t> h_bases_dev;   // staging for host batches
@@ -1900,6 +1913,15 @@ Status launch_scan(Launch &L, const DeviceBatch &b, uint64_t seg_cap, uint32_t *
     return Status::ok();
 }
 
+void launch_plan(Workspace &ws, uint64_t nbins, uint32_t per_item, hipStream_t st) {
+    const unsigned nblk = unsigned((nbins + kPlanThreads - 1) / kPlanThreads);
+    const uint32_t per_shift = (per_item & (per_item - 1)) == 0 ? uint32_t(__builtin_ctz(per_item)) : 32u;
+    hipLaunchKernelGGL(k_plan_sums, dim3(nblk), dim3(kPlanThreads), 0, st, ws.bin_count.p, nbins, per_item, per_shift,
+                       ws.plan_blk.p);
+    hipLaunchKernelGGL(k_plan_write, dim3(nblk), dim3(kPlanThreads), 0, st, ws.bin_count.p, nbins, per_item,
+                       per_shift, ws.plan_blk.p, ws.bin_off.p, ws.cursor.p, ws.work.p, ws.stats.p);
+}
+
 // Batched front half shared by insert, query and spectrum: scan to records,
 // then group the records by bin (one or two tile-aggregated counting-sort
 // passes) and build the work list.
@@ -1920,6 +1942,7 @@ Status batched_front(Launch &L, const DeviceBatch &b, uint32_t bin_log, uint32_t
     SCB_HIP(ws.bin_off.reserve(nbins));
     SCB_HIP(ws.cursor.reserve(nbins));
     SCB_HIP(ws.work.reserve(nbins + cap / per_item + 2));
+    SCB_HIP(ws.plan_blk.reserve(2 * ((nbins + kPlanThreads - 1) / kPlanThreads)));
     hipStream_t st = ws.stream;
     fill(reinterpret_cast<uint32_t *>(ws.stats.p), 2 * kStatsLen, 0, st);
     fill(ws.seg_count.p, kSeg + 1, 0, st);
@@ -1950,15 +1973,13 @@ Status batched_front(Launch &L, const DeviceBatch &b, uint32_t bin_log, uint32_t
         SegIn comp{ws.sorted.p, ws.seg_count.p + kSeg, 1, cap};
         KeySpec fine{bin_shift, 0, fine_bits, 1, 1};
         hipLaunchKernelGGL(k_seg_hist, pg, pb, 0, st, comp, fine, ws.bin_count.p, bin_kmers);
-        hipLaunchKernelGGL(k_bin_plan, dim3(1), dim3(1024), 0, st, ws.bin_count.p, nbins, ws.bin_off.p,
-                           ws.cursor.p, ws.work.p, per_item, ws.stats.p);
+        launch_plan(ws, nbins, per_item, st);
         hipLaunchKernelGGL(k_partition, pg, pb, 0, st, comp, fine, ws.cursor.p, ws.rec.p);
         ws.final_sorted = ws.rec.p;
     } else {
         KeySpec one{bin_shift, 0, 0, 0, 1};
         hipLaunchKernelGGL(k_seg_hist, pg, pb, 0, st, segs, one, ws.bin_count.p, bin_kmers);
-        hipLaunchKernelGGL(k_bin_plan, dim3(1), dim3(1024), 0, st, ws.bin_count.p, nbins, ws.bin_off.p,
-                           ws.cursor.p, ws.work.p, per_item, ws.stats.p);
+        launch_plan(ws, nbins, per_item, st);
         hipLaunchKernelGGL(k_partition, pg, pb, 0, st, segs, one, ws.cursor.p, ws.sorted.p);
         ws.final_sorted = ws.sorted.p;
     }
