@@ -854,6 +854,9 @@ __device__ __forceinline__ uint32_t tile_segment(const uint32_t *s_pre, uint32_t
 }
 
 // Histogram of keys (records per key, and k-mers per key when kmers != null).
+// The workgroup's tiles u = blockIdx.x + i * gridDim.x share u % copies (the
+// grid is a multiple of the copy count), so counts accumulate in LDS across
+// tiles and are flushed once per run of tiles with the same key window.
 __global__ __launch_bounds__(kPartThreads) void k_seg_hist(SegIn in, KeySpec ks, uint32_t *hist,
                                                            uint32_t *kmers) {
     __shared__ uint32_t s_h[kPartRange], s_k[kPartRange];
@@ -861,16 +864,30 @@ __global__ __launch_bounds__(kPartThreads) void k_seg_hist(SegIn in, KeySpec ks,
     __shared__ uint32_t s_pre[65];
     const uint32_t tid = threadIdx.x;
     const uint32_t total = dense_tiles(in, s_pre, tid);
+    const uint32_t copy = blockIdx.x % ks.copies;
+    uint32_t lo = 0xFFFFFFFFu;  // key window of the counts held in LDS (none yet)
+    auto flush = [&]() {
+        for (uint32_t i = tid; i < kPartRange; i += kPartThreads) {
+            if (s_h[i]) atomicAdd(&hist[(lo + i) * ks.copies + copy], s_h[i]);
+            if (kmers && s_k[i]) atomicAdd(&kmers[lo + i], s_k[i]);
+            s_h[i] = 0, s_k[i] = 0;
+        }
+    };
+    for (uint32_t i = tid; i < kPartRange; i += kPartThreads) s_h[i] = 0, s_k[i] = 0;
     for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
         const uint32_t seg = tile_segment(s_pre, in.nseg, u);
         const uint64_t t0 = uint64_t(u - s_pre[seg]) * kPartTile;
         const uint64_t n = min<uint64_t>(in.counts[seg], in.seg_cap);
         const uint64_t *tile = in.src + seg * in.seg_cap + t0;
         const uint32_t cnt = uint32_t(min<uint64_t>(kPartTile, n - t0));
-        for (uint32_t i = tid; i < kPartRange; i += kPartThreads) s_h[i] = 0, s_k[i] = 0;
         if (tid == 0) s_lo = tile_lo(tile, ks);
         __syncthreads();
-        const uint32_t lo = s_lo;
+        const uint32_t tlo = s_lo;
+        if (tlo != lo) {  // uniform: a new key window
+            if (lo != 0xFFFFFFFFu) flush();
+            lo = tlo;
+            __syncthreads();
+        }
         for (uint32_t i = tid; i < cnt; i += kPartThreads) {
             uint64_t r = tile[i];
             uint32_t key = rec_bin(r, ks.bin_shift) >> ks.key_shift;
@@ -879,18 +896,13 @@ __global__ __launch_bounds__(kPartThreads) void k_seg_hist(SegIn in, KeySpec ks,
                 atomicAdd(&s_h[key - lo], 1u);
                 if (kmers) atomicAdd(&s_k[key - lo], nk);
             } else {
-                atomicAdd(&hist[key * ks.copies + u % ks.copies], 1u);
+                atomicAdd(&hist[key * ks.copies + copy], 1u);
                 if (kmers) atomicAdd(&kmers[key], nk);
             }
         }
         __syncthreads();
-        const uint32_t copy = uint32_t(u % ks.copies);
-        for (uint32_t i = tid; i < kPartRange; i += kPartThreads) {
-            if (s_h[i]) atomicAdd(&hist[(lo + i) * ks.copies + copy], s_h[i]);
-            if (kmers && s_k[i]) atomicAdd(&kmers[lo + i], s_k[i]);
-        }
-        __syncthreads();
     }
+    if (lo != 0xFFFFFFFFu) flush();
 }
 
 // Exclusive scan of a histogram into cursors (one workgroup; each thread scans
@@ -1955,6 +1967,9 @@ Status batched_front(Launch &L, const DeviceBatch &b, uint32_t bin_log, uint32_t
     mark(ws, 1);
     const int pgrid = cu_count(L.device) * 16;
     const dim3 pg(pgrid), pb(kPartThreads);
+    // histograms accumulate many tiles per workgroup; their grid is a multiple of
+    // the coarse copy count (each workgroup's tiles then share one copy)
+    const dim3 hg(unsigned((cu_count(L.device) * 6 + 31) / 32 * 32));
     SegIn segs{ws.rec.p, ws.seg_count.p, kSeg, seg_cap};
     if (nbins > kPartRange) {
         // coarse pass by bin >> fine_bits, then a fine pass by bin
@@ -1966,19 +1981,19 @@ Status batched_front(Launch &L, const DeviceBatch &b, uint32_t bin_log, uint32_t
         SCB_HIP(ws.ccursor.reserve(ncoarse * kCopies));
         fill(ws.coarse_hist.p, ncoarse * kCopies, 0, st);
         KeySpec coarse{bin_shift, fine_bits, 0, 0, kCopies};
-        hipLaunchKernelGGL(k_seg_hist, pg, pb, 0, st, segs, coarse, ws.coarse_hist.p, (uint32_t *)nullptr);
+        hipLaunchKernelGGL(k_seg_hist, hg, pb, 0, st, segs, coarse, ws.coarse_hist.p, (uint32_t *)nullptr);
         hipLaunchKernelGGL(k_small_scan, dim3(1), dim3(1024), 0, st, ws.coarse_hist.p, ncoarse * kCopies,
                            ws.ccursor.p, ws.stats.p, ws.seg_count.p + kSeg);
         hipLaunchKernelGGL(k_partition, pg, pb, 0, st, segs, coarse, ws.ccursor.p, ws.sorted.p);
         SegIn comp{ws.sorted.p, ws.seg_count.p + kSeg, 1, cap};
         KeySpec fine{bin_shift, 0, fine_bits, 1, 1};
-        hipLaunchKernelGGL(k_seg_hist, pg, pb, 0, st, comp, fine, ws.bin_count.p, bin_kmers);
+        hipLaunchKernelGGL(k_seg_hist, hg, pb, 0, st, comp, fine, ws.bin_count.p, bin_kmers);
         launch_plan(ws, nbins, per_item, st);
         hipLaunchKernelGGL(k_partition, pg, pb, 0, st, comp, fine, ws.cursor.p, ws.rec.p);
         ws.final_sorted = ws.rec.p;
     } else {
         KeySpec one{bin_shift, 0, 0, 0, 1};
-        hipLaunchKernelGGL(k_seg_hist, pg, pb, 0, st, segs, one, ws.bin_count.p, bin_kmers);
+        hipLaunchKernelGGL(k_seg_hist, hg, pb, 0, st, segs, one, ws.bin_count.p, bin_kmers);
         launch_plan(ws, nbins, per_item, st);
         hipLaunchKernelGGL(k_partition, pg, pb, 0, st, segs, one, ws.cursor.p, ws.sorted.p);
         ws.final_sorted = ws.sorted.p;
