@@ -795,8 +795,8 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_write(const uint32_t *__r
 // one contiguous range per key with a single global atomicAdd. When there are
 // more bins than LDS entries, a coarse pass (bin >> fine_bits) runs first; the
 // fine pass then sees tiles whose records share a few coarse buckets.
-constexpr int kPartTile = 4096;
-constexpr int kPartThreads = 256;
+constexpr int kPartTile = 8192;
+constexpr int kPartThreads = 1024;
 constexpr int kPartItems = kPartTile / kPartThreads;
 constexpr uint32_t kPartRange = 2048;
 
