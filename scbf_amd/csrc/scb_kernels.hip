@@ -1052,7 +1052,8 @@ constexpr int kInsertThreads = 1024, kQueryThreads = 512;
 // of nthr threads cover the largest item; all R keys are loaded up front.
 template <int R>
 __device__ __forceinline__ void order_by_length(const uint64_t *sorted, uint32_t rb, uint32_t n, uint16_t *s_perm,
-                                                uint32_t *s_cls, uint32_t tid, uint32_t nthr) {
+                                                uint32_t *s_cls, uint32_t tid, uint32_t nthr,
+                                                uint32_t *s_smers = nullptr, uint32_t k_minus_s = 0) {
     const uint32_t lane = tid & 63;
     const uint64_t lt = (uint64_t(1) << lane) - 1;
     if (tid < 32) s_cls[tid] = 0;
@@ -1086,12 +1087,14 @@ __device__ __forceinline__ void order_by_length(const uint64_t *sorted, uint32_t
         __syncthreads();
         if (pass == 0) {
             if (tid == 0) {
-                uint32_t run = 0;
+                uint32_t run = 0, smers = 0;
                 for (int c = 0; c < 32; ++c) {
                     const uint32_t v = s_cls[c];
                     s_cls[c] = run;
                     run += v;
+                    smers += v * (32 - c + k_minus_s);  // class c holds records of 32 - c k-mers
                 }
+                if (s_smers) *s_smers = smers;
             }
             __syncthreads();
         }
@@ -1176,17 +1179,29 @@ __global__ __launch_bounds__(256) void k_zero_cold(const uint32_t *bin_count, ui
 // min(MAX, old + count), which equals applying the +1s one by one (saturating
 // increments commute).
 template <int BITS>
-__global__ __launch_bounds__(kInsertThreads) void k_bin_insert(BinArgs a, DevGeo g) {
+// Counts are 16-bit halves of LDS words (a 2^15-slot bin is 64 KiB, so two
+// workgroups share a CU). A work item whose records make at most 65535
+// increments cannot carry out of a half; larger items are counted and merged
+// in sub-batches of records that each stay within that bound.
+__global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_bin_insert(BinArgs a,
+                                                                                                       DevGeo g) {
     extern __shared__ __align__(16) uint32_t s_cnt[];
+    uint16_t *s_c16 = reinterpret_cast<uint16_t *>(s_cnt);
     __shared__ uint64_t s_item;
     __shared__ uint16_t s_perm[kMaxItemRecords];
     __shared__ uint32_t s_cls[32];
+    __shared__ uint32_t s_smers;
     const uint32_t tid = threadIdx.x, nthr = blockDim.x;
     const uint32_t bin_slots = 1u << a.bin_log;
     const uint32_t lshift = a.bin_log - g.bse;
     const uint32_t amask = (1u << g.bse) - 1;
     const uint64_t nwork = a.stats[kStNWork];
     unsigned long long *head = (unsigned long long *)&a.stats[kStWorkHead];
+    // records per sub-batch: a record has at most w k-mers, so w + k - s s-mers
+    const uint32_t rec_incr = (g.w + g.k - g.s) * g.nh;
+    const uint32_t sub_recs = max(1u, 65535u / rec_incr);
+    constexpr uint32_t per = 32 / BITS;
+    const uint32_t nwords = bin_slots / per;
     for (;;) {
         if (tid == 0) s_item = atomicAdd(head, 1ull);
         __syncthreads();
@@ -1198,79 +1213,91 @@ __global__ __launch_bounds__(kInsertThreads) void k_bin_insert(BinArgs a, DevGeo
         uint32_t cnt = a.bin_count[bin];
         uint32_t rb = a.bin_off[bin] + part * a.per_item;
         uint32_t re = min(a.bin_off[bin] + cnt, rb + a.per_item);
-        bool multi = cnt > a.per_item;
-        zero_lds(s_cnt, bin_slots, tid, nthr);
-        order_by_length<kMaxItemRecords / kInsertThreads>(a.sorted, rb, re - rb, s_perm, s_cls, tid, nthr);  // (synchronizes)
-        walk_records(a.sorted, a.pk, rb, re - rb, s_perm, tid, nthr, [&](uint64_t rec, const uint32_t (&W)[5]) {
-            const uint64_t j = rec & 0xFFFFFFFFull;
-            const uint32_t nk = uint32_t((rec >> 32) & 31) + 1;
-            const uint32_t lbase = uint32_t((rec >> 37) & ((1u << lshift) - 1)) << g.bse;
-            uint32_t A0, A1, A2, A3;
-            align_words(W, j, A0, A1, A2, A3);
-            for_each_smer(A0, A1, A2, A3, nk + g.k - 1, g.s, [&](uint32_t, uint64_t smer) {
-                for_each_address(smer, g.nh, amask, [&](uint32_t ad) { atomicAdd(&s_cnt[lbase + ad], 1u); });
-            });
-        });
-        __syncthreads();
-        // merge: 32-bit table words of this bin
-        constexpr uint32_t per = 32 / BITS;
-        const uint32_t nwords = bin_slots / per;
+        const uint32_t n = re - rb;
+        const bool multi = cnt > a.per_item;
+        zero_lds(s_cnt, bin_slots / 2, tid, nthr);
+        order_by_length<kMaxItemRecords / kInsertThreads>(a.sorted, rb, n, s_perm, s_cls, tid, nthr, &s_smers,
+                                                          g.k - g.s);  // (synchronizes)
+        const bool one = uint64_t(s_smers) * g.nh <= 65535u;
+        const uint32_t step = one ? n : sub_recs;
         uint32_t *tw = reinterpret_cast<uint32_t *>(a.table) + bin * uint64_t(nwords);
-        if (!multi && (nwords & 3) == 0) {
-            // 16-byte read-modify-write with kU loads in flight per thread
-            constexpr int kU = 4;
-            uint4 *tv = reinterpret_cast<uint4 *>(tw);
-            const uint4 *cv = reinterpret_cast<const uint4 *>(s_cnt);
-            const uint32_t nvec = nwords >> 2;
-            for (uint32_t v0 = tid; v0 < nvec; v0 += nthr * kU) {
-                uint4 old[kU];
-                if (a.fresh) {  // the table holds no counts yet: write every word
+        for (uint32_t r0 = 0; r0 < n; r0 += step) {
+            const uint32_t m = min(step, n - r0);
+            walk_records(a.sorted, a.pk, rb, m, s_perm + r0, tid, nthr, [&](uint64_t rec, const uint32_t (&W)[5]) {
+                const uint64_t j = rec & 0xFFFFFFFFull;
+                const uint32_t nk = uint32_t((rec >> 32) & 31) + 1;
+                const uint32_t lbase = uint32_t((rec >> 37) & ((1u << lshift) - 1)) << g.bse;
+                uint32_t A0, A1, A2, A3;
+                align_words(W, j, A0, A1, A2, A3);
+                for_each_smer(A0, A1, A2, A3, nk + g.k - 1, g.s, [&](uint32_t, uint64_t smer) {
+                    for_each_address(smer, g.nh, amask, [&](uint32_t ad) {
+                        const uint32_t sl = lbase + ad;
+                        atomicAdd(&s_cnt[sl >> 1], 1u << ((sl & 1) * 16));
+                    });
+                });
+            });
+            __syncthreads();
+            // merge this batch's counts: 32-bit table words of the bin
+            const bool write_all = a.fresh && r0 == 0;  // the table holds no counts yet
+            if (!multi && (nwords & 3) == 0) {
+                // 16-byte read-modify-write with kU loads in flight per thread
+                constexpr int kU = 4;
+                uint4 *tv = reinterpret_cast<uint4 *>(tw);
+                const uint32_t nvec = nwords >> 2;
+                for (uint32_t v0 = tid; v0 < nvec; v0 += nthr * kU) {
+                    uint4 old[kU];
+                    if (write_all) {
 #pragma unroll
-                    for (int u = 0; u < kU; ++u) old[u] = make_uint4(0, 0, 0, 0);
-                } else {
+                        for (int u = 0; u < kU; ++u) old[u] = make_uint4(0, 0, 0, 0);
+                    } else {
 #pragma unroll
-                    for (int u = 0; u < kU; ++u) old[u] = tv[min(v0 + u * nthr, nvec - 1)];  // all in flight
-                }
-#pragma unroll
-                for (int u = 0; u < kU; ++u) {
-                    uint32_t vi = v0 + u * nthr;
-                    if (vi >= nvec) continue;
-                    uint32_t c[4 * per];
-                    uint32_t any = a.fresh;
-#pragma unroll
-                    for (uint32_t q = 0; q < per; ++q) {
-                        uint4 x = cv[vi * per + q];
-                        c[4 * q] = x.x, c[4 * q + 1] = x.y, c[4 * q + 2] = x.z, c[4 * q + 3] = x.w;
-                        any |= x.x | x.y | x.z | x.w;
+                        for (int u = 0; u < kU; ++u) old[u] = tv[min(v0 + u * nthr, nvec - 1)];  // all in flight
                     }
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) {
+                        const uint32_t vi = v0 + u * nthr;
+                        if (vi >= nvec) continue;
+                        uint32_t c[4 * per];
+                        uint32_t any = write_all;
+#pragma unroll
+                        for (uint32_t q = 0; q < 4 * per; q += 2) {  // two 16-bit counts per LDS word
+                            const uint32_t x = s_cnt[(vi * 4 * per + q) >> 1];
+                            c[q] = x & 0xFFFFu, c[q + 1] = x >> 16;
+                            any |= x;
+                        }
+                        if (!any) continue;
+                        uint4 nv;
+                        nv.x = sat_merge_word<BITS>(old[u].x, c);
+                        nv.y = sat_merge_word<BITS>(old[u].y, c + per);
+                        nv.z = sat_merge_word<BITS>(old[u].z, c + 2 * per);
+                        nv.w = sat_merge_word<BITS>(old[u].w, c + 3 * per);
+                        tv[vi] = nv;
+                    }
+                }
+            } else {
+                for (uint32_t wi = tid; wi < nwords; wi += nthr) {
+                    uint32_t c[per];
+                    uint32_t any = !multi && write_all;
+#pragma unroll
+                    for (uint32_t q = 0; q < per; ++q) c[q] = s_c16[wi * per + q], any |= c[q];
                     if (!any) continue;
-                    uint4 nv;
-                    nv.x = sat_merge_word<BITS>(old[u].x, c);
-                    nv.y = sat_merge_word<BITS>(old[u].y, c + per);
-                    nv.z = sat_merge_word<BITS>(old[u].z, c + 2 * per);
-                    nv.w = sat_merge_word<BITS>(old[u].w, c + 3 * per);
-                    tv[vi] = nv;
+                    if (!multi) {
+                        tw[wi] = sat_merge_word<BITS>(write_all ? 0u : tw[wi], c);
+                    } else {
+#pragma unroll
+                        for (uint32_t q = 0; q < per; ++q)
+                            if (c[q])
+                                sat_add_global<BITS>(a.table, (bin * uint64_t(nwords) + wi) * per + q,
+                                                     c[q] > Ctr<BITS>::kMax && BITS < 32 ? Ctr<BITS>::kMax : c[q]);
+                    }
                 }
             }
-        } else {
-            for (uint32_t wi = tid; wi < nwords; wi += nthr) {
-                uint32_t c[per];
-                uint32_t any = !multi && a.fresh;
-#pragma unroll
-                for (uint32_t q = 0; q < per; ++q) c[q] = s_cnt[wi * per + q], any |= c[q];
-                if (!any) continue;
-                if (!multi) {
-                    tw[wi] = sat_merge_word<BITS>(a.fresh ? 0u : tw[wi], c);
-                } else {
-#pragma unroll
-                    for (uint32_t q = 0; q < per; ++q)
-                        if (c[q])
-                            sat_add_global<BITS>(a.table, (bin * uint64_t(nwords) + wi) * per + q,
-                                                 c[q] > Ctr<BITS>::kMax && BITS < 32 ? Ctr<BITS>::kMax : c[q]);
-                }
+            __syncthreads();
+            if (r0 + step < n) {
+                zero_lds(s_cnt, bin_slots / 2, tid, nthr);
+                __syncthreads();
             }
         }
-        __syncthreads();
     }
 }
 
@@ -2084,7 +2111,7 @@ Status Engine::insert(const DeviceBatch &b, Method method, InsertResult &r) {
                                    ws->bin_count.p, nbins, a.per_item, reinterpret_cast<uint32_t *>(d_table_),
                                    words_per_bin);
             }
-            size_t lds = (size_t(1) << bin_log) * 4;
+            size_t lds = (size_t(1) << bin_log) * 2;  // 16-bit counts
             const int thr = kInsertThreads;
 #define SCB_INS(BITS)                                                                           \
     do {                                                                                        \

@@ -229,3 +229,23 @@ def test_weighted_mixture_u32_parity(gpu, oracle, method):
     ib = np.concatenate(reads_b)
     io = np.concatenate(reads_o).astype(np.uint64)
     _parity_roundtrip(scb, oracle, (21, 15, 17, 3, 32, 22, 10), ib, io, ib, io, method)
+
+
+@pytest.mark.parametrize("bits", [8, 16, 32])
+def test_work_items_counted_in_sub_batches(gpu, oracle, bits):
+    """About 2,600 records per 2^15-slot bin: each work item makes more than
+    65,535 counter increments, so it is counted and merged in sub-batches (the
+    16-bit LDS counts cannot overflow), including the first insert into a new
+    table and a second insert on top of it."""
+    scb = gpu
+    k, m, s, h, cse, bse = 31, 21, 25, 4, 20, 9
+    g = synth.genome(400_000, 81)
+    b1, o1 = synth.fixed_reads(g, 4000, 150, 82, sub_rate=0.01)
+    b2, o2 = synth.fixed_reads(g, 4000, 150, 83, sub_rate=0.01)
+    of = oracle.Filter(oracle.make_cfg(k, m, s, h, bits, cse, bse))
+    oracle.insert_records(of, b1, o1)
+    oracle.insert_records(of, b2, o2)
+    b = scb.SuperCountingBloomBuilder(_cfg(scb, k, m, s, h, bits, cse, bse))
+    b.add_sequences((b1, o1))
+    b.add_sequences((b2, o2))
+    assert np.array_equal(b.counters().astype(np.uint32), of.counters())
