@@ -50,7 +50,6 @@ constexpr int kPkWords = kSpan / 16 + 8;
 constexpr int kHashN = kTile + 64;
 constexpr int kWinN = kTile + 32;
 static_assert(kWinN <= kHashN, "window positions reuse a hash-position array");
-constexpr int kChunksMax = kSpan / 16;
 constexpr int kGPad = kSpan + kSpan / 16;  // padded XOR-prefix array length
 constexpr int kGArr = 2 * kGPad > 2 * kHashN ? 2 * kGPad : 2 * kHashN;  // prefixes, then four u16 arrays
 constexpr uint16_t kNoWin = 0xFFFF;
@@ -377,7 +376,6 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
     // bank-conflict free), then the four block prefix/suffix minimum arrays
     __shared__ uint32_t s_G[kGArr];
     __shared__ uint32_t s_wx[2][kTileThreads / 64];
-    __shared__ uint32_t s_of[2][kChunksMax];  // XOR prefix of all chunks before chunk c
     __shared__ uint32_t s_wc[kTileThreads / 64];
     // bit b set: window b + 1 ends the super-k-mer before it (invalid or a new start)
     __shared__ uint32_t s_bnd[kTile / 32 + 2];
@@ -459,16 +457,17 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
             if (tid < 2 * kSpanWords) s_bad16[tid] = 0xFFFF, s_tg16[tid] = 0;
         }
 
-        // (2) ntHash XOR prefixes: chunk-local exclusive prefixes go straight
-        // to LDS; the prefix of the preceding chunks is applied when read
+        // (2) ntHash XOR prefixes: the chunk-local exclusive prefixes stay in
+        // registers until the prefix of the preceding chunks is known, then go to
+        // LDS complete, so each m-mer hash reads two words per strand
+        uint32_t tf[16], tr[16];
         uint32_t xf = 0, xr = 0;
         if (tid < kChunks) {
 #pragma unroll
             for (int t = 0; t < 16; ++t) {
                 uint32_t c = (cw >> (2 * t)) & 3;
                 uint32_t j = uint32_t(16 * tid + t);
-                s_Gf[17 * tid + t] = xf;
-                s_Gr[17 * tid + t] = xr;
+                tf[t] = xf, tr[t] = xr;
                 xf ^= rotl32(nt_seed(c), (32 - (j & 31)) & 31);
                 xr ^= rotl32(nt_seed(c ^ 2), j & 31);
             }
@@ -492,8 +491,12 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
         if (tid < kChunks) {
             uint32_t pf = 0, pr = 0;
             for (int i = 0; i < wid; ++i) pf ^= s_wx[0][i], pr ^= s_wx[1][i];
-            s_of[0][tid] = pf ^ sf ^ xf;
-            s_of[1][tid] = pr ^ sr ^ xr;
+            const uint32_t of = pf ^ sf ^ xf, orr = pr ^ sr ^ xr;  // XOR of the chunks before this one
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                s_Gf[17 * tid + t] = tf[t] ^ of;
+                s_Gr[17 * tid + t] = tr[t] ^ orr;
+            }
         }
         __syncthreads();
 
@@ -501,8 +504,8 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
         for (int hp = tid; hp < NH; hp += kTileThreads) {
             int li = hp + kLeft - 1, lm = li + m;
             int pi = li + (li >> 4), pm = lm + (lm >> 4);
-            uint32_t gf = s_Gf[pm] ^ s_Gf[pi] ^ s_of[0][lm >> 4] ^ s_of[0][li >> 4];
-            uint32_t gr = s_Gr[pm] ^ s_Gr[pi] ^ s_of[1][lm >> 4] ^ s_of[1][li >> 4];
+            uint32_t gf = s_Gf[pm] ^ s_Gf[pi];
+            uint32_t gr = s_Gr[pm] ^ s_Gr[pi];
             uint32_t fw = rotl32(gf, uint32_t(lm - 1) & 31);
             uint32_t rc = rotr32(gr, uint32_t(li) & 31);
             s_K[hp] = uint16_t((fw + rc) >> 16);
@@ -686,14 +689,17 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
         }
     }
 
-    // Block reductions of the statistics, added into one of kSeg spread slots.
-    uint64_t vals[3] = {acc_valid, MODE == kModePlainQuery ? acc_pos : acc_smers,
-                        MODE == kModePlainQuery ? acc_sum : 0};
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        uint64_t v = vals[q];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-        if (lane == 0) s_red[q][wid] = v;
+    // Block reductions of the statistics, added into one of kSeg spread slots
+    // (per-tile counts fit 32 bits; only the plain query's estimate sum needs 64).
+    {
+        uint32_t v0 = uint32_t(acc_valid), v1 = uint32_t(MODE == kModePlainQuery ? acc_pos : acc_smers);
+        uint64_t v2 = acc_sum;
+        for (int off = 32; off > 0; off >>= 1) {
+            v0 += __shfl_down(v0, off, 64);
+            v1 += __shfl_down(v1, off, 64);
+            if (MODE == kModePlainQuery) v2 += __shfl_down(v2, off, 64);
+        }
+        if (lane == 0) s_red[0][wid] = v0, s_red[1][wid] = v1, s_red[2][wid] = MODE == kModePlainQuery ? v2 : 0;
     }
     __syncthreads();
     if (tid == 0) {
@@ -713,7 +719,6 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
 
 // ----------------------------------------------------------------------------
 // Bin planning: exclusive scan of per-bin record counts and the work list.
-// One workgroup of 1024 threads.
 // ----------------------------------------------------------------------------
 // Bin plan in two launches of nbins/1024 workgroups, one bin per thread: block
 // sums of records and work items, then each block adds the sums of the blocks
