@@ -39,7 +39,8 @@ def build(verbose=False):
     objs = []
     tmpdir = os.path.join(HERE, "_build")
     os.makedirs(tmpdir, exist_ok=True)
-    for src, dev in (("scb_kernels.hip", True), ("scb_api.cpp", False), ("scb_serve.cpp", False)):
+    for src, dev in (("scb_kernels.hip", True), ("scb_api.cpp", False), ("scb_serve.cpp", False),
+                     ("scb_fastx.cpp", False)):
         obj = os.path.join(tmpdir, src + ".o")
         cmd = [HIPCC] + COMMON + (DEVICE if dev else HOST) + ["-c", os.path.join(CSRC, src), "-o", obj]
         out = _run(cmd)
@@ -47,7 +48,7 @@ def build(verbose=False):
             print(out)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lz"])
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lz", "-ldl"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -5,12 +5,12 @@
 // persistence and the f64 host estimators. All counting work runs in the HIP
 // engine (scb_kernels.hip); nothing here computes counters on the CPU.
 
-#include <zlib.h>
-
 #include <algorithm>
 #include <cerrno>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -20,6 +20,7 @@
 
 #include "../../include/scb.h"
 #include "scb_engine.h"
+#include "scb_fastx.h"
 #include "scb_handles.h"
 
 using scb::Engine;
@@ -115,86 +116,6 @@ double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// ---- FASTA/FASTQ reader (replaces helicase 0.1.1 for the record path;
-// api.rs:893-927). Plain and gzip input (zlib reads both).
-class FastxReader {
-   public:
-    ~FastxReader() {
-        if (gz_) gzclose(gz_);
-    }
-    int open(const char *path) {
-        path_ = path;
-        gz_ = gzopen(path, "rb");
-        if (!gz_) {
-            int e = errno ? errno : ENOENT;
-            return fail(SCB_ERR_FASTX_OPEN, "failed to open FASTA/FASTQ '" + path_ + "': " +
-                                                std::string(strerror(e)) + " (os error " +
-                                                std::to_string(e) + ")");
-        }
-        gzbuffer(gz_, 1 << 20);
-        return SCB_OK;
-    }
-    // Next record; returns false at end of input.
-    bool next(std::string &header, std::vector<uint8_t> &seq) {
-        header.clear();
-        seq.clear();
-        std::string line;
-        // find a header line
-        for (;;) {
-            if (!have_line_ && !getline(line_)) return false;
-            have_line_ = false;
-            if (!line_.empty() && (line_[0] == '>' || line_[0] == '@')) break;
-        }
-        bool fastq = line_[0] == '@';
-        header.assign(line_.begin() + 1, line_.end());
-        if (fastq) {
-            if (!getline(line_)) return true;
-            seq.assign(line_.begin(), line_.end());
-            if (!getline(line_)) return true;  // '+'
-            if (!getline(line_)) return true;  // quality
-            return true;
-        }
-        while (getline(line_)) {
-            if (!line_.empty() && line_[0] == '>') {
-                have_line_ = true;
-                break;
-            }
-            seq.insert(seq.end(), line_.begin(), line_.end());
-        }
-        return true;
-    }
-    const std::string &path() const { return path_; }
-
-   private:
-    bool getline(std::string &out) {
-        out.clear();
-        for (;;) {
-            if (pos_ >= len_) {
-                int n = gzread(gz_, buf_, sizeof(buf_));
-                if (n <= 0) return !out.empty();
-                len_ = size_t(n);
-                pos_ = 0;
-            }
-            char *start = buf_ + pos_;
-            char *nl = static_cast<char *>(memchr(start, '\n', len_ - pos_));
-            if (nl) {
-                out.append(start, nl);
-                pos_ = size_t(nl - buf_) + 1;
-                if (!out.empty() && out.back() == '\r') out.pop_back();
-                return true;
-            }
-            out.append(start, buf_ + len_);
-            pos_ = len_;
-        }
-    }
-    gzFile gz_ = nullptr;
-    std::string path_;
-    std::string line_;
-    bool have_line_ = false;
-    char buf_[1 << 16];
-    size_t pos_ = 0, len_ = 0;
-};
-
 // api.rs:917-927
 std::string record_name(const std::string &header, uint64_t idx) {
     size_t e = 0;
@@ -223,32 +144,89 @@ constexpr size_t kFileBatchBytes = size_t(256) << 20;
 // Walks a FASTX file in batches of records. split=true yields the DnaChunks of
 // split_non_actg (fastx.rs:14-19: maximal ACGT runs, those shorter than k
 // dropped); split=false yields whole records (keep_non_actg, api.rs:24-27).
+// A parser thread fills the next batch while fn processes the current one
+// (two batches in flight), so parsing overlaps the device work.
 template <class Fn>
 int for_each_batch(const char *path, bool split, size_t k, Fn &&fn) {
-    FastxReader rd;
+    scb_fastx::Reader rd;
     int rc = rd.open(path);
     if (rc) return rc;
-    HostBatch hb;
-    std::string hdr;
-    std::vector<uint8_t> seq;
-    while (rd.next(hdr, seq)) {
-        if (split) {
-            size_t i = 0;
-            while (i < seq.size()) {
-                while (i < seq.size() && !is_acgt(seq[i])) ++i;
-                size_t a = i;
-                while (i < seq.size() && is_acgt(seq[i])) ++i;
-                if (i - a >= k && i > a) hb.add(seq.data() + a, i - a);
+    std::mutex mu;
+    std::condition_variable cv;
+    std::vector<std::unique_ptr<HostBatch>> free_list, full;
+    bool parsed_all = false, stop = false;
+    int parse_rc = SCB_OK;
+    std::string parse_msg;
+    for (int i = 0; i < 2; ++i) free_list.emplace_back(new HostBatch);
+    std::thread parser([&] {
+        std::string hdr;
+        std::vector<uint8_t> seq;
+        std::unique_ptr<HostBatch> hb;
+        auto take_free = [&]() -> bool {
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] { return !free_list.empty() || stop; });
+            if (stop) return false;
+            hb = std::move(free_list.back());
+            free_list.pop_back();
+            hb->clear();
+            return true;
+        };
+        auto publish = [&]() {
+            std::lock_guard<std::mutex> lk(mu);
+            full.push_back(std::move(hb));
+        };
+        if (!take_free()) return;
+        while (rd.next(hdr, seq)) {
+            if (split) {
+                size_t i = 0;
+                while (i < seq.size()) {
+                    while (i < seq.size() && !is_acgt(seq[i])) ++i;
+                    size_t a = i;
+                    while (i < seq.size() && is_acgt(seq[i])) ++i;
+                    if (i - a >= k && i > a) hb->add(seq.data() + a, i - a);
+                }
+            } else {
+                hb->add(seq.data(), seq.size());
             }
-        } else {
-            hb.add(seq.data(), seq.size());
+            if (hb->bases.size() >= kFileBatchBytes) {
+                publish();
+                cv.notify_all();
+                if (!take_free()) return;
+            }
         }
-        if (hb.bases.size() >= kFileBatchBytes) {
-            if ((rc = fn(hb))) return rc;
-            hb.clear();
+        int e = rd.error();  // records the message in this thread; hand it over
+        std::lock_guard<std::mutex> lk(mu);
+        if (e) {
+            parse_rc = e;
+            size_t n = scb_last_error(nullptr, 0);
+            parse_msg.resize(n);
+            scb_last_error(&parse_msg[0], n + 1);
         }
+        if (hb && hb->n()) full.push_back(std::move(hb));
+        parsed_all = true;
+        cv.notify_all();
+    });
+    for (;;) {
+        std::unique_ptr<HostBatch> hb;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] { return !full.empty() || parsed_all; });
+            if (full.empty()) break;
+            hb = std::move(full.front());
+            full.erase(full.begin());
+        }
+        rc = fn(*hb);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            free_list.push_back(std::move(hb));
+            if (rc) stop = true;
+        }
+        cv.notify_all();
+        if (rc) break;
     }
-    if (hb.n()) return fn(hb);
+    parser.join();
+    if (rc) return rc;
+    if (parse_rc) return fail(parse_rc, parse_msg);
     return SCB_OK;
 }
 
@@ -1012,7 +990,7 @@ int scb_write_spectrum(const uint64_t *values, const uint64_t *counts, size_t n,
 
 // api.rs:893-915
 int scb_for_each_fastx_record(const char *path, scb_record_visitor visit, void *user) {
-    FastxReader rd;
+    scb_fastx::Reader rd;
     int rc = rd.open(path);
     if (rc) return rc;
     std::string hdr;
@@ -1023,7 +1001,7 @@ int scb_for_each_fastx_record(const char *path, scb_record_visitor visit, void *
         std::string name = record_name(hdr, idx);
         if ((rc = visit(user, name.c_str(), seq.data(), seq.size()))) return rc;
     }
-    return SCB_OK;
+    return rd.error();
 }
 
 }  // extern "C"

@@ -141,6 +141,62 @@ def test_fastx_multiline_fastq_gzip_and_empty_header(tmp_path):
     assert seqs == [b"ACGT", b"TT"]
 
 
+def _zstd_compress(data: bytes) -> bytes:
+    import ctypes as C
+    z = C.CDLL("libzstd.so.1")
+    z.ZSTD_compressBound.restype = C.c_size_t
+    z.ZSTD_compressBound.argtypes = [C.c_size_t]
+    z.ZSTD_compress.restype = C.c_size_t
+    z.ZSTD_compress.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int]
+    cap = z.ZSTD_compressBound(len(data))
+    out = C.create_string_buffer(cap)
+    n = z.ZSTD_compress(out, cap, data, len(data), 3)
+    assert n < cap
+    return out.raw[:n]
+
+
+def _records(p):
+    names, seqs = [], []
+    scb.for_each_fastx_record(p, lambda n, s: (names.append(n), seqs.append(s)))
+    return names, seqs
+
+
+def test_fastx_zstd_and_multi_member_inputs(tmp_path):
+    """zstd (one and several frames) and multi-member gzip read like the plain file."""
+    import gzip
+    import random
+    rng = random.Random(5)
+    recs = [(f"r{i} d", "".join(rng.choice("ACGTN") for _ in range(rng.randint(0, 300)))) for i in range(3000)]
+    text = "".join(f">{n}\n{s[:70]}\n{s[70:]}\n" for n, s in recs).encode()
+    plain = tmp_path / "r.fa"
+    plain.write_bytes(text)
+    want = _records(plain)
+    assert want[0] == [f"r{i}" for i in range(3000)]
+    half = len(text) // 2
+    cuts = text.index(b">", half)
+    (tmp_path / "r.fa.zst").write_bytes(_zstd_compress(text))
+    (tmp_path / "r2.fa.zst").write_bytes(_zstd_compress(text[:cuts]) + _zstd_compress(text[cuts:]))
+    (tmp_path / "r.fa.gz").write_bytes(gzip.compress(text[:cuts]) + gzip.compress(text[cuts:]))
+    for name in ("r.fa.zst", "r2.fa.zst", "r.fa.gz"):
+        assert _records(tmp_path / name) == want, name
+
+
+def test_fastx_corrupt_compressed_input_is_an_io_error(tmp_path):
+    import gzip
+    data = gzip.compress(b">a\nACGT\n" * 10000)
+    p = tmp_path / "bad.fa.gz"
+    p.write_bytes(data[:len(data) // 2] + bytes(64) + data[len(data) // 2:])
+    with pytest.raises(scb.SuperCountingBloomError) as e:
+        _records(p)
+    assert e.value.kind == "Io" and "gzip error" in str(e.value)
+    z = _zstd_compress(b">a\nACGT\n" * 10000)
+    q = tmp_path / "bad.fa.zst"
+    q.write_bytes(z[:len(z) - 7])
+    with pytest.raises(scb.SuperCountingBloomError) as e:
+        _records(q)
+    assert e.value.kind == "Io" and "zstd error" in str(e.value)
+
+
 def test_fastx_open_error(tmp_path):
     with pytest.raises(scb.SuperCountingBloomError) as e:
         scb.for_each_fastx_record(tmp_path / "missing.fa", lambda n, s: None)
