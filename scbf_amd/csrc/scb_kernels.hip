@@ -367,11 +367,11 @@ enum ScanMode { kModeRecords = 0, kModePlainInsert = 1, kModePlainQuery = 2 };
 // (3) van Herk/Gil-Werman block prefix/suffix minima of (key16 << 16 | pos)
 // for the leftmost and (key16 << 16 | ~pos) for the rightmost argmin; (4) each
 // window takes the min of one suffix and one prefix value.
-template <int MODE, int BITS, int WINC>
+template <int MODE, int BITS, int WINC, int WREG = 0>
 __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
     __shared__ uint32_t s_bad[kSpanWords], s_tg[kSpanWords], s_brk[kSpanWords];
     __shared__ uint32_t s_pk[kPkWords];
-    __shared__ uint16_t s_K[kHashN];  // high 16 bits of each m-mer hash (the argmin key)
+    __shared__ __align__(16) uint16_t s_K[kHashN];  // high 16 bits of each m-mer hash (the argmin key)
     // Gf | Gr (padded: index li + li/16 keeps the 16-per-thread writes
     // bank-conflict free), then the four block prefix/suffix minimum arrays
     __shared__ uint32_t s_G[kGArr];
@@ -512,45 +512,94 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
         }
         __syncthreads();
 
-        // (3) van Herk/Gil-Werman: per block of w hash positions, the positions
-        // of the leftmost and rightmost minimum of every prefix and suffix
-        {
-            const int nblk = (NH + w - 1) / w;
-            for (int b = tid; b < nblk; b += kTileThreads) {
-                const int h0 = b * w, h1 = min(h0 + w, NH);
-                uint32_t cl = ~0u, cr = ~0u;
-                for (int hp = h0; hp < h1; ++hp) {
-                    uint32_t key = uint32_t(s_K[hp]) << 16;
-                    cl = min(cl, key | uint32_t(hp));
-                    cr = min(cr, key | uint32_t(0xFFFF - hp));
-                    s_PL[hp] = uint16_t(cl), s_PR[hp] = uint16_t(0xFFFF - (cr & 0xFFFF));
+        if constexpr (WREG > 0) {
+            // (3+4) with w = WREG known at compile time: each thread takes 8
+            // consecutive windows; their 8 + w - 1 keys come from LDS in 16-byte
+            // loads, and the minimum of window i of the group is
+            // min(suffix minimum of keys [i, 8), prefix minimum of keys [8, i + w)):
+            // w >= 9 makes every window reach key 8. Leftmost and rightmost
+            // minima pack (key, pos) and (key, ~pos) as for the blocks below.
+            static_assert(WREG >= 9 && WREG <= 25, "register windows need 9 <= w <= 25");
+            constexpr int NK = 8 + WREG - 1, NV = (NK + 7) / 8;
+            for (int grp = tid; grp * 8 < kTile + WREG; grp += kTileThreads) {
+                const int h0 = grp * 8;
+                uint32_t kw[NV * 4];  // two 16-bit keys per word
+                const uint4 *src = reinterpret_cast<const uint4 *>(s_K + h0);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    const uint4 x = src[v];
+                    kw[4 * v] = x.x, kw[4 * v + 1] = x.y, kw[4 * v + 2] = x.z, kw[4 * v + 3] = x.w;
                 }
-                cl = cr = ~0u;
-                for (int hp = h1 - 1; hp >= h0; --hp) {
-                    uint32_t key = uint32_t(s_K[hp]) << 16;
-                    cl = min(cl, key | uint32_t(hp));
-                    cr = min(cr, key | uint32_t(0xFFFF - hp));
-                    s_SL[hp] = uint16_t(cl), s_SR[hp] = uint16_t(0xFFFF - (cr & 0xFFFF));
+                auto packed = [&](int i, bool right) {
+                    const uint32_t key = ((kw[i >> 1] >> (16 * (i & 1))) & 0xFFFFu) << 16;
+                    return key | uint32_t(right ? 0xFFFF - (h0 + i) : h0 + i);
+                };
+                uint32_t sl[8], sr[8], ql[WREG - 1], qr[WREG - 1];
+#pragma unroll
+                for (int i = 7; i >= 0; --i) {
+                    sl[i] = i == 7 ? packed(i, false) : min(packed(i, false), sl[i + 1]);
+                    sr[i] = i == 7 ? packed(i, true) : min(packed(i, true), sr[i + 1]);
+                }
+#pragma unroll
+                for (int j = 0; j < WREG - 1; ++j) {
+                    ql[j] = j == 0 ? packed(8, false) : min(packed(8 + j, false), ql[j - 1]);
+                    qr[j] = j == 0 ? packed(8, true) : min(packed(8 + j, true), qr[j - 1]);
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int hj = h0 + i;
+                    if (hj < kTile + WREG) {
+                        const int li = hj + kLeft - 1;
+                        const uint32_t bad = get_bits(s_bad, li, k) | get_bits(s_brk, li + 1, k - 1);
+                        const bool left = 2 * int(__popc(get_bits(s_tg, li, k))) > k;  // leftmost on ties
+                        const uint32_t pl = min(sl[i], ql[i + WREG - 9]), pr = min(sr[i], qr[i + WREG - 9]);
+                        const uint32_t pos = left ? (pl & 0xFFFFu) : 0xFFFFu - (pr & 0xFFFFu);
+                        s_mpos[hj] = bad ? kNoWin : uint16_t(pos);
+                    }
                 }
             }
-        }
-        __syncthreads();
+            __syncthreads();
+        } else {
+            // (3) van Herk/Gil-Werman: per block of w hash positions, the positions
+            // of the leftmost and rightmost minimum of every prefix and suffix
+            {
+                const int nblk = (NH + w - 1) / w;
+                for (int b = tid; b < nblk; b += kTileThreads) {
+                    const int h0 = b * w, h1 = min(h0 + w, NH);
+                    uint32_t cl = ~0u, cr = ~0u;
+                    for (int hp = h0; hp < h1; ++hp) {
+                        uint32_t key = uint32_t(s_K[hp]) << 16;
+                        cl = min(cl, key | uint32_t(hp));
+                        cr = min(cr, key | uint32_t(0xFFFF - hp));
+                        s_PL[hp] = uint16_t(cl), s_PR[hp] = uint16_t(0xFFFF - (cr & 0xFFFF));
+                    }
+                    cl = cr = ~0u;
+                    for (int hp = h1 - 1; hp >= h0; --hp) {
+                        uint32_t key = uint32_t(s_K[hp]) << 16;
+                        cl = min(cl, key | uint32_t(hp));
+                        cr = min(cr, key | uint32_t(0xFFFF - hp));
+                        s_SL[hp] = uint16_t(cl), s_SR[hp] = uint16_t(0xFFFF - (cr & 0xFFFF));
+                    }
+                }
+            }
+            __syncthreads();
 
-        // (4) minimizer position of every window j = t0-1+hj, hj in [0, kTile + w)
-        // Branch-free: both tie rules read the same shape of data, so the rule
-        // only selects the arrays and the direction of the final comparison.
-        for (int hj = tid; hj < kTile + w; hj += kTileThreads) {
-            const int li = hj + kLeft - 1;
-            const uint32_t bad = get_bits(s_bad, li, k) | get_bits(s_brk, li + 1, k - 1);
-            const bool left = 2 * int(__popc(get_bits(s_tg, li, k))) > k;  // leftmost minimum on ties
-            const int e = hj + w - 1;
-            const uint32_t c1 = (left ? s_SL : s_SR)[hj], c2 = (left ? s_PL : s_PR)[e];
-            const uint32_t k1 = s_K[c1], k2 = s_K[c2];
-            // leftmost: the suffix candidate c1 (left of c2) wins ties; rightmost: c2 wins ties
-            const uint32_t pos = left ? (k1 <= k2 ? c1 : c2) : (k2 <= k1 ? c2 : c1);
-            s_mpos[hj] = bad ? kNoWin : uint16_t(pos);
+            // (4) minimizer position of every window j = t0-1+hj, hj in [0, kTile + w)
+            // Branch-free: both tie rules read the same shape of data, so the rule
+            // only selects the arrays and the direction of the final comparison.
+            for (int hj = tid; hj < kTile + w; hj += kTileThreads) {
+                const int li = hj + kLeft - 1;
+                const uint32_t bad = get_bits(s_bad, li, k) | get_bits(s_brk, li + 1, k - 1);
+                const bool left = 2 * int(__popc(get_bits(s_tg, li, k))) > k;  // leftmost minimum on ties
+                const int e = hj + w - 1;
+                const uint32_t c1 = (left ? s_SL : s_SR)[hj], c2 = (left ? s_PL : s_PR)[e];
+                const uint32_t k1 = s_K[c1], k2 = s_K[c2];
+                // leftmost: the suffix candidate c1 (left of c2) wins ties; rightmost: c2 wins ties
+                const uint32_t pos = left ? (k1 <= k2 ? c1 : c2) : (k2 <= k1 ? c2 : c1);
+                s_mpos[hj] = bad ? kNoWin : uint16_t(pos);
+            }
+            __syncthreads();
         }
-        __syncthreads();
 
         // owned windows hj in [1, kTile]: validity bitmap, super-k-mer starts
         constexpr int kPer = kTile / kTileThreads;
@@ -1947,10 +1996,20 @@ Status launch_scan(Launch &L, const DeviceBatch &b, uint64_t seg_cap, uint32_t *
         if (wide) hipLaunchKernelGGL((k_scan<MODE, BITS, 32>), grid, blk, 0, st, a, L.dg);     \
         else hipLaunchKernelGGL((k_scan<MODE, BITS, 8>), grid, blk, 0, st, a, L.dg);           \
     } while (0)
-    switch (L.geo.bits) {
-        case 8: SCB_SCAN(8); break;
-        case 16: SCB_SCAN(16); break;
-        default: SCB_SCAN(32); break;
+    if constexpr (MODE == kModeRecords) {
+        // records do not depend on the counter width or the fimpera window; the
+        // minimizer window w has register-resident variants for common values
+        switch (L.geo.w) {
+            case 11: hipLaunchKernelGGL((k_scan<kModeRecords, 16, 8, 11>), grid, blk, 0, st, a, L.dg); break;
+            case 21: hipLaunchKernelGGL((k_scan<kModeRecords, 16, 8, 21>), grid, blk, 0, st, a, L.dg); break;
+            default: hipLaunchKernelGGL((k_scan<kModeRecords, 16, 8>), grid, blk, 0, st, a, L.dg); break;
+        }
+    } else {
+        switch (L.geo.bits) {
+            case 8: SCB_SCAN(8); break;
+            case 16: SCB_SCAN(16); break;
+            default: SCB_SCAN(32); break;
+        }
     }
 #undef SCB_SCAN
     SCB_HIP(hipGetLastError());
