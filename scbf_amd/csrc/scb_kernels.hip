@@ -1234,13 +1234,15 @@ __global__ __launch_bounds__(256) void k_zero_cold(const uint32_t *bin_count, ui
 // increments commute).
 template <int BITS>
 // Counts are 16-bit halves of LDS words (a 2^15-slot bin is 64 KiB, so two
-// workgroups share a CU). A work item whose records make at most 65535
+// workgroups share a CU): slot s lives in word s mod (slots/2), low half for
+// the first half of the bin and high half for the second. With several blocks
+// per bin a record's half is fixed by its block, so an increment costs what a
+// 32-bit counter would. A work item whose records make at most 65535
 // increments cannot carry out of a half; larger items are counted and merged
 // in sub-batches of records that each stay within that bound.
 __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_bin_insert(BinArgs a,
                                                                                                        DevGeo g) {
     extern __shared__ __align__(16) uint32_t s_cnt[];
-    uint16_t *s_c16 = reinterpret_cast<uint16_t *>(s_cnt);
     __shared__ uint64_t s_item;
     __shared__ uint16_t s_perm[kMaxItemRecords];
     __shared__ uint32_t s_cls[32];
@@ -1256,6 +1258,9 @@ __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(
     const uint32_t sub_recs = max(1u, 65535u / rec_incr);
     constexpr uint32_t per = 32 / BITS;
     const uint32_t nwords = bin_slots / per;
+    const uint32_t half_log = a.bin_log - 1, half_words = bin_slots / 2;
+    // 16-bit count of slot sl
+    auto count_of = [&](uint32_t sl) { return (s_cnt[sl & (half_words - 1)] >> (16 * (sl >> half_log))) & 0xFFFFu; };
     for (;;) {
         if (tid == 0) s_item = atomicAdd(head, 1ull);
         __syncthreads();
@@ -1283,18 +1288,26 @@ __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(
                 const uint32_t lbase = uint32_t((rec >> 37) & ((1u << lshift) - 1)) << g.bse;
                 uint32_t A0, A1, A2, A3;
                 align_words(W, j, A0, A1, A2, A3);
-                for_each_smer(A0, A1, A2, A3, nk + g.k - 1, g.s, [&](uint32_t, uint64_t smer) {
-                    for_each_address(smer, g.nh, amask, [&](uint32_t ad) {
-                        const uint32_t sl = lbase + ad;
-                        atomicAdd(&s_cnt[sl >> 1], 1u << ((sl & 1) * 16));
+                if (lshift > 0) {  // the record's block lies in one half of the bin
+                    const uint32_t wbase = lbase & (half_words - 1), inc = 1u << (16 * (lbase >> half_log));
+                    for_each_smer(A0, A1, A2, A3, nk + g.k - 1, g.s, [&](uint32_t, uint64_t smer) {
+                        for_each_address(smer, g.nh, amask,
+                                         [&](uint32_t ad) { atomicAdd(&s_cnt[wbase + ad], inc); });
                     });
-                });
+                } else {  // one block per bin: its slots span both halves
+                    for_each_smer(A0, A1, A2, A3, nk + g.k - 1, g.s, [&](uint32_t, uint64_t smer) {
+                        for_each_address(smer, g.nh, amask, [&](uint32_t ad) {
+                            atomicAdd(&s_cnt[ad & (half_words - 1)], 1u << (16 * (ad >> half_log)));
+                        });
+                    });
+                }
             });
             __syncthreads();
             // merge this batch's counts: 32-bit table words of the bin
             const bool write_all = a.fresh && r0 == 0;  // the table holds no counts yet
-            if (!multi && (nwords & 3) == 0) {
+            if (!multi && (nwords & 7) == 0) {
                 // 16-byte read-modify-write with kU loads in flight per thread
+                // (at least two vectors per bin, so none straddles the halves)
                 constexpr int kU = 4;
                 uint4 *tv = reinterpret_cast<uint4 *>(tw);
                 const uint32_t nvec = nwords >> 2;
@@ -1311,13 +1324,17 @@ __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(
                     for (int u = 0; u < kU; ++u) {
                         const uint32_t vi = v0 + u * nthr;
                         if (vi >= nvec) continue;
+                        // the vector's 4 * per slots are consecutive words of one half
                         uint32_t c[4 * per];
                         uint32_t any = write_all;
+                        const uint32_t s0 = vi * 4 * per, sh = 16 * (s0 >> half_log);
+                        const uint4 *cw = reinterpret_cast<const uint4 *>(s_cnt + (s0 & (half_words - 1)));
 #pragma unroll
-                        for (uint32_t q = 0; q < 4 * per; q += 2) {  // two 16-bit counts per LDS word
-                            const uint32_t x = s_cnt[(vi * 4 * per + q) >> 1];
-                            c[q] = x & 0xFFFFu, c[q + 1] = x >> 16;
-                            any |= x;
+                        for (uint32_t q = 0; q < per; ++q) {
+                            const uint4 x = cw[q];
+                            c[4 * q] = (x.x >> sh) & 0xFFFFu, c[4 * q + 1] = (x.y >> sh) & 0xFFFFu;
+                            c[4 * q + 2] = (x.z >> sh) & 0xFFFFu, c[4 * q + 3] = (x.w >> sh) & 0xFFFFu;
+                            any |= c[4 * q] | c[4 * q + 1] | c[4 * q + 2] | c[4 * q + 3];
                         }
                         if (!any) continue;
                         uint4 nv;
@@ -1333,7 +1350,7 @@ __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(
                     uint32_t c[per];
                     uint32_t any = !multi && write_all;
 #pragma unroll
-                    for (uint32_t q = 0; q < per; ++q) c[q] = s_c16[wi * per + q], any |= c[q];
+                    for (uint32_t q = 0; q < per; ++q) c[q] = count_of(wi * per + q), any |= c[q];
                     if (!any) continue;
                     if (!multi) {
                         tw[wi] = sat_merge_word<BITS>(write_all ? 0u : tw[wi], c);
@@ -2133,7 +2150,8 @@ Status Engine::insert(const DeviceBatch &b, Method method, InsertResult &r) {
     if (!s.good()) return s;
     if (b.n_bases < geo_.k || b.n_reads == 0) return Status::ok();
     SCB_HIP(hipSetDevice(device_));
-    bool batched = method == kBatched && batched_ok(geo_, tune_.insert_bin_log);
+    // the 16-bit LDS counts of a bin need at least two slots (one word)
+    bool batched = method == kBatched && batched_ok(geo_, tune_.insert_bin_log) && tune_.insert_bin_log >= 1;
     bool fresh = false;
     {
         std::lock_guard<std::mutex> lk(zero_mu_);

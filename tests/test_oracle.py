@@ -229,7 +229,9 @@ def test_fastx_counts():
 
 # ---- filter: brute force and closed forms ------------------------------------------
 GEOS = [(9, 5, 7, 3, 8, 16, 6), (13, 7, 9, 3, 16, 14, 6), (31, 21, 25, 4, 16, 16, 9),
-        (21, 15, 17, 3, 32, 16, 10), (5, 3, 4, 2, 8, 12, 5), (31, 11, 15, 2, 16, 14, 8)]
+        (21, 15, 17, 3, 32, 16, 10), (5, 3, 4, 2, 8, 12, 5), (31, 11, 15, 2, 16, 14, 8),
+        # tables of 16, 16 and 2 counters (config.rs:76 admits any bse < cse)
+        (9, 5, 7, 3, 16, 4, 1), (9, 5, 7, 3, 8, 4, 2), (9, 5, 7, 2, 32, 1, 0)]
 
 
 @pytest.mark.parametrize("geo", GEOS)
