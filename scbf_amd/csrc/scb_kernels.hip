@@ -34,6 +34,7 @@
 #include <thread>
 
 #include "scb_engine.h"
+#include "scb_store_run.h"
 
 namespace scb {
 namespace {
@@ -119,6 +120,44 @@ __device__ __forceinline__ void for_each_address(uint64_t smer, uint32_t nh, uin
         f(uint32_t(h) & amask);
     }
     f(xs64_low15(h) & amask);
+}
+
+// Minimum of the counters base[a] over the addresses of one canonical s-mer
+// (filter.rs:478-489), with all NH loads issued before the first is used.
+template <int NH, class T>
+__device__ __forceinline__ uint32_t min_at_nh(uint64_t h, uint32_t amask, const T *base) {
+    uint32_t v[NH];
+#pragma unroll
+    for (int t = 0; t + 1 < NH; ++t) {
+        h = xs64(h);
+        v[t] = base[uint32_t(h) & amask];
+    }
+    v[NH - 1] = base[xs64_low15(h) & amask];
+    uint32_t m = v[0];
+#pragma unroll
+    for (int t = 1; t < NH; ++t) m = v[t] < m ? v[t] : m;
+    return m;
+}
+
+// The same for any n_hashes (amask < 2^15); chains of up to four hashes are
+// unrolled so that one load latency is paid per s-mer instead of one per hash.
+template <class T>
+__device__ __forceinline__ uint32_t min_at_addresses(uint64_t smer, uint32_t nh, uint32_t amask, const T *base,
+                                                     uint32_t kmax) {
+    switch (nh) {
+        case 1: return min_at_nh<1>(smer, amask, base);
+        case 2: return min_at_nh<2>(smer, amask, base);
+        case 3: return min_at_nh<3>(smer, amask, base);
+        case 4: return min_at_nh<4>(smer, amask, base);
+        default: {
+            uint32_t est = kmax;
+            for_each_address(smer, nh, amask, [&](uint32_t ad) {
+                const uint32_t v = base[ad];
+                est = v < est ? v : est;
+            });
+            return est;
+        }
+    }
 }
 
 // Reverse complement of a len-base 2-bit word (packed-seq read_revcomp_kmer).
@@ -1157,10 +1196,16 @@ __device__ __forceinline__ void order_by_length(const uint64_t *sorted, uint32_t
 
 // The five packed words holding bases [b & ~15, (b & ~15) + 80) and the four
 // words of 16 bases each starting exactly at base b.
+// One 16-byte load (dword-aligned: global memory allows it) and one dword:
+// two memory instructions per record instead of five.
 __device__ __forceinline__ void load_words5(const uint32_t *pk, uint64_t b, uint32_t (&W)[5]) {
+    struct __attribute__((aligned(4))) Q {
+        uint32_t x, y, z, w;
+    } q;
     const uint64_t wi = b >> 4;
-#pragma unroll
-    for (int q = 0; q < 5; ++q) W[q] = pk[wi + q];
+    __builtin_memcpy(&q, pk + wi, sizeof(q));
+    W[0] = q.x, W[1] = q.y, W[2] = q.z, W[3] = q.w;
+    W[4] = pk[wi + 4];
 }
 __device__ __forceinline__ void align_words(const uint32_t (&W)[5], uint64_t b, uint32_t &A0, uint32_t &A1,
                                             uint32_t &A2, uint32_t &A3) {
@@ -1372,6 +1417,11 @@ __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(
     }
 }
 
+// Records per work item of the query kernel (its LDS holds the 64 KiB bin,
+// the record order and a 16-byte estimate slot per thread: two workgroups
+// share a CU).
+constexpr uint32_t kQueryItemRecords = 3584;
+
 // filter.rs:462-508 (+ 562-571) for every super-k-mer of one bin: counters of
 // the bin staged in LDS at native width, min over hashes per s-mer, sliding
 // min over k-s+1 s-mers per k-mer, abundances written at their window index.
@@ -1380,9 +1430,11 @@ __global__ __launch_bounds__(kQueryThreads) void k_bin_query(BinArgs a, DevGeo g
     extern __shared__ __align__(16) uint32_t s_tab[];
     __shared__ uint64_t s_item;
     __shared__ uint64_t s_red[3][kQueryThreads / 64];
-    __shared__ uint16_t s_perm[kMaxItemRecords];
+    __shared__ uint16_t s_perm[kQueryItemRecords];
     __shared__ uint32_t s_cls[32];
+    __shared__ uint4 s_slot[kQueryThreads];  // a thread's first 16 bytes of estimates of its record
     using T = typename Ctr<BITS>::T;
+    constexpr uint32_t kV = 16 / sizeof(T);
     const uint32_t tid = threadIdx.x, nthr = blockDim.x, lane = tid & 63, wid = tid >> 6;
     const uint32_t bin_slots = 1u << a.bin_log;
     const uint32_t lshift = a.bin_log - g.bse;
@@ -1405,7 +1457,7 @@ __global__ __launch_bounds__(kQueryThreads) void k_bin_query(BinArgs a, DevGeo g
         uint32_t re = min(a.bin_off[bin] + cnt, rb + a.per_item);
         const uint32_t *tw = reinterpret_cast<const uint32_t *>(a.table) + bin * uint64_t(nwords);
         stage_to_lds<8>(s_tab, tw, nwords, tid, nthr);
-        order_by_length<kMaxItemRecords / kQueryThreads>(a.sorted, rb, re - rb, s_perm, s_cls, tid, nthr);  // (synchronizes)
+        order_by_length<kQueryItemRecords / kQueryThreads>(a.sorted, rb, re - rb, s_perm, s_cls, tid, nthr);  // (synchronizes)
         walk_records(a.sorted, a.pk, rb, re - rb, s_perm, tid, nthr, [&](uint64_t rec, const uint32_t (&W)[5]) {
             const uint64_t j0 = rec & 0xFFFFFFFFull;
             const uint32_t nk = uint32_t((rec >> 32) & 31) + 1;
@@ -1414,24 +1466,24 @@ __global__ __launch_bounds__(kQueryThreads) void k_bin_query(BinArgs a, DevGeo g
             align_words(W, j0, A0, A1, A2, A3);
             WinMin<WINC> wm;
             T *out = reinterpret_cast<T *>(a.out);
+            T *slot = reinterpret_cast<T *>(&s_slot[tid]);
             // per-record sums: at most 32 k-mers, so 8/16-bit estimates sum in 32 bits
             uint32_t rec_pos = 0, rec_sum = 0;
             uint64_t rec_sum64 = 0;
             for_each_smer(A0, A1, A2, A3, nk + g.k - 1, g.s, [&](uint32_t j, uint64_t smer) {
-                uint32_t est = Ctr<BITS>::kMax;
-                for_each_address(smer, g.nh, amask, [&](uint32_t ad) {
-                    uint32_t v = lt[lbase + ad];
-                    est = v < est ? v : est;
-                });
+                const uint32_t est = min_at_addresses(smer, g.nh, amask, lt + lbase, Ctr<BITS>::kMax);
                 const uint32_t ab = EXACT ? wm.push_exact(est) : wm.push(est, g.win);
                 if (j + 1 >= g.win) {
-                    uint64_t q = j + 1 - g.win;
-                    if (out) out[j0 + q] = T(ab);
+                    const uint32_t q = j + 1 - g.win;
+                    if (q < kV) slot[q] = T(ab);
+                    else if (out) out[j0 + q] = T(ab);
                     rec_pos += ab > 0;
                     if (BITS < 32) rec_sum += ab;
                     else rec_sum64 += ab;
                 }
             });
+            // the first estimates of the record in two or three stores
+            if (out) store_run(reinterpret_cast<uint8_t *>(out + j0), s_slot[tid], min(nk, kV) * uint32_t(sizeof(T)));
             queried += nk;
             positive += rec_pos;
             checksum += BITS < 32 ? uint64_t(rec_sum) : rec_sum64;
@@ -1593,12 +1645,16 @@ __global__ __launch_bounds__(512) void k_bin_spectrum(SpecArgs sa, DevGeo g) {
             load_aligned(a.pk, j0, A0, A1, A2, A3);
             WinMin<WINC> wm;
             for_each_smer(A0, A1, A2, A3, nk + g.k - 1, g.s, [&](uint32_t j, uint64_t smer) {
-                uint64_t h = xs64(smer);
                 uint32_t est = Ctr<BITS>::kMax;
-                for (uint32_t t = 0; t < g.nh; ++t) {
-                    uint32_t v = tbase[lbase + (uint32_t(h) & amask)];
-                    est = v < est ? v : est;
-                    if (t + 1 < g.nh) h = xs64(h);
+                if (g.bse <= 15) {
+                    est = min_at_addresses(smer, g.nh, amask, tbase + lbase, Ctr<BITS>::kMax);
+                } else {
+                    uint64_t h = xs64(smer);
+                    for (uint32_t t = 0; t < g.nh; ++t) {
+                        uint32_t v = tbase[lbase + (uint32_t(h) & amask)];
+                        est = v < est ? v : est;
+                        if (t + 1 < g.nh) h = xs64(h);
+                    }
                 }
                 uint32_t ab = wm.push(est, g.win);
                 if (j + 1 >= g.win) {
@@ -1886,6 +1942,7 @@ void table_release(int device, uint64_t bytes, void *p) {
 }  // namespace
 
 static_assert(kMaxItemRecords <= 65536, "record order indices are u16");
+static_assert(kQueryItemRecords % kQueryThreads == 0, "query items are walked in whole rounds");
 
 Status Engine::create(const Geometry &g, std::unique_ptr<Engine> &out) {
     if (device_count() <= 0) return Status{8, "device error: no HIP device is visible"};
@@ -2255,8 +2312,10 @@ Status Engine::query(const DeviceBatch &b, Method method, void *d_values, uint32
         mark(*ws, 1);
     } else {
         uint64_t nbins = 0;
-        uint32_t bin_log = tune_.query_bin_log;
-        s = batched_front(L, b, bin_log, std::max<uint32_t>(1, std::min<uint32_t>(tune_.max_records_per_item, kMaxItemRecords)), d_valid_bits, nbins);
+        const uint32_t bin_log = tune_.query_bin_log;
+        const uint32_t per_item =
+            std::max<uint32_t>(1, std::min<uint32_t>(tune_.max_records_per_item, kQueryItemRecords));
+        s = batched_front(L, b, bin_log, per_item, d_valid_bits, nbins);
         if (s.good()) {
             BinArgs a{};
             a.sorted = ws->final_sorted;
@@ -2268,7 +2327,7 @@ Status Engine::query(const DeviceBatch &b, Method method, void *d_values, uint32
             a.table = d_table_;
             a.out = d_values;
             a.bin_log = bin_log;
-            a.per_item = std::max<uint32_t>(1, std::min<uint32_t>(tune_.max_records_per_item, kMaxItemRecords));
+            a.per_item = per_item;
             size_t lds = (size_t(1) << bin_log) * geo_.bytes;
             const int thr = kQueryThreads;
 #define SCB_QRY(BITS, WC, EX)                                                                          \
