@@ -1196,16 +1196,18 @@ __device__ __forceinline__ void order_by_length(const uint64_t *sorted, uint32_t
 
 // The five packed words holding bases [b & ~15, (b & ~15) + 80) and the four
 // words of 16 bases each starting exactly at base b.
-// One 16-byte load (dword-aligned: global memory allows it) and one dword:
-// two memory instructions per record instead of five.
-__device__ __forceinline__ void load_words5(const uint32_t *pk, uint64_t b, uint32_t (&W)[5]) {
+// One 16-byte load (dword-aligned: global memory allows it), and the fifth
+// word only for a record whose bases reach past the first four: one or two
+// memory requests per record instead of five.
+__device__ __forceinline__ void load_words5(const uint32_t *pk, uint64_t rec, uint32_t k, uint32_t (&W)[5]) {
     struct __attribute__((aligned(4))) Q {
         uint32_t x, y, z, w;
     } q;
-    const uint64_t wi = b >> 4;
+    const uint64_t b = rec & 0xFFFFFFFFull, wi = b >> 4;
+    const uint32_t end = uint32_t(b & 15) + uint32_t((rec >> 32) & 31) + k;  // past the record's last base
     __builtin_memcpy(&q, pk + wi, sizeof(q));
     W[0] = q.x, W[1] = q.y, W[2] = q.z, W[3] = q.w;
-    W[4] = pk[wi + 4];
+    W[4] = end > 64 ? pk[wi + 4] : 0u;
 }
 __device__ __forceinline__ void align_words(const uint32_t (&W)[5], uint64_t b, uint32_t &A0, uint32_t &A1,
                                             uint32_t &A2, uint32_t &A3) {
@@ -1222,17 +1224,17 @@ __device__ __forceinline__ void align_words(const uint32_t (&W)[5], uint64_t b, 
 // and its bases are two dependent global loads, which would otherwise stall
 // every round. Indices past the end are clamped, so every load is in bounds.
 template <class F>
-__device__ __forceinline__ void walk_records(const uint64_t *sorted, const uint32_t *pk, uint32_t rb, uint32_t n,
+__device__ __forceinline__ void walk_records(const uint64_t *sorted, const uint32_t *pk, uint32_t k, uint32_t rb, uint32_t n,
                                              const uint16_t *s_perm, uint32_t tid, uint32_t nthr, F &&f) {
     if (tid >= n) return;
     const uint32_t last = n - 1;
     uint64_t rc = sorted[rb + s_perm[tid]];
     uint64_t rn = sorted[rb + s_perm[min(tid + nthr, last)]];
     uint32_t W[5];
-    load_words5(pk, rc & 0xFFFFFFFFull, W);
+    load_words5(pk, rc, k, W);
     for (uint32_t i = tid; i < n; i += nthr) {
         uint32_t V[5];
-        load_words5(pk, rn & 0xFFFFFFFFull, V);
+        load_words5(pk, rn, k, V);
         const uint64_t rnn = sorted[rb + s_perm[min(i + 2 * nthr, last)]];
         f(rc, W);
         rc = rn;
@@ -1327,7 +1329,7 @@ __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(
         uint32_t *tw = reinterpret_cast<uint32_t *>(a.table) + bin * uint64_t(nwords);
         for (uint32_t r0 = 0; r0 < n; r0 += step) {
             const uint32_t m = min(step, n - r0);
-            walk_records(a.sorted, a.pk, rb, m, s_perm + r0, tid, nthr, [&](uint64_t rec, const uint32_t (&W)[5]) {
+            walk_records(a.sorted, a.pk, g.k, rb, m, s_perm + r0, tid, nthr, [&](uint64_t rec, const uint32_t (&W)[5]) {
                 const uint64_t j = rec & 0xFFFFFFFFull;
                 const uint32_t nk = uint32_t((rec >> 32) & 31) + 1;
                 const uint32_t lbase = uint32_t((rec >> 37) & ((1u << lshift) - 1)) << g.bse;
@@ -1432,7 +1434,7 @@ __global__ __launch_bounds__(kQueryThreads) void k_bin_query(BinArgs a, DevGeo g
     __shared__ uint64_t s_red[3][kQueryThreads / 64];
     __shared__ uint16_t s_perm[kQueryItemRecords];
     __shared__ uint32_t s_cls[32];
-    __shared__ uint4 s_slot[kQueryThreads];  // a thread's first 16 bytes of estimates of its record
+    __shared__ uint4 s_slot[kQueryThreads];  // a thread's next 16 bytes of estimates of its record
     using T = typename Ctr<BITS>::T;
     constexpr uint32_t kV = 16 / sizeof(T);
     const uint32_t tid = threadIdx.x, nthr = blockDim.x, lane = tid & 63, wid = tid >> 6;
@@ -1458,7 +1460,7 @@ __global__ __launch_bounds__(kQueryThreads) void k_bin_query(BinArgs a, DevGeo g
         const uint32_t *tw = reinterpret_cast<const uint32_t *>(a.table) + bin * uint64_t(nwords);
         stage_to_lds<8>(s_tab, tw, nwords, tid, nthr);
         order_by_length<kQueryItemRecords / kQueryThreads>(a.sorted, rb, re - rb, s_perm, s_cls, tid, nthr);  // (synchronizes)
-        walk_records(a.sorted, a.pk, rb, re - rb, s_perm, tid, nthr, [&](uint64_t rec, const uint32_t (&W)[5]) {
+        walk_records(a.sorted, a.pk, g.k, rb, re - rb, s_perm, tid, nthr, [&](uint64_t rec, const uint32_t (&W)[5]) {
             const uint64_t j0 = rec & 0xFFFFFFFFull;
             const uint32_t nk = uint32_t((rec >> 32) & 31) + 1;
             const uint32_t lbase = uint32_t((rec >> 37) & ((1u << lshift) - 1)) << g.bse;
@@ -1475,15 +1477,18 @@ __global__ __launch_bounds__(kQueryThreads) void k_bin_query(BinArgs a, DevGeo g
                 const uint32_t ab = EXACT ? wm.push_exact(est) : wm.push(est, g.win);
                 if (j + 1 >= g.win) {
                     const uint32_t q = j + 1 - g.win;
-                    if (q < kV) slot[q] = T(ab);
-                    else if (out) out[j0 + q] = T(ab);
+                    slot[q & (kV - 1)] = T(ab);
+                    if ((q & (kV - 1)) == kV - 1 && out)  // a full slot: kV estimates in one store
+                        store_run(reinterpret_cast<uint8_t *>(out + j0 + q - (kV - 1)), s_slot[tid], 16);
                     rec_pos += ab > 0;
                     if (BITS < 32) rec_sum += ab;
                     else rec_sum64 += ab;
                 }
             });
-            // the first estimates of the record in two or three stores
-            if (out) store_run(reinterpret_cast<uint8_t *>(out + j0), s_slot[tid], min(nk, kV) * uint32_t(sizeof(T)));
+            // the estimates left in the slot, in two or three stores
+            const uint32_t rest = nk & (kV - 1);
+            if (rest && out)
+                store_run(reinterpret_cast<uint8_t *>(out + j0 + (nk - rest)), s_slot[tid], rest * uint32_t(sizeof(T)));
             queried += nk;
             positive += rec_pos;
             checksum += BITS < 32 ? uint64_t(rec_sum) : rec_sum64;
