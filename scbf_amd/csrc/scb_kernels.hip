@@ -122,6 +122,28 @@ __device__ __forceinline__ void for_each_address(uint64_t smer, uint32_t nh, uin
     f(xs64_low15(h) & amask);
 }
 
+// for_each_address with the chain unrolled for n_hashes up to four (a
+// uniform branch on nh; longer chains take the loop).
+template <int NH, class F>
+__device__ __forceinline__ void for_each_address_n(uint64_t h, uint32_t amask, F &&f) {
+#pragma unroll
+    for (int t = 0; t + 1 < NH; ++t) {
+        h = xs64(h);
+        f(uint32_t(h) & amask);
+    }
+    f(xs64_low15(h) & amask);
+}
+template <class F>
+__device__ __forceinline__ void for_each_address_any(uint64_t smer, uint32_t nh, uint32_t amask, F &&f) {
+    switch (nh) {
+        case 1: for_each_address_n<1>(smer, amask, f); break;
+        case 2: for_each_address_n<2>(smer, amask, f); break;
+        case 3: for_each_address_n<3>(smer, amask, f); break;
+        case 4: for_each_address_n<4>(smer, amask, f); break;
+        default: for_each_address(smer, nh, amask, f); break;
+    }
+}
+
 // Minimum of the counters base[a] over the addresses of one canonical s-mer
 // (filter.rs:478-489), with all NH loads issued before the first is used.
 template <int NH, class T>
@@ -573,6 +595,20 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
                     const uint32_t key = ((kw[i >> 1] >> (16 * (i & 1))) & 0xFFFFu) << 16;
                     return key | uint32_t(right ? 0xFFFF - (h0 + i) : h0 + i);
                 };
+                // the group's windows read bits [base, base + 7 + k) of the masks:
+                // three words per mask, then one funnel shift per window
+                const int base = h0 + kLeft - 1;
+                const uint32_t bsh = uint32_t(base & 31), bwi = uint32_t(base >> 5);
+                auto span64 = [&](const uint32_t *bm, uint32_t &lo, uint32_t &hi) {
+                    const uint32_t w0 = bm[bwi], w1 = bm[bwi + 1], w2 = bm[bwi + 2];
+                    lo = __builtin_amdgcn_alignbit(w1, w0, bsh);
+                    hi = __builtin_amdgcn_alignbit(w2, w1, bsh);
+                };
+                uint32_t bad_lo, bad_hi, brk_lo, brk_hi, tg_lo, tg_hi;
+                span64(s_bad, bad_lo, bad_hi);
+                span64(s_brk, brk_lo, brk_hi);
+                span64(s_tg, tg_lo, tg_hi);
+                const uint32_t mk = k >= 32 ? ~0u : (1u << k) - 1, mk1 = (1u << (k - 1)) - 1;
                 uint32_t sl[8], sr[8], ql[WREG - 1], qr[WREG - 1];
 #pragma unroll
                 for (int i = 7; i >= 0; --i) {
@@ -588,9 +624,11 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
                 for (int i = 0; i < 8; ++i) {
                     const int hj = h0 + i;
                     if (hj < kTile + WREG) {
-                        const int li = hj + kLeft - 1;
-                        const uint32_t bad = get_bits(s_bad, li, k) | get_bits(s_brk, li + 1, k - 1);
-                        const bool left = 2 * int(__popc(get_bits(s_tg, li, k))) > k;  // leftmost on ties
+                        // bits [li, li + k) with li = base + i (and the breaks after li)
+                        const uint32_t bad = (__builtin_amdgcn_alignbit(bad_hi, bad_lo, i) & mk) |
+                                             (__builtin_amdgcn_alignbit(brk_hi, brk_lo, i + 1) & mk1);
+                        const bool left =  // leftmost on ties
+                            2 * int(__popc(__builtin_amdgcn_alignbit(tg_hi, tg_lo, i) & mk)) > k;
                         const uint32_t pl = min(sl[i], ql[i + WREG - 9]), pr = min(sr[i], qr[i + WREG - 9]);
                         const uint32_t pos = left ? (pl & 0xFFFFu) : 0xFFFFu - (pr & 0xFFFFu);
                         s_mpos[hj] = bad ? kNoWin : uint16_t(pos);
@@ -1338,12 +1376,12 @@ __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(
                 if (lshift > 0) {  // the record's block lies in one half of the bin
                     const uint32_t wbase = lbase & (half_words - 1), inc = 1u << (16 * (lbase >> half_log));
                     for_each_smer(A0, A1, A2, A3, nk + g.k - 1, g.s, [&](uint32_t, uint64_t smer) {
-                        for_each_address(smer, g.nh, amask,
-                                         [&](uint32_t ad) { atomicAdd(&s_cnt[wbase + ad], inc); });
+                        for_each_address_any(smer, g.nh, amask,
+                                             [&](uint32_t ad) { atomicAdd(&s_cnt[wbase + ad], inc); });
                     });
                 } else {  // one block per bin: its slots span both halves
                     for_each_smer(A0, A1, A2, A3, nk + g.k - 1, g.s, [&](uint32_t, uint64_t smer) {
-                        for_each_address(smer, g.nh, amask, [&](uint32_t ad) {
+                        for_each_address_any(smer, g.nh, amask, [&](uint32_t ad) {
                             atomicAdd(&s_cnt[ad & (half_words - 1)], 1u << (16 * (ad >> half_log)));
                         });
                     });
