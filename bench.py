@@ -45,6 +45,64 @@ def gen_workload(seed=1):
     return ins_b, ins_o, q_b, q_o
 
 
+def write_fastq(path, bases, offsets):
+    """Fixed-length reads as FASTQ records ('@r', the bases, '+', quality 'I' per base)."""
+    n = offsets.size - 1
+    L = int(offsets[1] - offsets[0])
+    if not np.all(np.diff(offsets.astype(np.int64)) == L):
+        raise ValueError("write_fastq expects reads of one length")
+    rec = np.empty((n, 2 * L + 7), dtype=np.uint8)
+    rec[:, 0:3] = np.frombuffer(b"@r\n", np.uint8)
+    rec[:, 3:3 + L] = bases.reshape(n, L)
+    rec[:, 3 + L:6 + L] = np.frombuffer(b"\n+\n", np.uint8)
+    rec[:, 6 + L:6 + 2 * L] = ord("I")
+    rec[:, 6 + 2 * L] = ord("\n")
+    rec.tofile(path)
+
+
+def io_bench(scb, cfg, ins_b, ins_o, q_b, q_o, inserted, queried):
+    """The file boundaries of the path on the same reads: build from a FASTQ file (parse, copy,
+    insert), query a FASTQ file (parse, copy, query, reports), and the index written to and read
+    back from a .scb file. Files live in a temporary directory that is removed afterwards."""
+    import shutil
+    import tempfile
+    d = tempfile.mkdtemp(prefix="scb_io_")
+    try:
+        fi, fq, fs = (os.path.join(d, x) for x in ("insert.fq", "query.fq", "index.scb"))
+        write_fastq(fi, ins_b, ins_o)
+        write_fastq(fq, q_b, q_o)
+        t0 = time.perf_counter()
+        rep = scb.SuperCountingBloomIndex.build_from_fastx(fi, cfg)
+        t1 = time.perf_counter()
+        qr = rep.index.query_fasta(fq)
+        t2 = time.perf_counter()
+        rep.index.save(fs)
+        t3 = time.perf_counter()
+        loaded = scb.SuperCountingBloomIndex.load(fs)
+        t4 = time.perf_counter()
+        scb_bytes = os.path.getsize(fs)
+        del loaded
+        return {
+            "fastq_build": {"G_kmers_per_s": round(rep.indexed_kmers / (t1 - t0) / 1e9, 3),
+                            "file_MB_per_s": round(os.path.getsize(fi) / (t1 - t0) / 1e6, 1),
+                            "seconds": round(t1 - t0, 3),
+                            "kmers_equal_device_path": rep.indexed_kmers == inserted},
+            "fastq_query": {"G_kmers_per_s": round(qr.valid_kmers / (t2 - t1) / 1e9, 3),
+                            "file_MB_per_s": round(os.path.getsize(fq) / (t2 - t1) / 1e6, 1),
+                            "seconds": round(t2 - t1, 3),
+                            "kmers_equal_device_path": qr.valid_kmers == queried},
+            "scb_save_GB_per_s": round(scb_bytes / (t3 - t2) / 1e9, 2),
+            "scb_load_GB_per_s": round(scb_bytes / (t4 - t3) / 1e9, 2),
+            "scb_bytes": scb_bytes,
+            "note": "plain FASTQ of the benchmark reads and a .scb file in the temporary directory; "
+                    "file reads may be served from the host page cache",
+        }
+    except Exception as e:  # a full or missing temporary directory is reported, not fatal
+        return {"error": f"{type(e).__name__}: {e}"}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def cfg_c2(scb):
     return scb.SuperCountingBloomConfig(k=31, m=21, s=25, n_hashes=4, counter_bits=16,
                                         counter_slots_exponent=30, block_slots_exponent=9,
@@ -257,6 +315,7 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-host", action="store_true")
+    ap.add_argument("--no-io", action="store_true", help="skip the FASTQ and .scb file measurements")
     ap.add_argument("--serve-qps", default="100,400")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--copy-probe", action="store_true",
@@ -399,6 +458,11 @@ def main():
         log(f"host-buffer path: {host_api}")
         del hidx
 
+    io = None
+    if not args.no_io:
+        io = io_bench(scb, cfg, ins_b, ins_o, q_b, q_o, ir.inserted_kmers, qr.valid_kmers)
+        log(f"file boundaries: {io}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(ins_b, ins_o, q_b, q_o)
@@ -422,6 +486,7 @@ def main():
         "roofline": roof,
         "served": served,
         "host_buffers": host_api,
+        "files": io,
         "served_p50_ms": served["steps"][0]["p50_ms"] if served else None,
         "served_p99_ms": served["steps"][0]["p99_ms"] if served else None,
         "cpu_baseline": cpu,

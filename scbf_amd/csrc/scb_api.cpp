@@ -147,7 +147,7 @@ constexpr size_t kFileBatchBytes = size_t(256) << 20;
 // A parser thread fills the next batch while fn processes the current one
 // (two batches in flight), so parsing overlaps the device work.
 template <class Fn>
-int for_each_batch(const char *path, bool split, size_t k, Fn &&fn) {
+int for_each_batch(const char *path, Fn &&fn) {
     scb_fastx::Reader rd;
     int rc = rd.open(path);
     if (rc) return rc;
@@ -177,17 +177,9 @@ int for_each_batch(const char *path, bool split, size_t k, Fn &&fn) {
         };
         if (!take_free()) return;
         while (rd.next(hdr, seq)) {
-            if (split) {
-                size_t i = 0;
-                while (i < seq.size()) {
-                    while (i < seq.size() && !is_acgt(seq[i])) ++i;
-                    size_t a = i;
-                    while (i < seq.size() && is_acgt(seq[i])) ++i;
-                    if (i - a >= k && i > a) hb->add(seq.data() + a, i - a);
-                }
-            } else {
-                hb->add(seq.data(), seq.size());
-            }
+            // whole records: the device scan drops every window with a non-ACGT
+            // base, so a record gives the k-mers of its ACGT runs (fastx.rs:14-19)
+            hb->add(seq.data(), seq.size());
             if (hb->bases.size() >= kFileBatchBytes) {
                 publish();
                 cv.notify_all();
@@ -247,6 +239,28 @@ uint64_t count_indexed(const uint8_t *bases, const uint64_t *offsets, size_t n, 
         uint64_t c = 0;
         for (size_t i = i0; i < i1; ++i)
             c += valid_runs(bases + offsets[i], size_t(offsets[i + 1] - offsets[i]), k).second > 0;
+        part[t] = c;
+    };
+    std::vector<std::thread> ts;
+    for (unsigned t = 1; t < nt; ++t) ts.emplace_back(run, t);
+    run(0);
+    for (auto &t : ts) t.join();
+    uint64_t c = 0;
+    for (uint64_t v : part) c += v;
+    return c;
+}
+
+// ACGT runs of at least k bases over all records (fastx.rs:14-19 chunks),
+// counted by host threads over record ranges.
+uint64_t count_chunks(const uint8_t *bases, const uint64_t *offsets, size_t n, size_t k) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned nt = offsets[n] - offsets[0] < (uint64_t(1) << 22) ? 1u : std::min(16u, hw);
+    std::vector<uint64_t> part(nt, 0);
+    auto run = [&](unsigned t) {
+        const size_t i0 = n * t / nt, i1 = n * (t + 1) / nt;
+        uint64_t c = 0;
+        for (size_t i = i0; i < i1; ++i)
+            c += valid_runs(bases + offsets[i], size_t(offsets[i + 1] - offsets[i]), k).first;
         part[t] = c;
     };
     std::vector<std::thread> ts;
@@ -494,7 +508,7 @@ int scb_builder_add_fasta(scb_builder *b, const char *path, scb_add_report *repo
     if (rc) return rc;
     scb_add_report rep{};
     double t0 = now_s();
-    rc = for_each_batch(path, false, size_t(b->cfg.k), [&](HostBatch &hb) -> int {
+    rc = for_each_batch(path, [&](HostBatch &hb) -> int {
         return builder_insert_host(b, hb.bases.data(), hb.offsets.data(), hb.n(), &rep);
     });
     if (rc) return rc;
@@ -553,11 +567,15 @@ int scb_index_build_from_fastx(const char *path, const scb_config *cfg, scb_inde
     idx->cfg = *cfg;
     if ((rc = make_engine(*cfg, idx->engine))) return rc;
     scb_build_report rep{};
-    rc = for_each_batch(path, true, size_t(cfg->k), [&](HostBatch &hb) -> int {
+    rc = for_each_batch(path, [&](HostBatch &hb) -> int {
         scb::InsertResult r;
+        // the chunk count overlaps the device insert
+        uint64_t chunks = 0;
+        std::thread counter([&] { chunks = count_chunks(hb.bases.data(), hb.offsets.data(), hb.n(), size_t(cfg->k)); });
         Status s = idx->engine->insert_host(hb.bases.data(), hb.offsets.data(), hb.n(), idx->method, r);
+        counter.join();
         if (!s.good()) return fail(s);
-        rep.chunks += hb.n();
+        rep.chunks += chunks;
         rep.indexed_kmers += r.inserted_kmers;
         rep.inserted_smers += r.inserted_smers;
         return SCB_OK;
@@ -780,7 +798,7 @@ int scb_index_summarize_sequence(const scb_index *idx, const uint8_t *seq, size_
 int scb_index_query_fasta(const scb_index *idx, const char *path, scb_query_report *report) {
     scb_query_report rep{};
     size_t k = size_t(idx->cfg.k);
-    int rc = for_each_batch(path, false, k, [&](HostBatch &hb) -> int {
+    int rc = for_each_batch(path, [&](HostBatch &hb) -> int {
         scb::QueryResult r;
         Status s = idx->engine->query_host(hb.bases.data(), hb.offsets.data(), hb.n(), idx->method,
                                            nullptr, nullptr, r);
@@ -908,7 +926,7 @@ int scb_estimate_spectrum_from_fastx_pair(const char *index_path, const char *qu
     // one spectrum call per batch of the file; keys shared between batches are
     // only counted once because the spectrum engine keeps a per-call key set.
     HostBatch all;
-    rc = for_each_batch(query_path, true, size_t(cfg->k), [&](HostBatch &hb) -> int {
+    rc = for_each_batch(query_path, [&](HostBatch &hb) -> int {
         all.bases.insert(all.bases.end(), hb.bases.begin(), hb.bases.end());
         uint64_t base = all.offsets.back();
         for (size_t i = 1; i < hb.offsets.size(); ++i) all.offsets.push_back(base + hb.offsets[i]);
@@ -952,7 +970,7 @@ int scb_stream_estimates_from_fastx_pair(const char *index_path, const char *que
     std::unique_ptr<scb_index, void (*)(scb_index *)> guard(idx, scb_index_free);
     double tq = now_s();
     scb_stream_stats st{};
-    rc = for_each_batch(query_path, true, size_t(cfg->k), [&](HostBatch &hb) -> int {
+    rc = for_each_batch(query_path, [&](HostBatch &hb) -> int {
         scb::QueryResult r;
         Status s = idx->engine->query_host(hb.bases.data(), hb.offsets.data(), hb.n(), idx->method,
                                            nullptr, nullptr, r);
