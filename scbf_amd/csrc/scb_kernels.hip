@@ -33,6 +33,7 @@
 #include <cstring>
 #include <thread>
 
+#include "scb_bases.h"
 #include "scb_engine.h"
 #include "scb_store_run.h"
 
@@ -500,13 +501,12 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
         if (tid < kChunks) {
             uint32_t badm = 0, tgm = 0;
 #pragma unroll
-            for (int t = 0; t < 16; ++t) {
-                uint32_t ch = (wv[t >> 2] >> (8 * (t & 3))) & 0xFF;
-                uint32_t code = (ch >> 1) & 3;
-                bool ok = is_acgt(ch);
-                cw |= code << (2 * t);
-                badm |= uint32_t(!ok) << t;
-                tgm |= uint32_t(ok && (code >> 1)) << t;
+            for (int q = 0; q < 4; ++q) {  // four bases per word at once
+                uint32_t c4, b4, t4;
+                pack4_bases(wv[q], c4, b4, t4);
+                cw |= c4 << (8 * q);
+                badm |= b4 << (4 * q);
+                tgm |= t4 << (4 * q);
             }
             s_pk[tid] = cw;
             s_bad16[tid] = uint16_t(badm);
