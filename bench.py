@@ -103,6 +103,34 @@ def io_bench(scb, cfg, ins_b, ins_o, q_b, q_o, inserted, queried):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def spectrum_bench(scb, idx, dev, reps=3):
+    """Distinct-k-mer spectrum of the query reads (resident in HBM) against the built index:
+    scb_index_spectrum_device, one warm-up call, then the mean of `reps` calls."""
+    lib = scb.api.lib()
+    cap = 1 << 17
+    vals = np.zeros(cap, dtype=np.uint64)
+    cnts = np.zeros(cap, dtype=np.uint64)
+    n, q, d = C.c_size_t(), C.c_uint64(), C.c_uint64()
+    u64p = C.POINTER(C.c_uint64)
+
+    def once():
+        rc = lib.scb_index_spectrum_device(idx._h, C.c_void_p(dev["qb"]), C.c_void_p(dev["qo"]), dev["qn"],
+                                           dev["qN"], vals.ctypes.data_as(u64p), cnts.ctypes.data_as(u64p),
+                                           cap, C.byref(n), C.byref(q), C.byref(d))
+        if rc:
+            raise RuntimeError(f"scb_index_spectrum_device failed ({rc})")
+
+    once()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        once()
+    t = (time.perf_counter() - t0) / reps
+    st = stage_times(scb)
+    return {"G_kmers_per_s": round(q.value / t / 1e9, 3), "ms": round(t * 1e3, 3),
+            "queried_kmers": q.value, "distinct_keys": d.value, "rows": n.value,
+            "stage_ms": {"scan": st[0], "plan_scatter": st[1], "bin": st[2], "total": st[3]}}
+
+
 def cfg_c2(scb):
     return scb.SuperCountingBloomConfig(k=31, m=21, s=25, n_hashes=4, counter_bits=16,
                                         counter_slots_exponent=30, block_slots_exponent=9,
@@ -316,6 +344,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-host", action="store_true")
     ap.add_argument("--no-io", action="store_true", help="skip the FASTQ and .scb file measurements")
+    ap.add_argument("--no-spectrum", action="store_true", help="skip the distinct-k-mer spectrum timing")
     ap.add_argument("--serve-qps", default="100,400")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--copy-probe", action="store_true",
@@ -380,6 +409,10 @@ def main():
         from scbf_amd import synth
         served = serve_bench(scb, idx_b, synth.genome(5_000_000, 1),
                              [float(q) for q in args.serve_qps.split(",")], args.serve_seconds)
+    spectrum = None
+    if not args.no_spectrum:
+        spectrum = spectrum_bench(scb, idx_b, dev)
+        log(f"spectrum: {spectrum}")
     del idx_b
 
     plain = None
@@ -487,6 +520,7 @@ def main():
         "served": served,
         "host_buffers": host_api,
         "files": io,
+        "spectrum": spectrum,
         "served_p50_ms": served["steps"][0]["p50_ms"] if served else None,
         "served_p99_ms": served["steps"][0]["p99_ms"] if served else None,
         "cpu_baseline": cpu,

@@ -1,6 +1,7 @@
-// pack4_bases: four ASCII bases of one 32-bit word encoded at once. Device code
-// of the scan; tests/cpp/pack4_check.cpp compiles it on the host (with a
-// portable v_perm_b32) and checks it against the per-base rule for every byte.
+// Base encoding and 2-bit base streams shared by the scan and the bin kernels.
+// Device code; tests/cpp/pack4_check.cpp and tests/cpp/smer_stream_check.cpp
+// compile it on the host (with portable stand-ins for the two AMDGPU builtins)
+// and check it against direct per-base definitions.
 #pragma once
 
 #include <cstdint>
@@ -21,4 +22,82 @@ __device__ __forceinline__ void pack4_bases(uint32_t x, uint32_t &codes, uint32_
     bad = (nz * 0x00204081u) >> 28;
     codes = ((idx & 0x03030303u) * 0x01041040u) >> 24;
     tg = ((((idx & 0x02020202u) << 6) * 0x00204081u) >> 28) & ~bad;
+}
+
+// Reverse complement of a len-base 2-bit word (packed-seq read_revcomp_kmer).
+__device__ __forceinline__ uint64_t revcomp(uint64_t x, uint32_t len) {
+    uint32_t lo = uint32_t(x), hi = uint32_t(x >> 32);
+    uint64_t r = (uint64_t(__builtin_bitreverse32(lo)) << 32) | __builtin_bitreverse32(hi);
+    r = ((r >> 1) & 0x5555555555555555ull) | ((r & 0x5555555555555555ull) << 1);
+    r ^= 0xAAAAAAAAAAAAAAAAull;
+    return r >> (64 - 2 * len);
+}
+
+__device__ __forceinline__ uint64_t kmask(uint32_t len) {
+    return len >= 32 ? ~0ull : ((1ull << (2 * len)) - 1);
+}
+
+// Streams the bases of one super-k-mer (from 4 pre-aligned words holding its
+// first 64 bases) and calls f(j, canonical s-mer) for each of its s-mers in
+// position order (filter.rs:224-226, 423-426 with canonical_kmer 622-627).
+// The first s-mer is read directly from the words; the rest roll one base at a
+// time.
+template <class F>
+__device__ __forceinline__ void for_each_smer(uint32_t A0, uint32_t A1, uint32_t A2, uint32_t A3,
+                                              uint32_t nbases, uint32_t s, F &&f) {
+    const uint64_t mask = kmask(s);
+    const uint32_t top = 2 * (s - 1);
+    uint64_t fw = ((uint64_t(A1) << 32) | A0) & mask;
+    uint64_t rc = revcomp(fw, s);
+    f(0u, fw < rc ? fw : rc);
+    // word stream positioned at base s (s is uniform across the wave)
+    if (s >= 16) A0 = A1, A1 = A2, A2 = A3;
+    if (s >= 32) A0 = A1, A1 = A2;
+    uint32_t cur = A0 >> (2 * (s & 15));
+    for (uint32_t i = s; i < nbases; ++i) {
+        uint32_t c = cur & 3;
+        cur >>= 2;
+        if ((i & 15) == 15) {
+            cur = A1;
+            A1 = A2;
+            A2 = A3;
+        }
+        fw = (fw >> 2) | (uint64_t(c) << top);
+        rc = ((rc << 2) | (c ^ 2)) & mask;
+        f(i + 1 - s, fw < rc ? fw : rc);
+    }
+}
+
+// for_each_smer that also rolls the k-mer ending at the same base (bases
+// packed first base lowest, as canonical_at reads them): f(j, canonical s-mer
+// j, canonical k-mer ending at s-mer j's last base). The k-mer is complete
+// from j = k - s on (j + 1 >= k - s + 1); before that it is not meaningful.
+// It starts as the first s-mer placed at the top of a k-base word, so the
+// bases shifted in after it complete it.
+template <class F>
+__device__ __forceinline__ void for_each_smer_kmer(uint32_t A0, uint32_t A1, uint32_t A2, uint32_t A3,
+                                                   uint32_t nbases, uint32_t s, uint32_t k, F &&f) {
+    const uint64_t mask = kmask(s), kmk = kmask(k);
+    const uint32_t top = 2 * (s - 1), ktop = 2 * (k - 1);
+    uint64_t fw = ((uint64_t(A1) << 32) | A0) & mask;
+    uint64_t rc = revcomp(fw, s);
+    uint64_t fk = fw << (2 * (k - s)), rk = rc;
+    f(0u, fw < rc ? fw : rc, fk < rk ? fk : rk);
+    if (s >= 16) A0 = A1, A1 = A2, A2 = A3;
+    if (s >= 32) A0 = A1, A1 = A2;
+    uint32_t cur = A0 >> (2 * (s & 15));
+    for (uint32_t i = s; i < nbases; ++i) {
+        uint32_t c = cur & 3;
+        cur >>= 2;
+        if ((i & 15) == 15) {
+            cur = A1;
+            A1 = A2;
+            A2 = A3;
+        }
+        fw = (fw >> 2) | (uint64_t(c) << top);
+        rc = ((rc << 2) | (c ^ 2)) & mask;
+        fk = (fk >> 2) | (uint64_t(c) << ktop);
+        rk = ((rk << 2) | (c ^ 2)) & kmk;
+        f(i + 1 - s, fw < rc ? fw : rc, fk < rk ? fk : rk);
+    }
 }

@@ -183,19 +183,6 @@ __device__ __forceinline__ uint32_t min_at_addresses(uint64_t smer, uint32_t nh,
     }
 }
 
-// Reverse complement of a len-base 2-bit word (packed-seq read_revcomp_kmer).
-__device__ __forceinline__ uint64_t revcomp(uint64_t x, uint32_t len) {
-    uint32_t lo = uint32_t(x), hi = uint32_t(x >> 32);
-    uint64_t r = (uint64_t(__builtin_bitreverse32(lo)) << 32) | __builtin_bitreverse32(hi);
-    r = ((r >> 1) & 0x5555555555555555ull) | ((r & 0x5555555555555555ull) << 1);
-    r ^= 0xAAAAAAAAAAAAAAAAull;
-    return r >> (64 - 2 * len);
-}
-
-__device__ __forceinline__ uint64_t kmask(uint32_t len) {
-    return len >= 32 ? ~0ull : ((1ull << (2 * len)) - 1);
-}
-
 // Canonical k-mer of len bases starting at base index b of a packed array.
 __device__ __forceinline__ uint64_t canonical_at(const uint32_t *words, uint64_t b, uint32_t len) {
     uint64_t wi = b >> 4;
@@ -288,37 +275,6 @@ __device__ __forceinline__ void sat_add_global(void *table, uint64_t idx, uint32
 template <int BITS>
 __device__ __forceinline__ uint32_t load_ctr(const void *base, uint64_t idx) {
     return reinterpret_cast<const typename Ctr<BITS>::T *>(base)[idx];
-}
-
-// Streams the bases of one super-k-mer (from 4 pre-aligned words holding its
-// first 64 bases) and calls f(j, canonical s-mer) for each of its s-mers in
-// position order (filter.rs:224-226, 423-426 with canonical_kmer 622-627).
-// The first s-mer is read directly from the words; the rest roll one base at a
-// time.
-template <class F>
-__device__ __forceinline__ void for_each_smer(uint32_t A0, uint32_t A1, uint32_t A2, uint32_t A3,
-                                              uint32_t nbases, uint32_t s, F &&f) {
-    const uint64_t mask = kmask(s);
-    const uint32_t top = 2 * (s - 1);
-    uint64_t fw = ((uint64_t(A1) << 32) | A0) & mask;
-    uint64_t rc = revcomp(fw, s);
-    f(0u, fw < rc ? fw : rc);
-    // word stream positioned at base s (s is uniform across the wave)
-    if (s >= 16) A0 = A1, A1 = A2, A2 = A3;
-    if (s >= 32) A0 = A1, A1 = A2;
-    uint32_t cur = A0 >> (2 * (s & 15));
-    for (uint32_t i = s; i < nbases; ++i) {
-        uint32_t c = cur & 3;
-        cur >>= 2;
-        if ((i & 15) == 15) {
-            cur = A1;
-            A1 = A2;
-            A2 = A3;
-        }
-        fw = (fw >> 2) | (uint64_t(c) << top);
-        rc = ((rc << 2) | (c ^ 2)) & mask;
-        f(i + 1 - s, fw < rc ? fw : rc);
-    }
 }
 
 // Loads 5 packed words starting at base b and aligns them so that A0 holds
@@ -1173,7 +1129,7 @@ __device__ __forceinline__ uint32_t sat_merge_word(uint32_t old, const uint32_t 
 // Work items hold at most this many records (their order lives in LDS as u16).
 constexpr uint32_t kMaxItemRecords = 4096;
 // Workgroup sizes of the bin kernels (their launch bounds).
-constexpr int kInsertThreads = 1024, kQueryThreads = 512;
+constexpr int kInsertThreads = 1024, kQueryThreads = 512, kSpectrumThreads = 1024;
 
 // Orders the records [rb, rb + n) of a work item by k-mer count, longest
 // first, into s_perm (indices relative to rb). A wave's s-mer loop runs as
@@ -1640,11 +1596,12 @@ __device__ __forceinline__ void spec_hist_add(const SpecArgs &a, uint32_t *s_his
 }
 
 template <int BITS, int WINC>
-__global__ __launch_bounds__(512) void k_bin_spectrum(SpecArgs sa, DevGeo g) {
+__global__ __launch_bounds__(kSpectrumThreads) void k_bin_spectrum(SpecArgs sa, DevGeo g) {
     extern __shared__ __align__(16) uint8_t s_raw[];
     __shared__ uint64_t s_item;
-    __shared__ uint32_t s_global;
-    __shared__ uint64_t s_red[3][8];
+    __shared__ uint64_t s_red[3][kSpectrumThreads / 64];
+    __shared__ uint16_t s_perm[kMaxItemRecords];
+    __shared__ uint32_t s_cls[32];
     using T = typename Ctr<BITS>::T;
     const BinArgs &a = sa.b;
     const uint32_t tid = threadIdx.x, nthr = blockDim.x, lane = tid & 63, wid = tid >> 6;
@@ -1679,15 +1636,15 @@ __global__ __launch_bounds__(512) void k_bin_spectrum(SpecArgs sa, DevGeo g) {
             for (uint32_t i = tid; i < kSetCap; i += nthr) s_set[i] = kEmptyKey;
         __syncthreads();
         const T *tbase = sa.stage ? lt : reinterpret_cast<const T *>(a.table) + (bin << a.bin_log);
-        for (uint32_t r = rb + tid; r < re; r += nthr) {
-            uint64_t rec = a.sorted[r];
-            uint64_t j0 = rec & 0xFFFFFFFFull;
-            uint32_t nk = uint32_t((rec >> 32) & 31) + 1;
-            uint32_t lbase = uint32_t((rec >> 37) & ((1u << lshift) - 1)) << g.bse;
+        order_by_length<kMaxItemRecords / kSpectrumThreads>(a.sorted, rb, re - rb, s_perm, s_cls, tid, nthr);  // (synchronizes)
+        walk_records(a.sorted, a.pk, g.k, rb, re - rb, s_perm, tid, nthr, [&](uint64_t rec, const uint32_t (&W)[5]) {
+            const uint32_t nk = uint32_t((rec >> 32) & 31) + 1;
+            const uint32_t lbase = uint32_t((rec >> 37) & ((1u << lshift) - 1)) << g.bse;
             uint32_t A0, A1, A2, A3;
-            load_aligned(a.pk, j0, A0, A1, A2, A3);
+            align_words(W, rec & 0xFFFFFFFFull, A0, A1, A2, A3);
             WinMin<WINC> wm;
-            for_each_smer(A0, A1, A2, A3, nk + g.k - 1, g.s, [&](uint32_t j, uint64_t smer) {
+            // the canonical k-mer key rolls along with the s-mers
+            for_each_smer_kmer(A0, A1, A2, A3, nk + g.k - 1, g.s, g.k, [&](uint32_t j, uint64_t smer, uint64_t key) {
                 uint32_t est = Ctr<BITS>::kMax;
                 if (g.bse <= 15) {
                     est = min_at_addresses(smer, g.nh, amask, tbase + lbase, Ctr<BITS>::kMax);
@@ -1701,7 +1658,6 @@ __global__ __launch_bounds__(512) void k_bin_spectrum(SpecArgs sa, DevGeo g) {
                 }
                 uint32_t ab = wm.push(est, g.win);
                 if (j + 1 >= g.win) {
-                    uint64_t key = canonical_at(a.pk, j0 + (j + 1 - g.win), g.k);
                     bool fresh = global ? set_insert(sa.gset, sa.glog2, key)
                                         : set_insert(s_set, 13, key);
                     ++queried;
@@ -1711,7 +1667,7 @@ __global__ __launch_bounds__(512) void k_bin_spectrum(SpecArgs sa, DevGeo g) {
                     }
                 }
             });
-        }
+        });
         __syncthreads();
     }
     for (uint32_t i = tid; i < kHistSmall; i += nthr)
@@ -2485,7 +2441,7 @@ Status Engine::spectrum(const DeviceBatch &b, SpectrumResult &r) const {
         sa.big_counts = ws.hist_big_counts.p;
         sa.stage = stage;
         size_t lds = (stage ? (size_t(1) << bin_log) * geo_.bytes : 0) + kSetCap * 8 + kHistSmall * 4;
-        const int thr = 512;
+        const int thr = kSpectrumThreads;
         bool wide = geo_.win > 8;
 #define SCB_SPEC(BITS, WC)                                                                      \
     do {                                                                                        \
@@ -2505,6 +2461,7 @@ Status Engine::spectrum(const DeviceBatch &b, SpectrumResult &r) const {
             s = Status{8, std::string("device error: k_bin_spectrum: ") + hipGetErrorString(e)};
             break;
         }
+        mark(ws, 3);
         std::vector<unsigned long long> dense(dense_n);
         std::vector<uint32_t> keys;
         std::vector<unsigned long long> cnts;
@@ -2516,6 +2473,7 @@ Status Engine::spectrum(const DeviceBatch &b, SpectrumResult &r) const {
             (void)hipMemcpyAsync(cnts.data(), ws.hist_big_counts.p, kHistBigCap * 8, hipMemcpyDeviceToHost, st);
         }
         if ((s = finish(ws)).code) break;
+        collect_times(ws, true);  // the bin stage includes the plan's host round trip
         if (ws.h_stats[kStOverflow]) {
             s = Status{8, "device error: spectrum has more than 2^20 distinct large estimates"};
             break;

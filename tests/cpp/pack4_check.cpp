@@ -15,7 +15,13 @@ static uint32_t host_perm(uint32_t src0, uint32_t src1, uint32_t sel) {
     }
     return out;
 }
+static uint32_t host_bitreverse32(uint32_t x) {  // used by the header's revcomp, not checked here
+    uint32_t r = 0;
+    for (int i = 0; i < 32; ++i) r |= ((x >> i) & 1u) << (31 - i);
+    return r;
+}
 #define __builtin_amdgcn_perm host_perm
+#define __builtin_bitreverse32 host_bitreverse32
 #define __device__
 #define __forceinline__ inline
 #include "scb_bases.h"

@@ -1,8 +1,10 @@
 """Device helpers with bit-level tricks, compiled for the host and checked
 exhaustively under the address and undefined-behaviour sanitizers:
 store_run (the query kernel's packed estimate stores) byte by byte for every
-alignment and length, and pack4_bases (the scan's four-at-a-time base encoding)
-against the per-base rule for every byte value at every position."""
+alignment and length, pack4_bases (the scan's four-at-a-time base encoding)
+against the per-base rule for every byte value at every position, and the
+rolling canonical s-mer / k-mer streams of the bin kernels against a direct
+evaluation for every k <= 32 and s <= k."""
 import os
 import shutil
 import subprocess
@@ -30,3 +32,8 @@ def test_store_run_writes_exactly_the_run(tmp_path):
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
 def test_pack4_bases_matches_the_per_base_rule(tmp_path):
     _run_check(tmp_path, "pack4_check")
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_smer_and_kmer_streams_match_direct_evaluation(tmp_path):
+    _run_check(tmp_path, "smer_stream_check")
