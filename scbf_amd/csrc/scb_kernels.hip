@@ -210,11 +210,6 @@ __device__ __forceinline__ uint32_t nt_seed(uint32_t c) {
     return (c & 2) ? b : a;
 }
 
-__device__ __forceinline__ bool is_acgt(uint32_t ch) {
-    uint32_t u = ch & 0xDF;
-    return u == 'A' || u == 'C' || u == 'G' || u == 'T';
-}
-
 __device__ __forceinline__ uint32_t get_bits(const uint32_t *bm, int li, int n) {
     if (n <= 0) return 0;
     uint32_t w0 = bm[li >> 5], w1 = bm[(li >> 5) + 1];
