@@ -107,8 +107,8 @@ __device__ __forceinline__ uint32_t xs64_low15(uint64_t x) {
     const uint64_t p = uint64_t(yl) * uint32_t(kC2);
     const uint32_t hi = uint32_t(p >> 32) + __umul24(yh & 0x3FFFu, uint32_t(kC2) & 0x3FFFu) +
                         __umul24(yl & 0x3FFFu, uint32_t(kC2 >> 32) & 0x3FFFu);
-    const uint64_t z = (uint64_t(hi) << 32) | uint32_t(p);
-    return uint32_t(z ^ (z >> 31));
+    // low word of z ^ (z >> 31) for z = hi:lo; (z >> 31) is one funnel shift
+    return uint32_t(p) ^ __builtin_amdgcn_alignbit(hi, uint32_t(p), 31);
 }
 
 // Calls f(address) for the n_hashes chain values xs64(x), xs64(xs64(x)), ...
