@@ -191,3 +191,61 @@ def test_cpp_wrapper_builder_roundtrip(gpu, tmp_path):
     out = subprocess.run([str(exe), str(tmp_path)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     assert "cpp api ok" in out.stdout
+
+
+def _write_fasta(path, records):
+    path.write_bytes(b"".join(b">r%d\n%s\n" % (i, r) for i, r in enumerate(records)))
+
+
+def _ragged_records(seed, n):
+    """Records of 0..400 bases with N runs, runs shorter than k, lowercase stretches
+    and other IUPAC letters: every record splits into several chunks or none."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        r = bytearray(rng.choice(np.frombuffer(b"ACGT", np.uint8), size=int(rng.integers(0, 400))).tobytes())
+        for _ in range(int(rng.integers(0, 5))):
+            if not r:
+                break
+            a = int(rng.integers(0, len(r)))
+            run = rng.choice(np.frombuffer(b"NNNNRY", np.uint8), size=int(rng.integers(1, 8))).tobytes()
+            r[a:a + len(run)] = run
+        if r and rng.random() < 0.3:
+            a = int(rng.integers(0, len(r)))
+            r[a:a + 40] = bytes(r[a:a + 40]).lower()
+        out.append(bytes(r))
+    return out
+
+
+@pytest.mark.parametrize("bits", [8, 16, 32])
+def test_fastx_pipeline_with_n_runs_matches_oracle(gpu, oracle, tmp_path, bits):
+    """Whole records go to the device and chunks (ACGT runs of at least k bases,
+    fastx.rs:14-19) are counted on the host: chunk counts, indexed and queried
+    k-mers, s-mers and the estimate checksum of the stream pipeline equal the
+    oracle's restatement of the threaded pipeline (fastx.rs:172-204, 206-369)."""
+    scb = gpu
+    k, m, s, h, cse, bse = 21, 11, 15, 3, 18, 8
+    ins, qry = _ragged_records(21, 400), _ragged_records(22, 300) + _ragged_records(21, 100)
+    fi, fq = tmp_path / "index.fa", tmp_path / "query.fa"
+    _write_fasta(fi, ins)
+    _write_fasta(fq, qry)
+    cfg = scb.SuperCountingBloomConfig(k=k, m=m, s=s, n_hashes=h, counter_bits=bits, counter_slots_exponent=cse,
+                                       block_slots_exponent=bse, threads=2, queue=4)
+    st = scb.stream_estimates_from_fastx_pair(fi, fq, cfg).stats
+    build = scb.SuperCountingBloomIndex.build_from_fastx(fi, cfg)
+
+    def concat(recs):
+        offs = np.zeros(len(recs) + 1, dtype=np.uint64)
+        offs[1:] = np.cumsum([len(r) for r in recs])
+        return np.frombuffer(b"".join(recs), dtype=np.uint8).copy(), offs
+
+    ib, io = concat(ins)
+    qb, qo = concat(qry)
+    ob = oracle.cpu_baseline(oracle.make_cfg(k, m, s, h, bits, cse, bse, 2, 4), ib, io, qb, qo)
+    got = (st.chunks, st.indexed_kmers, st.inserted_smers, st.queried_kmers, st.positive_estimates,
+           st.estimate_checksum)
+    want = (ob.chunks, ob.indexed_kmers, ob.inserted_smers, ob.queried_kmers, ob.positive_estimates,
+            ob.estimate_checksum)
+    assert got == want
+    assert (build.chunks, build.indexed_kmers, build.inserted_smers) == want[:3]
+    assert st.chunks > len(ins)  # the N runs split records into several chunks
