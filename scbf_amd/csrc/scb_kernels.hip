@@ -1849,6 +1849,48 @@ void fill(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_u32, dim3(unsigned(blocks)), dim3(256), 0, st, p, n, v);
 }
 
+// Fills of several buffers in one launch: a pipeline starts by clearing five
+// or six small buffers, and one launch each costs more than the stores.
+struct FillJobs {
+    static constexpr int kMax = 8;
+    uint32_t *p[kMax];
+    uint64_t n[kMax];
+    uint32_t v[kMax];
+};
+
+__global__ void k_fill_jobs(FillJobs j) {
+    const int q = blockIdx.y;
+    uint32_t *p = j.p[q];
+    const uint64_t n = j.n[q];
+    const uint32_t v = j.v[q];
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        p[i] = v;
+}
+
+class FillBatch {
+   public:
+    explicit FillBatch(hipStream_t st) : st_(st) {}
+    void add(uint32_t *p, uint64_t n, uint32_t v) {
+        if (!n) return;
+        if (count_ == FillJobs::kMax) flush();
+        jobs_.p[count_] = p, jobs_.n[count_] = n, jobs_.v[count_] = v;
+        ++count_;
+        maxn_ = std::max(maxn_, n);
+    }
+    void flush() {
+        if (!count_) return;
+        const uint64_t blocks = std::min<uint64_t>((maxn_ + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_fill_jobs, dim3(unsigned(blocks), unsigned(count_)), dim3(256), 0, st_, jobs_);
+        count_ = 0, maxn_ = 0;
+    }
+
+   private:
+    hipStream_t st_;
+    FillJobs jobs_{};
+    int count_ = 0;
+    uint64_t maxn_ = 0;
+};
+
 int cu_count(int dev) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 256;
@@ -2115,11 +2157,25 @@ Status batched_front(Launch &L, const DeviceBatch &b, uint32_t bin_log, uint32_t
     SCB_HIP(ws.work.reserve(nbins + cap / per_item + 2));
     SCB_HIP(ws.plan_blk.reserve(2 * ((nbins + kPlanThreads - 1) / kPlanThreads)));
     hipStream_t st = ws.stream;
-    fill(reinterpret_cast<uint32_t *>(ws.stats.p), 2 * kStatsLen, 0, st);
-    fill(ws.seg_count.p, kSeg + 1, 0, st);
-    fill(ws.bin_count.p, nbins, 0, st);
-    if (bin_kmers) fill(bin_kmers, nbins, 0, st);
-    fill(ws.pk.p + (words - kPkPadWords), kPkPadWords, 0, st);
+    // coarse pass by bin >> fine_bits (then a fine pass by bin) when the bins
+    // exceed one tile's key range
+    const bool two_pass = nbins > kPartRange;
+    const uint32_t lb = uint32_t(__builtin_ctzll(nbins));
+    const uint32_t fine_bits = std::max<uint32_t>(7, lb > 11 ? lb - 11 : 0);
+    const uint32_t ncoarse = two_pass ? uint32_t(nbins >> fine_bits) : 0;
+    constexpr uint32_t kCopies = 32;  // coarse counters are hot: spread each over 32 copies
+    if (two_pass) {
+        SCB_HIP(ws.coarse_hist.reserve(ncoarse * kCopies));
+        SCB_HIP(ws.ccursor.reserve(ncoarse * kCopies));
+    }
+    FillBatch fb(st);
+    fb.add(reinterpret_cast<uint32_t *>(ws.stats.p), 2 * kStatsLen, 0);
+    fb.add(ws.seg_count.p, kSeg + 1, 0);
+    fb.add(ws.bin_count.p, nbins, 0);
+    if (bin_kmers) fb.add(bin_kmers, nbins, 0);
+    fb.add(ws.pk.p + (words - kPkPadWords), kPkPadWords, 0);
+    if (two_pass) fb.add(ws.coarse_hist.p, ncoarse * kCopies, 0);
+    fb.flush();
     mark(ws, 0);
     Status s = launch_scan<kModeRecords>(L, b, seg_cap, valid_bits, nullptr);
     if (!s.good()) return s;
@@ -2130,15 +2186,7 @@ Status batched_front(Launch &L, const DeviceBatch &b, uint32_t bin_log, uint32_t
     // the coarse copy count (each workgroup's tiles then share one copy)
     const dim3 hg(unsigned((cu_count(L.device) * 6 + 31) / 32 * 32));
     SegIn segs{ws.rec.p, ws.seg_count.p, kSeg, seg_cap};
-    if (nbins > kPartRange) {
-        // coarse pass by bin >> fine_bits, then a fine pass by bin
-        const uint32_t lb = uint32_t(__builtin_ctzll(nbins));
-        const uint32_t fine_bits = std::max<uint32_t>(7, lb > 11 ? lb - 11 : 0);
-        const uint32_t ncoarse = uint32_t(nbins >> fine_bits);
-        constexpr uint32_t kCopies = 32;  // coarse counters are hot: spread each over 32 copies
-        SCB_HIP(ws.coarse_hist.reserve(ncoarse * kCopies));
-        SCB_HIP(ws.ccursor.reserve(ncoarse * kCopies));
-        fill(ws.coarse_hist.p, ncoarse * kCopies, 0, st);
+    if (two_pass) {
         KeySpec coarse{bin_shift, fine_bits, 0, 0, kCopies};
         hipLaunchKernelGGL(k_seg_hist, hg, pb, 0, st, segs, coarse, ws.coarse_hist.p, (uint32_t *)nullptr);
         hipLaunchKernelGGL(k_small_scan, dim3(1), dim3(1024), 0, st, ws.coarse_hist.p, ncoarse * kCopies,
