@@ -991,28 +991,34 @@ __global__ __launch_bounds__(kPartThreads) void k_seg_hist(SegIn in, KeySpec ks,
 // a contiguous run of n/1024 entries); the total becomes the record count.
 __global__ __launch_bounds__(1024) void k_small_scan(const uint32_t *hist, uint32_t n, uint32_t *cursor,
                                                      uint64_t *stats, uint32_t *total_out) {
-    __shared__ uint32_t p[1024];
-    const uint32_t tid = threadIdx.x;
+    __shared__ uint32_t s_wave[16];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const uint32_t per = (n + 1023) / 1024;
     const uint32_t b0 = tid * per, b1 = min(n, b0 + per);
     uint32_t sum = 0;
     for (uint32_t i = b0; i < b1; ++i) sum += hist[i];
-    p[tid] = sum;
-    __syncthreads();
-    for (uint32_t d = 1; d < 1024; d <<= 1) {
-        uint32_t v = tid >= d ? p[tid - d] : 0;
-        __syncthreads();
-        p[tid] += v;
-        __syncthreads();
+    // exclusive prefix of the per-thread sums: wave scans, then the wave totals
+    uint32_t incl = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off, 64);
+        if (lane >= uint32_t(off)) incl += y;
     }
-    uint32_t run = p[tid] - sum;
+    if (lane == 63) s_wave[wid] = incl;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (uint32_t i = 0; i < 16; ++i) {
+        const uint32_t w = s_wave[i];
+        before += i < wid ? w : 0u;
+        total += w;
+    }
+    uint32_t run = before + incl - sum;
     for (uint32_t i = b0; i < b1; ++i) {
         cursor[i] = run;
         run += hist[i];
     }
-    if (tid == 1023) {
-        stats[kStRecords] = p[1023];
-        *total_out = p[1023];
+    if (tid == 0) {
+        stats[kStRecords] = total;
+        *total_out = total;
     }
 }
 
