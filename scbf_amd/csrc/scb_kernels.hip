@@ -1106,6 +1106,14 @@ __device__ __forceinline__ void zero_lds(uint32_t *lds, uint32_t nwords, uint32_
     }
 }
 
+// Two 16-bit counters per word plus two 16-bit counts, each saturated at 65535
+// (one packed add with clamp).
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t sat_add_u16x2(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t,
+                              __builtin_elementwise_add_sat(__builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b)));
+}
+
 // One 32-bit table word plus the increment counts of its counters, saturated
 // per counter (counter.rs:20-23 applied count times).
 template <int BITS>
@@ -1371,6 +1379,22 @@ __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(
                         uint32_t any = write_all;
                         const uint32_t s0 = vi * 4 * per, sh = 16 * (s0 >> half_log);
                         const uint4 *cw = reinterpret_cast<const uint4 *>(s_cnt + (s0 & (half_words - 1)));
+                        if (BITS == 16) {
+                            // counts of slots s0..s0+7: the low (first half of the
+                            // bin) or high halves of eight LDS words, paired into
+                            // table order by a byte permute, then one clamped
+                            // packed add per table word
+                            const uint4 x0 = cw[0], x1 = cw[1];
+                            const uint32_t sel = sh ? 0x07060302u : 0x05040100u;
+                            const uint32_t p0 = __builtin_amdgcn_perm(x0.y, x0.x, sel);
+                            const uint32_t p1 = __builtin_amdgcn_perm(x0.w, x0.z, sel);
+                            const uint32_t p2 = __builtin_amdgcn_perm(x1.y, x1.x, sel);
+                            const uint32_t p3 = __builtin_amdgcn_perm(x1.w, x1.z, sel);
+                            if (!(any | p0 | p1 | p2 | p3)) continue;
+                            tv[vi] = make_uint4(sat_add_u16x2(old[u].x, p0), sat_add_u16x2(old[u].y, p1),
+                                                sat_add_u16x2(old[u].z, p2), sat_add_u16x2(old[u].w, p3));
+                            continue;
+                        }
 #pragma unroll
                         for (uint32_t q = 0; q < per; ++q) {
                             const uint4 x = cw[q];

@@ -25,6 +25,10 @@ GEOMETRIES = [
     ("k31m21s25_u8_sat", 31, 21, 25, 4, 8, 14, 9),  # dense: saturates u8
     ("k31m21s25_u16_bse15", 31, 21, 25, 4, 16, 20, 15),  # 15-bit slot addresses (largest staged block)
     ("k15m9s11_u32_h1", 15, 9, 11, 1, 32, 18, 12),  # one hash per s-mer
+    ("k31m21s30_u16_win2", 31, 21, 30, 3, 16, 20, 9),  # s-mers of 60 bits, fimpera window 2
+    # one 2^15-slot bin holds every record: work items split at 4096 records
+    # merge into the table with atomics, sub-batches of ~960 records each
+    ("k31m21s25_u16_onebin", 31, 21, 25, 4, 16, 15, 9),
     # tiny tables (config.rs:76 admits any bse < cse): one bin smaller than the
     # merge vectors, 16-bit counts saturating across insert sub-batches
     ("k9m5s7_u16_16slots", 9, 5, 7, 3, 16, 4, 1),
