@@ -909,10 +909,13 @@ __device__ __forceinline__ uint32_t tile_lo(const uint64_t *tile, const KeySpec 
 
 // Numbers the non-empty tiles of all segments densely: s_pre[s] is the first
 // tile of segment s (s_pre[nseg] the total), so that a grid-stride loop visits
-// only tiles holding records. Run by the whole workgroup (nseg <= 64).
-__device__ __forceinline__ uint32_t dense_tiles(const SegIn &in, uint32_t *s_pre, uint32_t tid) {
+// only tiles holding records. Run by the whole workgroup (nseg <= 64); s_n
+// (if given) receives the record count of every segment.
+__device__ __forceinline__ uint32_t dense_tiles(const SegIn &in, uint32_t *s_pre, uint32_t tid,
+                                                uint32_t *s_n = nullptr) {
     if (tid < 64) {
         const uint32_t n = tid < in.nseg ? uint32_t(min<uint64_t>(in.counts[tid], in.seg_cap)) : 0;
+        if (s_n) s_n[tid] = n;
         uint32_t incl = (n + kPartTile - 1) / kPartTile;
         for (int off = 1; off < 64; off <<= 1) {
             uint32_t y = __shfl_up(incl, off, 64);
@@ -1022,35 +1025,53 @@ __global__ __launch_bounds__(1024) void k_small_scan(const uint32_t *hist, uint3
     }
 }
 
+// Persistent: a workgroup takes tiles u = blockIdx.x + i * gridDim.x and
+// loads the records of its next tile into registers while the current one is
+// ranked, reserved and scattered, so neither the record loads nor the cursor
+// reservations of consecutive tiles wait on each other.
 __global__ __launch_bounds__(kPartThreads) void k_partition(SegIn in, KeySpec ks, uint32_t *cursor,
                                                             uint64_t *dst) {
     __shared__ uint32_t s_hist[kPartRange];
     __shared__ uint32_t s_lo;
-    __shared__ uint32_t s_pre[65];
+    __shared__ uint32_t s_pre[65], s_n[64];
     const uint32_t tid = threadIdx.x;
-    const uint32_t total = dense_tiles(in, s_pre, tid);
-    for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+    const uint32_t total = dense_tiles(in, s_pre, tid, s_n);
+    // records of dense tile u, item j of this thread at j * kPartThreads + tid;
+    // indices are clamped to the tile (which holds at least one record), so the
+    // loads carry no condition and stay in flight until the tile is processed
+    auto load_tile = [&](uint32_t u, uint64_t (&r)[kPartItems], uint32_t &cnt) {
         const uint32_t seg = tile_segment(s_pre, in.nseg, u);
-        const uint64_t t0 = uint64_t(u - s_pre[seg]) * kPartTile;
-        const uint64_t n = min<uint64_t>(in.counts[seg], in.seg_cap);
+        const uint32_t t0 = (u - s_pre[seg]) * uint32_t(kPartTile);
+        cnt = min<uint32_t>(kPartTile, s_n[seg] - t0);
         const uint64_t *tile = in.src + seg * in.seg_cap + t0;
-        const uint32_t cnt = uint32_t(min<uint64_t>(kPartTile, n - t0));
+#pragma unroll
+        for (int j = 0; j < kPartItems; ++j) r[j] = tile[min(uint32_t(j) * kPartThreads + tid, cnt - 1)];
+    };
+    uint64_t rn[kPartItems];
+    uint32_t cnt_n = 0;
+    if (blockIdx.x < total) load_tile(blockIdx.x, rn, cnt_n);
+    for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
+        uint64_t r[kPartItems];
+#pragma unroll
+        for (int j = 0; j < kPartItems; ++j) r[j] = rn[j];
+        const uint32_t cnt = cnt_n;
+        if (u + gridDim.x < total) load_tile(u + gridDim.x, rn, cnt_n);
         for (uint32_t i = tid; i < kPartRange; i += kPartThreads) s_hist[i] = 0;
-        if (tid == 0) s_lo = tile_lo(tile, ks);
+        // thread 0's first item is the tile's first record
+        if (tid == 0) s_lo = ks.tile_based ? (rec_bin(r[0], ks.bin_shift) >> ks.fine_bits) << ks.fine_bits : 0u;
         __syncthreads();
         const uint32_t lo = s_lo;
-        uint64_t r[kPartItems];
-        uint32_t key[kPartItems], rank[kPartItems];
+        // key << 13 | rank in the tile's run of that key (key < 2^11, rank < 2^13)
+        static_assert(kPartRange <= (1u << 11) && kPartTile <= (1 << 13), "key and rank share a word");
+        uint32_t kr[kPartItems];
 #pragma unroll
         for (int j = 0; j < kPartItems; ++j) {
             uint32_t i = uint32_t(j) * kPartThreads + tid;
-            key[j] = 0xFFFFFFFFu;
+            kr[j] = 0xFFFFFFFFu;
             if (i < cnt) {
-                r[j] = tile[i];
                 uint32_t kk = (rec_bin(r[j], ks.bin_shift) >> ks.key_shift) - lo;
                 if (kk < kPartRange) {
-                    key[j] = kk;
-                    rank[j] = atomicAdd(&s_hist[kk], 1u);
+                    kr[j] = kk << 13 | atomicAdd(&s_hist[kk], 1u);
                 } else {  // outside the tile's range: reserve one slot directly
                     dst[atomicAdd(&cursor[(kk + lo) * ks.copies + u % ks.copies], 1u)] = r[j];
                 }
@@ -1065,7 +1086,7 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(SegIn in, KeySpec ks
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < kPartItems; ++j)
-            if (key[j] != 0xFFFFFFFFu) dst[s_hist[key[j]] + rank[j]] = r[j];
+            if (kr[j] != 0xFFFFFFFFu) dst[s_hist[kr[j] >> 13] + (kr[j] & 0x1FFFu)] = r[j];
         __syncthreads();
     }
 }
