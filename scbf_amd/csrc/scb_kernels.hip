@@ -901,12 +901,6 @@ __device__ __forceinline__ uint32_t rec_bin(uint64_t r, uint32_t bin_shift) {
     return uint32_t((r >> 37) >> bin_shift);
 }
 
-__device__ __forceinline__ uint32_t tile_lo(const uint64_t *tile, const KeySpec &ks) {
-    if (!ks.tile_based) return 0;
-    uint32_t b0 = rec_bin(tile[0], ks.bin_shift);
-    return (b0 >> ks.fine_bits) << ks.fine_bits;
-}
-
 // Numbers the non-empty tiles of all segments densely: s_pre[s] is the first
 // tile of segment s (s_pre[nseg] the total), so that a grid-stride loop visits
 // only tiles holding records. Run by the whole workgroup (nseg <= 64); s_n
@@ -938,6 +932,27 @@ __device__ __forceinline__ uint32_t tile_segment(const uint32_t *s_pre, uint32_t
     return lo;
 }
 
+// Records of dense tile u (s_pre and s_n from dense_tiles), item j of this
+// thread at j * kPartThreads + tid; cnt = records in the tile. Indices are
+// clamped to the tile (which holds at least one record), so the loads carry
+// no condition and stay in flight until the records are used; thread 0's first
+// item is the tile's first record.
+__device__ __forceinline__ void load_part_tile(const SegIn &in, const uint32_t *s_pre, const uint32_t *s_n,
+                                               uint32_t u, uint32_t tid, uint64_t (&r)[kPartItems],
+                                               uint32_t &cnt) {
+    const uint32_t seg = tile_segment(s_pre, in.nseg, u);
+    const uint32_t t0 = (u - s_pre[seg]) * uint32_t(kPartTile);
+    cnt = min<uint32_t>(kPartTile, s_n[seg] - t0);
+    const uint64_t *tile = in.src + seg * in.seg_cap + t0;
+#pragma unroll
+    for (int j = 0; j < kPartItems; ++j) r[j] = tile[min(uint32_t(j) * kPartThreads + tid, cnt - 1)];
+}
+
+// The key window of a tile whose first record is r0 (see KeySpec).
+__device__ __forceinline__ uint32_t window_lo(uint64_t r0, const KeySpec &ks) {
+    return ks.tile_based ? (rec_bin(r0, ks.bin_shift) >> ks.fine_bits) << ks.fine_bits : 0u;
+}
+
 // Histogram of keys (records per key, and k-mers per key when kmers != null).
 // The workgroup's tiles u = blockIdx.x + i * gridDim.x share u % copies (the
 // grid is a multiple of the copy count), so counts accumulate in LDS across
@@ -946,9 +961,9 @@ __global__ __launch_bounds__(kPartThreads) void k_seg_hist(SegIn in, KeySpec ks,
                                                            uint32_t *kmers) {
     __shared__ uint32_t s_h[kPartRange], s_k[kPartRange];
     __shared__ uint32_t s_lo;
-    __shared__ uint32_t s_pre[65];
+    __shared__ uint32_t s_pre[65], s_n[64];
     const uint32_t tid = threadIdx.x;
-    const uint32_t total = dense_tiles(in, s_pre, tid);
+    const uint32_t total = dense_tiles(in, s_pre, tid, s_n);
     const uint32_t copy = blockIdx.x % ks.copies;
     uint32_t lo = 0xFFFFFFFFu;  // key window of the counts held in LDS (none yet)
     auto flush = [&]() {
@@ -959,13 +974,17 @@ __global__ __launch_bounds__(kPartThreads) void k_seg_hist(SegIn in, KeySpec ks,
         }
     };
     for (uint32_t i = tid; i < kPartRange; i += kPartThreads) s_h[i] = 0, s_k[i] = 0;
+    // the next tile's records load while the current one is counted
+    uint64_t rn[kPartItems];
+    uint32_t cnt_n = 0;
+    if (blockIdx.x < total) load_part_tile(in, s_pre, s_n, blockIdx.x, tid, rn, cnt_n);
     for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
-        const uint32_t seg = tile_segment(s_pre, in.nseg, u);
-        const uint64_t t0 = uint64_t(u - s_pre[seg]) * kPartTile;
-        const uint64_t n = min<uint64_t>(in.counts[seg], in.seg_cap);
-        const uint64_t *tile = in.src + seg * in.seg_cap + t0;
-        const uint32_t cnt = uint32_t(min<uint64_t>(kPartTile, n - t0));
-        if (tid == 0) s_lo = tile_lo(tile, ks);
+        uint64_t r[kPartItems];
+#pragma unroll
+        for (int j = 0; j < kPartItems; ++j) r[j] = rn[j];
+        const uint32_t cnt = cnt_n;
+        if (u + gridDim.x < total) load_part_tile(in, s_pre, s_n, u + gridDim.x, tid, rn, cnt_n);
+        if (tid == 0) s_lo = window_lo(r[0], ks);
         __syncthreads();
         const uint32_t tlo = s_lo;
         if (tlo != lo) {  // uniform: a new key window
@@ -973,10 +992,11 @@ __global__ __launch_bounds__(kPartThreads) void k_seg_hist(SegIn in, KeySpec ks,
             lo = tlo;
             __syncthreads();
         }
-        for (uint32_t i = tid; i < cnt; i += kPartThreads) {
-            uint64_t r = tile[i];
-            uint32_t key = rec_bin(r, ks.bin_shift) >> ks.key_shift;
-            uint32_t nk = uint32_t((r >> 32) & 31) + 1;
+#pragma unroll
+        for (int j = 0; j < kPartItems; ++j) {
+            if (uint32_t(j) * kPartThreads + tid >= cnt) continue;
+            const uint32_t key = rec_bin(r[j], ks.bin_shift) >> ks.key_shift;
+            const uint32_t nk = uint32_t((r[j] >> 32) & 31) + 1;
             if (key - lo < kPartRange) {
                 atomicAdd(&s_h[key - lo], 1u);
                 if (kmers) atomicAdd(&s_k[key - lo], nk);
@@ -1036,29 +1056,17 @@ __global__ __launch_bounds__(kPartThreads) void k_partition(SegIn in, KeySpec ks
     __shared__ uint32_t s_pre[65], s_n[64];
     const uint32_t tid = threadIdx.x;
     const uint32_t total = dense_tiles(in, s_pre, tid, s_n);
-    // records of dense tile u, item j of this thread at j * kPartThreads + tid;
-    // indices are clamped to the tile (which holds at least one record), so the
-    // loads carry no condition and stay in flight until the tile is processed
-    auto load_tile = [&](uint32_t u, uint64_t (&r)[kPartItems], uint32_t &cnt) {
-        const uint32_t seg = tile_segment(s_pre, in.nseg, u);
-        const uint32_t t0 = (u - s_pre[seg]) * uint32_t(kPartTile);
-        cnt = min<uint32_t>(kPartTile, s_n[seg] - t0);
-        const uint64_t *tile = in.src + seg * in.seg_cap + t0;
-#pragma unroll
-        for (int j = 0; j < kPartItems; ++j) r[j] = tile[min(uint32_t(j) * kPartThreads + tid, cnt - 1)];
-    };
     uint64_t rn[kPartItems];
     uint32_t cnt_n = 0;
-    if (blockIdx.x < total) load_tile(blockIdx.x, rn, cnt_n);
+    if (blockIdx.x < total) load_part_tile(in, s_pre, s_n, blockIdx.x, tid, rn, cnt_n);
     for (uint32_t u = blockIdx.x; u < total; u += gridDim.x) {
         uint64_t r[kPartItems];
 #pragma unroll
         for (int j = 0; j < kPartItems; ++j) r[j] = rn[j];
         const uint32_t cnt = cnt_n;
-        if (u + gridDim.x < total) load_tile(u + gridDim.x, rn, cnt_n);
+        if (u + gridDim.x < total) load_part_tile(in, s_pre, s_n, u + gridDim.x, tid, rn, cnt_n);
         for (uint32_t i = tid; i < kPartRange; i += kPartThreads) s_hist[i] = 0;
-        // thread 0's first item is the tile's first record
-        if (tid == 0) s_lo = ks.tile_based ? (rec_bin(r[0], ks.bin_shift) >> ks.fine_bits) << ks.fine_bits : 0u;
+        if (tid == 0) s_lo = window_lo(r[0], ks);
         __syncthreads();
         const uint32_t lo = s_lo;
         // key << 13 | rank in the tile's run of that key (key < 2^11, rank < 2^13)
