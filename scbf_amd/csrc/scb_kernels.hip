@@ -395,6 +395,11 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
     __shared__ uint64_t s_red[3][kTileThreads / 64];
     __shared__ uint64_t s_base;
     __shared__ uint32_t s_r0;
+    // ntHash XOR-prefix terms by (strand, code, rotation): forward rotr(seed(c), r),
+    // reverse rotl(seed(c ^ 2), r); rows of 33 words put the four codes of one
+    // rotation in different banks
+    constexpr int kSeedRow = 33;
+    __shared__ uint32_t s_seed[2 * 4 * kSeedRow];
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     constexpr int nwave = kTileThreads / 64;
@@ -446,6 +451,11 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
         const int64_t LB = t0 - kLeft;
         for (int i = tid; i < kSpanWords; i += kTileThreads) s_brk[i] = 0;
         if (tid == 0) s_r0 = a.tile_r0[tile];
+        for (int e = tid; e < 2 * 4 * 32; e += kTileThreads) {
+            const uint32_t strand = uint32_t(e) >> 7, c = (uint32_t(e) >> 5) & 3, r = uint32_t(e) & 31;
+            s_seed[(strand * 4 + c) * kSeedRow + r] =
+                strand ? rotl32(nt_seed(c ^ 2), r) : rotl32(nt_seed(c), (32 - r) & 31);
+        }
 
         // (1) codes and masks of this thread's 16 bases
         uint32_t cw = 0;
@@ -471,17 +481,20 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
 
         // (2) ntHash XOR prefixes: the chunk-local exclusive prefixes stay in
         // registers until the prefix of the preceding chunks is known, then go to
-        // LDS complete, so each m-mer hash reads two words per strand
+        // LDS complete, so each m-mer hash reads two words per strand. Base
+        // 16 * tid + t is rotated by r0 + t (r0 = 16 for odd threads): its terms
+        // come from the seed table.
+        __syncthreads();  // the seed table is complete
         uint32_t tf[16], tr[16];
         uint32_t xf = 0, xr = 0;
         if (tid < kChunks) {
+            const uint32_t r0 = 16 * (uint32_t(tid) & 1);
 #pragma unroll
             for (int t = 0; t < 16; ++t) {
-                uint32_t c = (cw >> (2 * t)) & 3;
-                uint32_t j = uint32_t(16 * tid + t);
+                const uint32_t ix = ((cw >> (2 * t)) & 3) * kSeedRow + r0 + uint32_t(t);
                 tf[t] = xf, tr[t] = xr;
-                xf ^= rotl32(nt_seed(c), (32 - (j & 31)) & 31);
-                xr ^= rotl32(nt_seed(c ^ 2), j & 31);
+                xf ^= s_seed[ix];
+                xr ^= s_seed[4 * kSeedRow + ix];
             }
         }
         uint32_t sf = xf, sr = xr;
