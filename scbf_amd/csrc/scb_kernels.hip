@@ -394,7 +394,6 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
     __shared__ uint32_t s_bnd[kTile / 32 + 2];
     __shared__ uint64_t s_red[3][kTileThreads / 64];
     __shared__ uint64_t s_base;
-    __shared__ uint32_t s_r0;
     // ntHash XOR-prefix terms by (strand, code, rotation): forward rotr(seed(c), r),
     // reverse rotl(seed(c ^ 2), r); rows of 33 words put the four codes of one
     // rotation in different banks
@@ -450,7 +449,13 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
         const int64_t t0 = int64_t(tile) * kTile;
         const int64_t LB = t0 - kLeft;
         for (int i = tid; i < kSpanWords; i += kTileThreads) s_brk[i] = 0;
-        if (tid == 0) s_r0 = a.tile_r0[tile];
+        // the first read start after the span's left edge (uniform), and this
+        // thread's first candidate start, loaded now so that the load overlaps
+        // the work before the read boundaries are marked (the index is clamped,
+        // the test is made where the value is used)
+        const uint32_t r0 = a.tile_r0[tile];
+        const uint64_t rfirst = uint64_t(r0) + tid;
+        const int64_t ofirst = int64_t(a.offsets[min<uint64_t>(rfirst, a.n_reads - 1)]);
         for (int e = tid; e < 2 * 4 * 32; e += kTileThreads) {
             const uint32_t strand = uint32_t(e) >> 7, c = (uint32_t(e) >> 5) & 3, r = uint32_t(e) & 31;
             s_seed[(strand * 4 + c) * kSeedRow + r] =
@@ -505,12 +510,13 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
         if (lane == 63) s_wx[0][wid] = sf, s_wx[1][wid] = sr;
         __syncthreads();
         {   // record starts inside the span mark read boundaries
-            const uint32_t r0 = s_r0;
-            for (uint64_t r = uint64_t(r0) + tid; r < a.n_reads; r += kTileThreads) {
-                int64_t o = int64_t(a.offsets[r]);
-                if (o >= LB + kSpan) break;
-                int li = int(o - LB);
+            uint64_t r = rfirst;
+            int64_t o = ofirst;
+            while (r < a.n_reads && o < LB + kSpan) {
+                const int li = int(o - LB);
                 atomicOr(&s_brk[li >> 5], 1u << (li & 31));
+                r += kTileThreads;
+                if (r < a.n_reads) o = int64_t(a.offsets[r]);
             }
         }
         if (tid < kChunks) {
