@@ -384,10 +384,14 @@ template <int MODE, int BITS, int WINC, int WREG = 0>
 __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
     __shared__ uint32_t s_bad[kSpanWords], s_tg[kSpanWords], s_brk[kSpanWords];
     __shared__ uint32_t s_pk[kPkWords];
-    __shared__ __align__(16) uint16_t s_K[kHashN];  // high 16 bits of each m-mer hash (the argmin key)
     // Gf | Gr (padded: index li + li/16 keeps the 16-per-thread writes
-    // bank-conflict free), then the four block prefix/suffix minimum arrays
-    __shared__ uint32_t s_G[kGArr];
+    // bank-conflict free); before them the seed table below, after them the
+    // block prefix/suffix minimum arrays (and, with register windows, the keys)
+    __shared__ __align__(16) uint32_t s_G[kGArr];
+    // high 16 bits of each m-mer hash (the argmin key): its own array for the
+    // block minima, which use all four u16 arrays of s_G; with register windows
+    // it lives in s_G (LDS of 20 KiB per tile: eight tiles share a CU)
+    __shared__ __align__(16) uint16_t s_Kb[WREG > 0 ? 8 : kHashN];
     __shared__ uint32_t s_wx[2][kTileThreads / 64];
     __shared__ uint32_t s_wc[kTileThreads / 64];
     // bit b set: window b + 1 ends the super-k-mer before it (invalid or a new start)
@@ -396,9 +400,10 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
     __shared__ uint64_t s_base;
     // ntHash XOR-prefix terms by (strand, code, rotation): forward rotr(seed(c), r),
     // reverse rotl(seed(c ^ 2), r); rows of 33 words put the four codes of one
-    // rotation in different banks
+    // rotation in different banks. Read before the prefixes are written to s_G.
     constexpr int kSeedRow = 33;
-    __shared__ uint32_t s_seed[2 * 4 * kSeedRow];
+    static_assert(2 * 4 * kSeedRow <= kGArr, "the seed table fits in s_G");
+    uint32_t *s_seed = s_G;
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     constexpr int nwave = kTileThreads / 64;
@@ -411,6 +416,9 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
     // positions of the leftmost / rightmost minimum of every block prefix and suffix
     uint16_t *s_PL = reinterpret_cast<uint16_t *>(s_G);
     uint16_t *s_SL = s_PL + kHashN, *s_PR = s_PL + 2 * kHashN, *s_SR = s_PL + 3 * kHashN;
+    static_assert(kGArr * 2 >= 4 * kHashN, "four u16 arrays fit in s_G");
+    // register windows use only s_PL (start list) and s_SL (minimizer positions)
+    uint16_t *s_K = WREG > 0 ? s_PR : s_Kb;
     // minimizer position per window overwrites s_SL in place: window hj reads
     // s_SL only at index hj before writing it
     uint16_t *s_mpos = s_SL;
@@ -532,14 +540,31 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
         __syncthreads();
 
         // canonical hash of the m-mer at p = t0-1+hp, hp in [0, NH)
-        for (int hp = tid; hp < NH; hp += kTileThreads) {
-            int li = hp + kLeft - 1, lm = li + m;
-            int pi = li + (li >> 4), pm = lm + (lm >> 4);
-            uint32_t gf = s_Gf[pm] ^ s_Gf[pi];
-            uint32_t gr = s_Gr[pm] ^ s_Gr[pi];
-            uint32_t fw = rotl32(gf, uint32_t(lm - 1) & 31);
-            uint32_t rc = rotr32(gr, uint32_t(li) & 31);
-            s_K[hp] = uint16_t((fw + rc) >> 16);
+        auto mmer_key = [&](int hp) {
+            const int li = hp + kLeft - 1, lm = li + m;
+            const int pi = li + (li >> 4), pm = lm + (lm >> 4);
+            const uint32_t gf = s_Gf[pm] ^ s_Gf[pi];
+            const uint32_t gr = s_Gr[pm] ^ s_Gr[pi];
+            const uint32_t fw = rotl32(gf, uint32_t(lm - 1) & 31);
+            const uint32_t rc = rotr32(gr, uint32_t(li) & 31);
+            return uint16_t((fw + rc) >> 16);
+        };
+        if constexpr (WREG > 0) {
+            // the keys overwrite part of the prefixes: held in registers until
+            // every thread has read its prefixes
+            constexpr int NHW = kTile + 2 * WREG - 1, KR = (NHW + kTileThreads - 1) / kTileThreads;
+            uint16_t key[KR];
+#pragma unroll
+            for (int q = 0; q < KR; ++q) {
+                const int hp = tid + q * kTileThreads;
+                key[q] = hp < NHW ? mmer_key(hp) : uint16_t(0);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < KR; ++q)
+                if (tid + q * kTileThreads < NHW) s_K[tid + q * kTileThreads] = key[q];
+        } else {
+            for (int hp = tid; hp < NH; hp += kTileThreads) s_K[hp] = mmer_key(hp);
         }
         __syncthreads();
 
