@@ -41,15 +41,34 @@ __device__ __forceinline__ uint64_t kmask(uint32_t len) {
 // first 64 bases) and calls f(j, canonical s-mer) for each of its s-mers in
 // position order (filter.rs:224-226, 423-426 with canonical_kmer 622-627).
 // The first s-mer is read directly from the words; the rest roll one base at a
-// time.
+// time. one_word (uniform) asserts 17 <= s <= 32 and nbases - s <= 16: every
+// base after the first s-mer then sits in one word, and the s-mer halves roll
+// as 32-bit words (the low half needs no mask).
 template <class F>
 __device__ __forceinline__ void for_each_smer(uint32_t A0, uint32_t A1, uint32_t A2, uint32_t A3,
-                                              uint32_t nbases, uint32_t s, F &&f) {
+                                              uint32_t nbases, uint32_t s, F &&f, bool one_word = false) {
     const uint64_t mask = kmask(s);
     const uint32_t top = 2 * (s - 1);
     uint64_t fw = ((uint64_t(A1) << 32) | A0) & mask;
     uint64_t rc = revcomp(fw, s);
     f(0u, fw < rc ? fw : rc);
+    if (one_word) {
+        // bases s .. s + 15, base s in bits 0-1
+        uint32_t cur = s < 32 ? __builtin_amdgcn_alignbit(A2, A1, 2 * (s & 15)) : A2;
+        uint32_t fl = uint32_t(fw), fh = uint32_t(fw >> 32), rl = uint32_t(rc), rh = uint32_t(rc >> 32);
+        const uint32_t mh = uint32_t(mask >> 32), th = top - 32;
+        for (uint32_t i = s; i < nbases; ++i) {
+            const uint32_t c = cur & 3;
+            cur >>= 2;
+            fl = __builtin_amdgcn_alignbit(fh, fl, 2);
+            fh = (fh >> 2) | (c << th);
+            rh = __builtin_amdgcn_alignbit(rh, rl, 30) & mh;
+            rl = (rl << 2) | (c ^ 2);
+            const uint64_t f64 = (uint64_t(fh) << 32) | fl, r64 = (uint64_t(rh) << 32) | rl;
+            f(i + 1 - s, f64 < r64 ? f64 : r64);
+        }
+        return;
+    }
     // word stream positioned at base s (s is uniform across the wave)
     if (s >= 16) A0 = A1, A1 = A2, A2 = A3;
     if (s >= 32) A0 = A1, A1 = A2;

@@ -77,6 +77,13 @@ struct DevGeo {
     uint64_t block_mask;
 };
 
+// Every base of a super-k-mer after its first s-mer fits in one 32-bit word
+// (a record has at most w k-mers, so at most w + k - 1 - s such bases), with
+// 17 <= s: the one-word s-mer stream applies (for_each_smer).
+__device__ __forceinline__ bool smer_tail_one_word(const DevGeo &g) {
+    return g.s >= 17 && g.w + g.k - 1 - g.s <= 16;
+}
+
 DevGeo dev_geo(const Geometry &g) {
     DevGeo d;
     d.k = g.k, d.m = g.m, d.s = g.s, d.nh = g.nh, d.w = g.w, d.win = g.win, d.bse = g.bse;
@@ -1378,6 +1385,7 @@ __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(
     unsigned long long *head = (unsigned long long *)&a.stats[kStWorkHead];
     // records per sub-batch: a record has at most w k-mers, so w + k - s s-mers
     const uint32_t rec_incr = (g.w + g.k - g.s) * g.nh;
+    const bool one_word = smer_tail_one_word(g);
     const uint32_t sub_recs = max(1u, 65535u / rec_incr);
     constexpr uint32_t per = 32 / BITS;
     const uint32_t nwords = bin_slots / per;
@@ -1413,16 +1421,22 @@ __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(
                 align_words(W, j, A0, A1, A2, A3);
                 if (lshift > 0) {  // the record's block lies in one half of the bin
                     const uint32_t wbase = lbase & (half_words - 1), inc = 1u << (16 * (lbase >> half_log));
-                    for_each_smer(A0, A1, A2, A3, nk + g.k - 1, g.s, [&](uint32_t, uint64_t smer) {
-                        for_each_address_any(smer, g.nh, amask,
-                                             [&](uint32_t ad) { atomicAdd(&s_cnt[wbase + ad], inc); });
-                    });
+                    for_each_smer(
+                        A0, A1, A2, A3, nk + g.k - 1, g.s,
+                        [&](uint32_t, uint64_t smer) {
+                            for_each_address_any(smer, g.nh, amask,
+                                                 [&](uint32_t ad) { atomicAdd(&s_cnt[wbase + ad], inc); });
+                        },
+                        one_word);
                 } else {  // one block per bin: its slots span both halves
-                    for_each_smer(A0, A1, A2, A3, nk + g.k - 1, g.s, [&](uint32_t, uint64_t smer) {
-                        for_each_address_any(smer, g.nh, amask, [&](uint32_t ad) {
-                            atomicAdd(&s_cnt[ad & (half_words - 1)], 1u << (16 * (ad >> half_log)));
-                        });
-                    });
+                    for_each_smer(
+                        A0, A1, A2, A3, nk + g.k - 1, g.s,
+                        [&](uint32_t, uint64_t smer) {
+                            for_each_address_any(smer, g.nh, amask, [&](uint32_t ad) {
+                                atomicAdd(&s_cnt[ad & (half_words - 1)], 1u << (16 * (ad >> half_log)));
+                            });
+                        },
+                        one_word);
                 }
             });
             __syncthreads();
@@ -1537,6 +1551,7 @@ __global__ __launch_bounds__(kQueryThreads) void k_bin_query(BinArgs a, DevGeo g
     const uint64_t nwork = a.stats[kStNWork];
     unsigned long long *head = (unsigned long long *)&a.stats[kStWorkHead];
     const T *lt = reinterpret_cast<const T *>(s_tab);
+    const bool one_word = smer_tail_one_word(g);
     uint64_t queried = 0, positive = 0, checksum = 0;
     for (;;) {
         if (tid == 0) s_item = atomicAdd(head, 1ull);
@@ -1564,19 +1579,22 @@ __global__ __launch_bounds__(kQueryThreads) void k_bin_query(BinArgs a, DevGeo g
             // per-record sums: at most 32 k-mers, so 8/16-bit estimates sum in 32 bits
             uint32_t rec_pos = 0, rec_sum = 0;
             uint64_t rec_sum64 = 0;
-            for_each_smer(A0, A1, A2, A3, nk + g.k - 1, g.s, [&](uint32_t j, uint64_t smer) {
-                const uint32_t est = min_at_addresses(smer, g.nh, amask, lt + lbase, Ctr<BITS>::kMax);
-                const uint32_t ab = EXACT ? wm.push_exact(est) : wm.push(est, g.win);
-                if (j + 1 >= g.win) {
-                    const uint32_t q = j + 1 - g.win;
-                    slot[q & (kV - 1)] = T(ab);
-                    if ((q & (kV - 1)) == kV - 1 && out)  // a full slot: kV estimates in one store
-                        store_run(reinterpret_cast<uint8_t *>(out + j0 + q - (kV - 1)), s_slot[tid], 16);
-                    rec_pos += ab > 0;
-                    if (BITS < 32) rec_sum += ab;
-                    else rec_sum64 += ab;
-                }
-            });
+            for_each_smer(
+                A0, A1, A2, A3, nk + g.k - 1, g.s,
+                [&](uint32_t j, uint64_t smer) {
+                    const uint32_t est = min_at_addresses(smer, g.nh, amask, lt + lbase, Ctr<BITS>::kMax);
+                    const uint32_t ab = EXACT ? wm.push_exact(est) : wm.push(est, g.win);
+                    if (j + 1 >= g.win) {
+                        const uint32_t q = j + 1 - g.win;
+                        slot[q & (kV - 1)] = T(ab);
+                        if ((q & (kV - 1)) == kV - 1 && out)  // a full slot: kV estimates in one store
+                            store_run(reinterpret_cast<uint8_t *>(out + j0 + q - (kV - 1)), s_slot[tid], 16);
+                        rec_pos += ab > 0;
+                        if (BITS < 32) rec_sum += ab;
+                        else rec_sum64 += ab;
+                    }
+                },
+                one_word);
             // the estimates left in the slot, in two or three stores
             const uint32_t rest = nk & (kV - 1);
             if (rest && out)

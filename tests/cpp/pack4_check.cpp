@@ -20,8 +20,12 @@ static uint32_t host_bitreverse32(uint32_t x) {  // used by the header's revcomp
     for (int i = 0; i < 32; ++i) r |= ((x >> i) & 1u) << (31 - i);
     return r;
 }
+static uint32_t host_alignbit(uint32_t hi, uint32_t lo, uint32_t sh) {  // used by the header's streams
+    return uint32_t(((uint64_t(hi) << 32) | lo) >> (sh & 31));
+}
 #define __builtin_amdgcn_perm host_perm
 #define __builtin_bitreverse32 host_bitreverse32
+#define __builtin_amdgcn_alignbit host_alignbit
 #define __device__
 #define __forceinline__ inline
 #include "scb_bases.h"

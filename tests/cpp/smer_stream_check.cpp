@@ -13,8 +13,13 @@ static uint32_t host_bitreverse32(uint32_t x) {
     return r;
 }
 static uint32_t host_perm(uint32_t, uint32_t, uint32_t) { return 0; }  // not used here
+// v_alignbit_b32: bits [sh, sh + 32) of hi:lo
+static uint32_t host_alignbit(uint32_t hi, uint32_t lo, uint32_t sh) {
+    return uint32_t(((uint64_t(hi) << 32) | lo) >> (sh & 31));
+}
 #define __builtin_bitreverse32 host_bitreverse32
 #define __builtin_amdgcn_perm host_perm
+#define __builtin_amdgcn_alignbit host_alignbit
 #define __device__
 #define __forceinline__ inline
 #include "scb_bases.h"
@@ -31,7 +36,7 @@ static uint64_t canon(const std::vector<uint32_t> &b, uint32_t a, uint32_t len) 
 
 int main() {
     std::mt19937 rng(11);
-    long bad = 0, checked = 0;
+    long bad = 0, checked = 0, checked_one_word = 0;
     for (uint32_t k = 1; k <= 32; ++k) {
         for (uint32_t s = 1; s <= k; ++s) {
             for (int rep = 0; rep < 40; ++rep) {
@@ -47,6 +52,19 @@ int main() {
                     if (j != sm.size()) ++bad;
                     sm.push_back(x);
                 });
+                if (s >= 17 && nbases - s <= 16) {  // the one-word stream gives the same s-mers
+                    uint32_t n1 = 0;
+                    for_each_smer(
+                        A[0], A[1], A[2], A[3], nbases, s,
+                        [&](uint32_t j, uint64_t x) {
+                            if (j != n1++ || x != sm[j]) {
+                                if (bad++ < 5) printf("k %u s %u j %u: one-word s-mer mismatch\n", k, s, j);
+                            }
+                        },
+                        true);
+                    if (n1 != sm.size()) ++bad;
+                    ++checked_one_word;
+                }
                 for_each_smer_kmer(A[0], A[1], A[2], A[3], nbases, s, k,
                                    [&](uint32_t j, uint64_t smer, uint64_t kmer) {
                                        ++checked;
@@ -61,6 +79,7 @@ int main() {
             }
         }
     }
-    if (!bad) printf("s-mer and k-mer streams exact (%ld positions)\n", checked);
+    if (!checked_one_word) ++bad;
+    if (!bad) printf("s-mer and k-mer streams exact (%ld positions, %ld one-word records)\n", checked, checked_one_word);
     return bad ? 1 : 0;
 }
