@@ -93,15 +93,40 @@ __device__ __forceinline__ void for_each_smer(uint32_t A0, uint32_t A1, uint32_t
 // from j = k - s on (j + 1 >= k - s + 1); before that it is not meaningful.
 // It starts as the first s-mer placed at the top of a k-base word, so the
 // bases shifted in after it complete it.
+// one_word as for for_each_smer (then k >= s >= 17 and both rolls use 32-bit halves).
 template <class F>
 __device__ __forceinline__ void for_each_smer_kmer(uint32_t A0, uint32_t A1, uint32_t A2, uint32_t A3,
-                                                   uint32_t nbases, uint32_t s, uint32_t k, F &&f) {
+                                                   uint32_t nbases, uint32_t s, uint32_t k, F &&f,
+                                                   bool one_word = false) {
     const uint64_t mask = kmask(s), kmk = kmask(k);
     const uint32_t top = 2 * (s - 1), ktop = 2 * (k - 1);
     uint64_t fw = ((uint64_t(A1) << 32) | A0) & mask;
     uint64_t rc = revcomp(fw, s);
     uint64_t fk = fw << (2 * (k - s)), rk = rc;
     f(0u, fw < rc ? fw : rc, fk < rk ? fk : rk);
+    if (one_word) {
+        uint32_t cur = s < 32 ? __builtin_amdgcn_alignbit(A2, A1, 2 * (s & 15)) : A2;
+        uint32_t fl = uint32_t(fw), fh = uint32_t(fw >> 32), rl = uint32_t(rc), rh = uint32_t(rc >> 32);
+        uint32_t kl = uint32_t(fk), kh = uint32_t(fk >> 32), ql = uint32_t(rk), qh = uint32_t(rk >> 32);
+        const uint32_t mh = uint32_t(mask >> 32), th = top - 32;
+        const uint32_t kmh = uint32_t(kmk >> 32), kth = ktop - 32;
+        for (uint32_t i = s; i < nbases; ++i) {
+            const uint32_t c = cur & 3;
+            cur >>= 2;
+            fl = __builtin_amdgcn_alignbit(fh, fl, 2);
+            fh = (fh >> 2) | (c << th);
+            rh = __builtin_amdgcn_alignbit(rh, rl, 30) & mh;
+            rl = (rl << 2) | (c ^ 2);
+            kl = __builtin_amdgcn_alignbit(kh, kl, 2);
+            kh = (kh >> 2) | (c << kth);
+            qh = __builtin_amdgcn_alignbit(qh, ql, 30) & kmh;
+            ql = (ql << 2) | (c ^ 2);
+            const uint64_t f64 = (uint64_t(fh) << 32) | fl, r64 = (uint64_t(rh) << 32) | rl;
+            const uint64_t k64 = (uint64_t(kh) << 32) | kl, q64 = (uint64_t(qh) << 32) | ql;
+            f(i + 1 - s, f64 < r64 ? f64 : r64, k64 < q64 ? k64 : q64);
+        }
+        return;
+    }
     if (s >= 16) A0 = A1, A1 = A2, A2 = A3;
     if (s >= 32) A0 = A1, A1 = A2;
     uint32_t cur = A0 >> (2 * (s & 15));

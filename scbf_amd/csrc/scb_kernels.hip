@@ -1760,29 +1760,31 @@ __global__ __launch_bounds__(kSpectrumThreads) void k_bin_spectrum(SpecArgs sa, 
             align_words(W, rec & 0xFFFFFFFFull, A0, A1, A2, A3);
             WinMin<WINC> wm;
             // the canonical k-mer key rolls along with the s-mers
-            for_each_smer_kmer(A0, A1, A2, A3, nk + g.k - 1, g.s, g.k, [&](uint32_t j, uint64_t smer, uint64_t key) {
-                uint32_t est = Ctr<BITS>::kMax;
-                if (g.bse <= 15) {
-                    est = min_at_addresses(smer, g.nh, amask, tbase + lbase, Ctr<BITS>::kMax);
-                } else {
-                    uint64_t h = xs64(smer);
-                    for (uint32_t t = 0; t < g.nh; ++t) {
-                        uint32_t v = tbase[lbase + (uint32_t(h) & amask)];
-                        est = v < est ? v : est;
-                        if (t + 1 < g.nh) h = xs64(h);
+            for_each_smer_kmer(
+                A0, A1, A2, A3, nk + g.k - 1, g.s, g.k,
+                [&](uint32_t j, uint64_t smer, uint64_t key) {
+                    uint32_t est = Ctr<BITS>::kMax;
+                    if (g.bse <= 15) {
+                        est = min_at_addresses(smer, g.nh, amask, tbase + lbase, Ctr<BITS>::kMax);
+                    } else {
+                        uint64_t h = xs64(smer);
+                        for (uint32_t t = 0; t < g.nh; ++t) {
+                            uint32_t v = tbase[lbase + (uint32_t(h) & amask)];
+                            est = v < est ? v : est;
+                            if (t + 1 < g.nh) h = xs64(h);
+                        }
                     }
-                }
-                uint32_t ab = wm.push(est, g.win);
-                if (j + 1 >= g.win) {
-                    bool fresh = global ? set_insert(sa.gset, sa.glog2, key)
-                                        : set_insert(s_set, 13, key);
-                    ++queried;
-                    if (fresh) {
-                        ++distinct;
-                        spec_hist_add<BITS>(sa, s_hist, ab);
+                    uint32_t ab = wm.push(est, g.win);
+                    if (j + 1 >= g.win) {
+                        bool fresh = global ? set_insert(sa.gset, sa.glog2, key) : set_insert(s_set, 13, key);
+                        ++queried;
+                        if (fresh) {
+                            ++distinct;
+                            spec_hist_add<BITS>(sa, s_hist, ab);
+                        }
                     }
-                }
-            });
+                },
+                smer_tail_one_word(g));
         });
         __syncthreads();
     }

@@ -76,6 +76,17 @@ int main() {
                                        }
                                    });
                 if (sm.size() != nbases - s + 1) ++bad;
+                if (s >= 17 && nbases - s <= 16) {  // one-word s-mer and k-mer streams
+                    for_each_smer_kmer(
+                        A[0], A[1], A[2], A[3], nbases, s, k,
+                        [&](uint32_t j, uint64_t smer, uint64_t kmer) {
+                            if (smer != canon(b, j, s) ||
+                                (j + 1 >= k - s + 1 && kmer != canon(b, j + s - k, k))) {
+                                if (bad++ < 5) printf("k %u s %u j %u: one-word k-mer stream mismatch\n", k, s, j);
+                            }
+                        },
+                        true);
+                }
             }
         }
     }
