@@ -16,12 +16,14 @@
 // Batched pipeline (one stream, no host sync inside):
 //   k_scan      per 2048-window tile: ASCII -> codes, ACGT-run and read-boundary
 //               masks, canonical ntHash of every m-mer, minimizer of every k-mer
-//               window, super-k-mer cut, block hash -> rec + bin histogram
+//               window, super-k-mer cut, block hash -> rec (appended to one of
+//               64 segments)
+//   k_seg_hist / k_small_scan / k_partition  rec -> records grouped by bin
+//               (tile-aggregated counting sort: a coarse pass, then a fine pass)
 //   k_plan_sums / k_plan_write  bin offsets + work list (hot bins split)
-//   k_partition rec -> records grouped by bin (tile-aggregated counting sort,
-//               one or two passes)
-//   k_bin_insert / k_bin_query  persistent work queue, one bin per item, counters
-//               staged in LDS, written back once (insert) or read once (query)
+//   k_bin_insert / k_bin_query / k_bin_spectrum  persistent work queue, one bin
+//               per item, counters staged in LDS, written back once (insert) or
+//               read once (query, spectrum)
 // Plain pipeline: k_scan alone, one thread per super-k-mer, global atomics/loads.
 
 #include <hip/hip_runtime.h>
