@@ -1228,10 +1228,13 @@ constexpr int kInsertThreads = 1024, kQueryThreads = 512, kSpectrumThreads = 102
 // length no lane idles. Lanes of equal length are matched with five ballots
 // and counted by one leader each, so the LDS atomics never collide. R rounds
 // of nthr threads cover the largest item; all R keys are loaded up front.
+// With s_lead given, only the records i with s_lead[i] == i are ordered, and
+// their number is written to *s_total.
 template <int R>
 __device__ __forceinline__ void order_by_length(const uint64_t *sorted, uint32_t rb, uint32_t n, uint16_t *s_perm,
                                                 uint32_t *s_cls, uint32_t tid, uint32_t nthr,
-                                                uint32_t *s_smers = nullptr, uint32_t k_minus_s = 0) {
+                                                uint32_t *s_smers = nullptr, uint32_t k_minus_s = 0,
+                                                const uint16_t *s_lead = nullptr, uint32_t *s_total = nullptr) {
     const uint32_t lane = tid & 63;
     const uint64_t lt = (uint64_t(1) << lane) - 1;
     if (tid < 32) s_cls[tid] = 0;
@@ -1248,7 +1251,7 @@ __device__ __forceinline__ void order_by_length(const uint64_t *sorted, uint32_t
         for (int r = 0; r < R; ++r) {
             if (uint32_t(r) * nthr >= n) break;  // uniform
             const uint32_t i = r * nthr + tid;
-            const bool act = i < n;
+            const bool act = i < n && (!s_lead || s_lead[i] == i);
             uint64_t m = __ballot(act);
 #pragma unroll
             for (int bit = 0; bit < 5; ++bit) {
@@ -1273,6 +1276,7 @@ __device__ __forceinline__ void order_by_length(const uint64_t *sorted, uint32_t
                     smers += v * (32 - c + k_minus_s);  // class c holds records of 32 - c k-mers
                 }
                 if (s_smers) *s_smers = smers;
+                if (s_total) *s_total = run;
             }
             __syncthreads();
         }
@@ -1308,22 +1312,28 @@ __device__ __forceinline__ void align_words(const uint32_t (&W)[5], uint64_t b, 
 // its packed words are in flight while the current one is processed: a record
 // and its bases are two dependent global loads, which would otherwise stall
 // every round. Indices past the end are clamped, so every load is in bounds.
-template <class F>
+// With IDX, f also receives the record's index in the item (its s_perm entry).
+template <bool IDX = false, class F>
 __device__ __forceinline__ void walk_records(const uint64_t *sorted, const uint32_t *pk, uint32_t k, uint32_t rb, uint32_t n,
                                              const uint16_t *s_perm, uint32_t tid, uint32_t nthr, F &&f) {
     if (tid >= n) return;
     const uint32_t last = n - 1;
-    uint64_t rc = sorted[rb + s_perm[tid]];
-    uint64_t rn = sorted[rb + s_perm[min(tid + nthr, last)]];
+    uint32_t pc = s_perm[tid], pn = s_perm[min(tid + nthr, last)];
+    uint64_t rc = sorted[rb + pc];
+    uint64_t rn = sorted[rb + pn];
     uint32_t W[5];
     load_words5(pk, rc, k, W);
     for (uint32_t i = tid; i < n; i += nthr) {
         uint32_t V[5];
         load_words5(pk, rn, k, V);
-        const uint64_t rnn = sorted[rb + s_perm[min(i + 2 * nthr, last)]];
-        f(rc, W);
+        const uint32_t pnn = s_perm[min(i + 2 * nthr, last)];
+        const uint64_t rnn = sorted[rb + pnn];
+        if constexpr (IDX) f(rc, W, pc);
+        else f(rc, W);
         rc = rn;
         rn = rnn;
+        pc = pn;
+        pn = pnn;
 #pragma unroll
         for (int q = 0; q < 5; ++q) W[q] = V[q];
     }
@@ -1333,6 +1343,7 @@ struct BinArgs {
     const uint64_t *sorted;
     const uint32_t *bin_off, *bin_count;
     const uint64_t *work;
+    const uint32_t *bin_kmers;  // k-mers per bin (insert: selects the items counted by distinct record)
     uint64_t *stats;
     const uint32_t *pk;
     void *table;
@@ -1360,6 +1371,82 @@ __global__ __launch_bounds__(256) void k_zero_cold(const uint32_t *bin_count, ui
     }
 }
 
+// Insert work items hold at most this many records (two per thread), so that
+// the distinct-record search below fits in the LDS of the bin's counts.
+constexpr uint32_t kInsertItemRecords = 2 * kInsertThreads;
+constexpr uint32_t kRecDedupSlots = 2 * kInsertItemRecords;  // open addressing, load <= 1/2
+// 16-byte keys and the high words of the records, then the table (words)
+constexpr uint32_t kRecDedupWords = 5 * kInsertItemRecords + kRecDedupSlots;
+static_assert(kInsertItemRecords <= 4096, "table entries hold 12-bit record indices");
+
+// Identical records of a work item: the same bases, k-mer count and block
+// (the record's high word), hence the same s-mers at the same addresses.
+// s_lead[i] becomes the first record of i's key to claim a table slot (i for
+// that record, the leader), and s_mult the number of records of each leader
+// (16-bit halves of words, record l in half l & 1 of word l >> 1). Identical
+// keys walk the same probe sequence and a slot is never released, so each key
+// has exactly one leader; at most n of the 2n slots are taken, so every probe
+// ends. s_ws (kRecDedupWords words) is free again after the closing barrier.
+template <int R>
+__device__ __forceinline__ void find_duplicates(const uint64_t *sorted, const uint32_t *pk, uint32_t rb, uint32_t n,
+                                                uint32_t k, uint32_t *s_ws, uint16_t *s_lead, uint32_t *s_mult,
+                                                uint32_t tid, uint32_t nthr) {
+    uint4 *s_key = reinterpret_cast<uint4 *>(s_ws);
+    uint32_t *s_hi = s_ws + 4 * kInsertItemRecords;
+    uint32_t *s_ht = s_hi + kInsertItemRecords;
+    for (uint32_t i = tid; i < kRecDedupSlots; i += nthr) s_ht[i] = 0;
+    for (uint32_t i = tid; i < kInsertItemRecords / 2; i += nthr) s_mult[i] = 0;
+    uint32_t h[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t i = r * nthr + tid;
+        const uint64_t rec = sorted[rb + min(i, n - 1)];  // clamped: in bounds
+        uint32_t W[5];
+        load_words5(pk, rec, k, W);
+        uint32_t A[4];
+        align_words(W, rec & 0xFFFFFFFFull, A[0], A[1], A[2], A[3]);
+        const uint32_t hi = uint32_t(rec >> 32);
+        const int bits = 2 * int((hi & 31) + k);  // 2 bits for each of the nk + k - 1 bases
+        uint32_t x = hi * 0x9E3779B1u;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int b = bits - 32 * t;
+            A[t] &= b >= 32 ? ~0u : b <= 0 ? 0u : (1u << b) - 1;
+            x = (x ^ A[t]) * 0x85EBCA77u;
+            x ^= x >> 15;
+        }
+        h[r] = x;
+        if (i < n) {
+            s_key[i] = make_uint4(A[0], A[1], A[2], A[3]);
+            s_hi[i] = hi;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t i = r * nthr + tid;
+        if (i >= n) continue;
+        const uint32_t entry = 0x80000000u | (h[r] & 0x7FFFF000u) | i;  // valid | tag | index
+        uint32_t slot = h[r] & (kRecDedupSlots - 1), lead = i;
+        for (;;) {
+            const uint32_t old = atomicCAS(&s_ht[slot], 0u, entry);
+            if (old == 0) break;
+            if (((old ^ entry) & 0x7FFFF000u) == 0) {
+                const uint32_t li = old & 0xFFFu;
+                const uint4 p = s_key[li], q = s_key[i];
+                if (s_hi[li] == s_hi[i] && p.x == q.x && p.y == q.y && p.z == q.z && p.w == q.w) {
+                    lead = li;
+                    break;
+                }
+            }
+            slot = (slot + 1) & (kRecDedupSlots - 1);
+        }
+        s_lead[i] = uint16_t(lead);
+        atomicAdd(&s_mult[lead >> 1], 1u << (16 * (lead & 1)));
+    }
+    __syncthreads();
+}
+
 // filter.rs:199-237 for every super-k-mer of one bin: n_hashes saturating +1
 // per s-mer, counted in LDS (u32 per slot) and merged once as
 // min(MAX, old + count), which equals applying the +1s one by one (saturating
@@ -1376,9 +1463,10 @@ __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(
                                                                                                        DevGeo g) {
     extern __shared__ __align__(16) uint32_t s_cnt[];
     __shared__ uint64_t s_item;
-    __shared__ uint16_t s_perm[kMaxItemRecords];
+    __shared__ uint16_t s_perm[kInsertItemRecords], s_lead[kInsertItemRecords];
+    __shared__ uint32_t s_mult[kInsertItemRecords / 2];
     __shared__ uint32_t s_cls[32];
-    __shared__ uint32_t s_smers;
+    __shared__ uint32_t s_smers, s_nwalk;
     const uint32_t tid = threadIdx.x, nthr = blockDim.x;
     const uint32_t bin_slots = 1u << a.bin_log;
     const uint32_t lshift = a.bin_log - g.bse;
@@ -1407,22 +1495,41 @@ __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(
         uint32_t re = min(a.bin_off[bin] + cnt, rb + a.per_item);
         const uint32_t n = re - rb;
         const bool multi = cnt > a.per_item;
-        zero_lds(s_cnt, bin_slots / 2, tid, nthr);
-        order_by_length<kMaxItemRecords / kInsertThreads>(a.sorted, rb, n, s_perm, s_cls, tid, nthr, &s_smers,
-                                                          g.k - g.s);  // (synchronizes)
-        const bool one = uint64_t(s_smers) * g.nh <= 65535u;
-        const uint32_t step = one ? n : sub_recs;
+        // A bin that is one work item and makes at most 65535 increments (its
+        // k-mers plus k - s per record, times n_hashes) is counted by distinct
+        // record: each leader adds its multiplicity c at every increment, which
+        // equals c single increments (and stays within one 16-bit pass).
+        const bool dedup = !multi && n > 1 && a.bin_kmers &&
+                           (uint64_t(a.bin_kmers[bin]) + uint64_t(g.k - g.s) * n) * g.nh <= 65535u;
+        uint32_t nwalk = n;
+        bool one = true;
+        if (dedup) {
+            find_duplicates<kInsertItemRecords / kInsertThreads>(a.sorted, a.pk, rb, n, g.k, s_cnt, s_lead, s_mult,
+                                                                 tid, nthr);  // (synchronizes)
+            zero_lds(s_cnt, bin_slots / 2, tid, nthr);
+            order_by_length<kInsertItemRecords / kInsertThreads>(a.sorted, rb, n, s_perm, s_cls, tid, nthr, nullptr,
+                                                                 0, s_lead, &s_nwalk);  // (synchronizes)
+            nwalk = s_nwalk;
+        } else {
+            zero_lds(s_cnt, bin_slots / 2, tid, nthr);
+            order_by_length<kInsertItemRecords / kInsertThreads>(a.sorted, rb, n, s_perm, s_cls, tid, nthr,
+                                                                 &s_smers, g.k - g.s);  // (synchronizes)
+            one = uint64_t(s_smers) * g.nh <= 65535u;
+        }
+        const uint32_t step = one ? nwalk : sub_recs;
         uint32_t *tw = reinterpret_cast<uint32_t *>(a.table) + bin * uint64_t(nwords);
-        for (uint32_t r0 = 0; r0 < n; r0 += step) {
-            const uint32_t m = min(step, n - r0);
-            walk_records(a.sorted, a.pk, g.k, rb, m, s_perm + r0, tid, nthr, [&](uint64_t rec, const uint32_t (&W)[5]) {
+        for (uint32_t r0 = 0; r0 < nwalk; r0 += step) {
+            const uint32_t m = min(step, nwalk - r0);
+            walk_records<true>(a.sorted, a.pk, g.k, rb, m, s_perm + r0, tid, nthr, [&](uint64_t rec, const uint32_t (&W)[5], uint32_t ri) {
                 const uint64_t j = rec & 0xFFFFFFFFull;
                 const uint32_t nk = uint32_t((rec >> 32) & 31) + 1;
                 const uint32_t lbase = uint32_t((rec >> 37) & ((1u << lshift) - 1)) << g.bse;
+                // increments per s-mer address: the number of identical records
+                const uint32_t mult = dedup ? (s_mult[ri >> 1] >> (16 * (ri & 1))) & 0xFFFFu : 1u;
                 uint32_t A0, A1, A2, A3;
                 align_words(W, j, A0, A1, A2, A3);
                 if (lshift > 0) {  // the record's block lies in one half of the bin
-                    const uint32_t wbase = lbase & (half_words - 1), inc = 1u << (16 * (lbase >> half_log));
+                    const uint32_t wbase = lbase & (half_words - 1), inc = mult << (16 * (lbase >> half_log));
                     for_each_smer(
                         A0, A1, A2, A3, nk + g.k - 1, g.s,
                         [&](uint32_t, uint64_t smer) {
@@ -1435,7 +1542,7 @@ __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(
                         A0, A1, A2, A3, nk + g.k - 1, g.s,
                         [&](uint32_t, uint64_t smer) {
                             for_each_address_any(smer, g.nh, amask, [&](uint32_t ad) {
-                                atomicAdd(&s_cnt[ad & (half_words - 1)], 1u << (16 * (ad >> half_log)));
+                                atomicAdd(&s_cnt[ad & (half_words - 1)], mult << (16 * (ad >> half_log)));
                             });
                         },
                         one_word);
@@ -1519,7 +1626,7 @@ __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(
                 }
             }
             __syncthreads();
-            if (r0 + step < n) {
+            if (r0 + step < nwalk) {
                 zero_lds(s_cnt, bin_slots / 2, tid, nthr);
                 __syncthreads();
             }
@@ -2398,7 +2505,11 @@ Status Engine::insert(const DeviceBatch &b, Method method, InsertResult &r) {
     } else {
         uint64_t nbins = 0;
         uint32_t bin_log = tune_.insert_bin_log;
-        s = batched_front(L, b, bin_log, std::max<uint32_t>(1, std::min<uint32_t>(tune_.max_records_per_item, kMaxItemRecords)), nullptr, nbins);
+        const uint32_t per_item =
+            std::max<uint32_t>(1, std::min<uint32_t>(tune_.max_records_per_item, kInsertItemRecords));
+        // k-mers per bin select the bins counted by distinct record
+        SCB_HIP(ws->spec_bin_kmers.reserve(uint64_t(1) << (geo_.cse - bin_log)));
+        s = batched_front(L, b, bin_log, per_item, nullptr, nbins, ws->spec_bin_kmers.p);
         if (s.good()) {
             BinArgs a{};
             a.sorted = ws->final_sorted;
@@ -2409,7 +2520,8 @@ Status Engine::insert(const DeviceBatch &b, Method method, InsertResult &r) {
             a.pk = ws->pk.p;
             a.table = d_table_;
             a.bin_log = bin_log;
-            a.per_item = std::max<uint32_t>(1, std::min<uint32_t>(tune_.max_records_per_item, kMaxItemRecords));
+            a.per_item = per_item;
+            a.bin_kmers = ws->spec_bin_kmers.p;
             a.fresh = fresh;
             if (fresh) {
                 const uint32_t words_per_bin = uint32_t(((uint64_t(1) << bin_log) * geo_.bytes) / 4);
@@ -2417,7 +2529,8 @@ Status Engine::insert(const DeviceBatch &b, Method method, InsertResult &r) {
                                    ws->bin_count.p, nbins, a.per_item, reinterpret_cast<uint32_t *>(d_table_),
                                    words_per_bin);
             }
-            size_t lds = (size_t(1) << bin_log) * 2;  // 16-bit counts
+            // 16-bit counts; the distinct-record search uses the same LDS first
+            size_t lds = std::max<size_t>((size_t(1) << bin_log) * 2, size_t(kRecDedupWords) * 4);
             const int thr = kInsertThreads;
 #define SCB_INS(BITS)                                                                           \
     do {                                                                                        \
