@@ -26,8 +26,8 @@ GEOMETRIES = [
     ("k31m21s25_u16_bse15", 31, 21, 25, 4, 16, 20, 15),  # 15-bit slot addresses (largest staged block)
     ("k15m9s11_u32_h1", 15, 9, 11, 1, 32, 18, 12),  # one hash per s-mer
     ("k31m21s30_u16_win2", 31, 21, 30, 3, 16, 20, 9),  # s-mers of 60 bits, fimpera window 2
-    # one 2^15-slot bin holds every record: work items split at 4096 records
-    # merge into the table with atomics, sub-batches of ~960 records each
+    # one 2^15-slot bin holds every record: work items split at 2048 records
+    # (insert) merge into the table with atomics, sub-batches of ~960 records each
     ("k31m21s25_u16_onebin", 31, 21, 25, 4, 16, 15, 9),
     # tiny tables (config.rs:76 admits any bse < cse): one bin smaller than the
     # merge vectors, 16-bit counts saturating across insert sub-batches
@@ -125,6 +125,38 @@ def test_counter_histogram(gpu, oracle, bits):
         oracle.insert_records(of, bases[:5000], offsets[offsets <= 5000])
     idx = b.into_index()
     assert idx.counter_histogram() == of.counter_histogram()
+
+
+@pytest.mark.parametrize("bits", [8, 16])
+def test_repeated_records_parity(gpu, oracle, bits):
+    """One read 300 times and others 3 times in one call: identical records are
+    counted once with their multiplicity (bins that are one work item), so large
+    multiplicities, both halves of the 16-bit LDS counts and 8-bit saturation are
+    checked against the oracle's one-by-one increments."""
+    scb = gpu
+    k, m, s, h, cse, bse = 31, 21, 25, 4, 20, 9  # 32 bins of ~300 records: each one work item
+    g = synth.genome(3000, 9)
+    rb, ro = synth.fixed_reads(g, 30, 150, 9)
+    parts = [np.tile(rb[ro[0]:ro[1]], 300)] + [np.tile(rb[ro[i]:ro[i + 1]], 3) for i in range(1, 30)]
+    lens = [150] * 300 + [150] * (3 * 29)
+    bases = np.concatenate(parts)
+    offsets = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    assert offsets[-1] == bases.size
+    of = oracle.Filter(oracle.make_cfg(k, m, s, h, bits, cse, bse))
+    ok_kmers, _ = oracle.insert_records(of, bases, offsets)
+    b = scb.SuperCountingBloomBuilder(_cfg(scb, k, m, s, h, bits, cse, bse))
+    rep = b.add_sequences((bases, offsets))
+    assert rep.inserted_kmers == ok_kmers
+    got = b.counters().astype(np.uint32)
+    want = of.counters()
+    assert np.array_equal(got, want), f"counters differ at {np.flatnonzero(got != want)[:10]}"
+    if bits == 8:
+        assert want.max() == 255  # the 300 copies saturate
+    idx = b.into_index()
+    vals, valid, _ = idx.estimate_sequences(bases, offsets)
+    wv, wok = oracle.estimate_records(of, bases, offsets, k)
+    assert np.array_equal(valid, wok)
+    assert np.array_equal(vals[wok], wv[wok])
 
 
 def test_c1_small_parity(gpu, oracle):
