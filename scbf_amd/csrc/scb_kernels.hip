@@ -1229,21 +1229,19 @@ constexpr int kInsertThreads = 1024, kQueryThreads = 512, kSpectrumThreads = 102
 // and counted by one leader each, so the LDS atomics never collide. R rounds
 // of nthr threads cover the largest item; all R keys are loaded up front.
 // With s_lead given, only the records i with s_lead[i] == i are ordered, and
-// their number is written to *s_total.
+// their number is written to *s_total. order_by_hi takes the records' high
+// words (hi[r] of record r * nthr + tid; any value past n) from registers.
 template <int R>
-__device__ __forceinline__ void order_by_length(const uint64_t *sorted, uint32_t rb, uint32_t n, uint16_t *s_perm,
-                                                uint32_t *s_cls, uint32_t tid, uint32_t nthr,
-                                                uint32_t *s_smers = nullptr, uint32_t k_minus_s = 0,
-                                                const uint16_t *s_lead = nullptr, uint32_t *s_total = nullptr) {
+__device__ __forceinline__ void order_by_hi(const uint32_t (&hi)[R], uint32_t n, uint16_t *s_perm, uint32_t *s_cls,
+                                            uint32_t tid, uint32_t nthr, uint32_t *s_smers = nullptr,
+                                            uint32_t k_minus_s = 0, const uint16_t *s_lead = nullptr,
+                                            uint32_t *s_total = nullptr) {
     const uint32_t lane = tid & 63;
     const uint64_t lt = (uint64_t(1) << lane) - 1;
     if (tid < 32) s_cls[tid] = 0;
     uint32_t key[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {  // longest first; clamped index keeps the load in bounds
-        const uint32_t i = min(r * nthr + tid, n ? n - 1 : 0);
-        key[r] = n ? 31 - uint32_t((sorted[rb + i] >> 32) & 31) : 0;
-    }
+    for (int r = 0; r < R; ++r) key[r] = n ? 31 - (hi[r] & 31) : 0;  // longest first
     __syncthreads();
 #pragma unroll
     for (int pass = 0; pass < 2; ++pass) {
@@ -1281,6 +1279,17 @@ __device__ __forceinline__ void order_by_length(const uint64_t *sorted, uint32_t
             __syncthreads();
         }
     }
+}
+
+template <int R>
+__device__ __forceinline__ void order_by_length(const uint64_t *sorted, uint32_t rb, uint32_t n, uint16_t *s_perm,
+                                                uint32_t *s_cls, uint32_t tid, uint32_t nthr,
+                                                uint32_t *s_smers = nullptr, uint32_t k_minus_s = 0) {
+    uint32_t hi[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r)  // clamped index keeps the load in bounds
+        hi[r] = n ? uint32_t(sorted[rb + min(r * nthr + tid, n - 1)] >> 32) : 0u;
+    order_by_hi<R>(hi, n, s_perm, s_cls, tid, nthr, s_smers, k_minus_s);
 }
 
 // The five packed words holding bases [b & ~15, (b & ~15) + 80) and the four
@@ -1390,7 +1399,7 @@ static_assert(kInsertItemRecords <= 4096, "table entries hold 12-bit record indi
 template <int R>
 __device__ __forceinline__ void find_duplicates(const uint64_t *sorted, const uint32_t *pk, uint32_t rb, uint32_t n,
                                                 uint32_t k, uint32_t *s_ws, uint16_t *s_lead, uint32_t *s_mult,
-                                                uint32_t tid, uint32_t nthr) {
+                                                uint32_t tid, uint32_t nthr, uint32_t (&hi_out)[R]) {
     uint4 *s_key = reinterpret_cast<uint4 *>(s_ws);
     uint32_t *s_hi = s_ws + 4 * kInsertItemRecords;
     uint32_t *s_ht = s_hi + kInsertItemRecords;
@@ -1406,6 +1415,7 @@ __device__ __forceinline__ void find_duplicates(const uint64_t *sorted, const ui
         uint32_t A[4];
         align_words(W, rec & 0xFFFFFFFFull, A[0], A[1], A[2], A[3]);
         const uint32_t hi = uint32_t(rec >> 32);
+        hi_out[r] = hi;
         const int bits = 2 * int((hi & 31) + k);  // 2 bits for each of the nk + k - 1 bases
         uint32_t x = hi * 0x9E3779B1u;
 #pragma unroll
@@ -1504,11 +1514,11 @@ __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(
         uint32_t nwalk = n;
         bool one = true;
         if (dedup) {
-            find_duplicates<kInsertItemRecords / kInsertThreads>(a.sorted, a.pk, rb, n, g.k, s_cnt, s_lead, s_mult,
-                                                                 tid, nthr);  // (synchronizes)
+            constexpr int R = kInsertItemRecords / kInsertThreads;
+            uint32_t hi[R];
+            find_duplicates<R>(a.sorted, a.pk, rb, n, g.k, s_cnt, s_lead, s_mult, tid, nthr, hi);  // (synchronizes)
             zero_lds(s_cnt, bin_slots / 2, tid, nthr);
-            order_by_length<kInsertItemRecords / kInsertThreads>(a.sorted, rb, n, s_perm, s_cls, tid, nthr, nullptr,
-                                                                 0, s_lead, &s_nwalk);  // (synchronizes)
+            order_by_hi<R>(hi, n, s_perm, s_cls, tid, nthr, nullptr, 0, s_lead, &s_nwalk);  // (synchronizes)
             nwalk = s_nwalk;
         } else {
             zero_lds(s_cnt, bin_slots / 2, tid, nthr);
