@@ -272,17 +272,20 @@ def test_weighted_mixture_u32_parity(gpu, oracle, method):
     _parity_roundtrip(scb, oracle, (21, 15, 17, 3, 32, 22, 10), ib, io, ib, io, method)
 
 
+@pytest.mark.parametrize("reads", [4000, 2500], ids=["split_items", "one_item"])
 @pytest.mark.parametrize("bits", [8, 16, 32])
-def test_work_items_counted_in_sub_batches(gpu, oracle, bits):
-    """About 2,600 records per 2^15-slot bin: each work item makes more than
-    65,535 counter increments, so it is counted and merged in sub-batches (the
-    16-bit LDS counts cannot overflow), including the first insert into a new
-    table and a second insert on top of it."""
+def test_work_items_counted_in_sub_batches(gpu, oracle, bits, reads):
+    """About 2,600 (4,000 reads) or 1,600 (2,500 reads) records per 2^15-slot
+    bin: each work item makes more than 65,535 counter increments, so it is
+    counted and merged in sub-batches (the 16-bit LDS counts cannot overflow),
+    including the first insert into a new table and a second insert on top of
+    it. With 2,500 reads every bin is one work item (at most 2048 records) that
+    is too large for distinct-record counting in one pass."""
     scb = gpu
     k, m, s, h, cse, bse = 31, 21, 25, 4, 20, 9
     g = synth.genome(400_000, 81)
-    b1, o1 = synth.fixed_reads(g, 4000, 150, 82, sub_rate=0.01)
-    b2, o2 = synth.fixed_reads(g, 4000, 150, 83, sub_rate=0.01)
+    b1, o1 = synth.fixed_reads(g, reads, 150, 82, sub_rate=0.01)
+    b2, o2 = synth.fixed_reads(g, reads, 150, 83, sub_rate=0.01)
     of = oracle.Filter(oracle.make_cfg(k, m, s, h, bits, cse, bse))
     oracle.insert_records(of, b1, o1)
     oracle.insert_records(of, b2, o2)
@@ -290,3 +293,45 @@ def test_work_items_counted_in_sub_batches(gpu, oracle, bits):
     b.add_sequences((b1, o1))
     b.add_sequences((b2, o2))
     assert np.array_equal(b.counters().astype(np.uint32), of.counters())
+
+
+def _low_complexity_reads(seed):
+    """Homopolymers, short tandem repeats (ACGT is its own reverse complement),
+    repeats broken by N, by lowercase and by single substitutions, repeats
+    glued to random sequence, and reads of length around k and m: every window
+    of a repeat has tied m-mer hashes, so the minimizer tie-break and the
+    canonical strand choice decide the records."""
+    rng = np.random.default_rng(seed)
+    units = ["A", "C", "G", "T", "AC", "AT", "CG", "AAC", "ACG", "ACGT", "AAAAC", "CAGCAG", "ACGTTGCA"]
+    reads = []
+    for u in units:
+        for n in (5, 9, 20, 21, 24, 25, 30, 31, 32, 33, 45, 62, 63, 150, 400):
+            reads.append((u * (n // len(u) + 1))[:n])
+    rand = lambda n: "".join(rng.choice(list("ACGT"), n))
+    for u in units:
+        r = (u * 200)[:200]
+        reads.append(r[:90] + "N" + r[91:])
+        reads.append(r[:70] + r[70:80].lower() + r[80:])
+        reads.append(r[:100] + ("C" if r[100] != "C" else "G") + r[101:])
+        reads.append(rand(60) + r[:120] + rand(40))
+        reads.append(r[:50] + rand(33) + r[:50] + rand(7) + r[:90])
+    return [np.frombuffer(x.encode(), dtype=np.uint8) for x in reads]
+
+
+@pytest.mark.parametrize("geo", [(31, 21, 25, 4, 16, 22, 9), (21, 15, 17, 3, 32, 22, 10), (9, 5, 7, 3, 8, 16, 6)],
+                         ids=["k31m21s25", "k21m15s17", "k9m5s7"])
+@pytest.mark.parametrize("method", METHODS, ids=["batched", "plain"])
+def test_low_complexity_reads_parity(gpu, oracle, geo, method):
+    """Low-complexity reads, each inserted 3 times, queried forward and reverse
+    complemented: counters and abundances equal the oracle's."""
+    scb = gpu
+    print(f"low complexity geo={geo} method={method}", flush=True)
+    reads = _low_complexity_reads(91) * 3
+    ib = np.concatenate(reads)
+    io = np.concatenate([[0], np.cumsum([r.size for r in reads])]).astype(np.uint64)
+    comp = np.frombuffer(bytes.maketrans(b"ACGTacgtN", b"TGCAtgcaN"), dtype=np.uint8)
+    rc = [comp[r[::-1]] for r in reads[: len(reads) // 3]]
+    qreads = reads[: len(reads) // 3] + rc
+    qb = np.concatenate(qreads)
+    qo = np.concatenate([[0], np.cumsum([r.size for r in qreads])]).astype(np.uint64)
+    _parity_roundtrip(scb, oracle, geo, ib, io, qb, qo, method)
