@@ -105,6 +105,22 @@ def test_minimizer_scan_matches_brute_force_and_invariants(k, m):
         assert np.array_equal(fast, (L - m - r)[::-1])
 
 
+@pytest.mark.parametrize("k,m", [(9, 5), (31, 21), (21, 15), (31, 11)])
+def test_minimizer_ties_on_repeats(k, m):
+    """Homopolymers and short tandem repeats tie every m-mer hash of a window
+    (ACGT and ACGTTGCA are their own reverse complements): the fast scan equals
+    brute force and the tie rule keeps the scan strand-symmetric."""
+    rng = random.Random(k * 7 + m)
+    for unit in ["A", "T", "AC", "CG", "AAC", "ACG", "ACGT", "AAAAC", "CAGCAG", "ACGTTGCA"]:
+        for n in (k, k + 1, k + 13, 150):
+            rep = (unit * (n // len(unit) + 1))[:n]
+            for s in (rep, rand_seq(rng, 20) + rep + rand_seq(rng, 9)):
+                fast = O.minimizer_positions(s, k, m)
+                assert np.array_equal(fast, O.minimizer_positions(s, k, m, brute=True))
+                r = O.minimizer_positions(rc_str(s), k, m)
+                assert np.array_equal(fast, (len(s) - m - r)[::-1]), (unit, n)
+
+
 def test_super_kmers_partition_and_density():
     rng = random.Random(7)
     k, m = 31, 21
