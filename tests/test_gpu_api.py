@@ -165,6 +165,41 @@ def test_spectrum_parity(gpu, oracle, geo):
     assert sum(c for _, c in rows) == distinct   # fastx.rs:161
 
 
+SPEC_DUP_GEOS = SPEC_GEOS + [(9, 5, 7, 2, 32, 1, 0), (9, 5, 7, 3, 8, 4, 2)]  # bins of 8 and 4 bytes
+
+
+@pytest.mark.parametrize("geo", SPEC_DUP_GEOS, ids=[f"k{g[0]}s{g[2]}u{g[4]}c{g[5]}b{g[6]}" for g in SPEC_DUP_GEOS])
+def test_spectrum_repeated_chunks_parity(gpu, oracle, geo):
+    """Every chunk five times over plus homopolymers and tandem repeats: most
+    records of a bin repeat an earlier one and most keys are found present;
+    queried, distinct and the rows equal the oracle's, also for bins of 8 and 4
+    bytes."""
+    scb = gpu
+    k, m, s, h, bits, cse, bse = geo
+    print(f"spectrum repeated case {geo}", flush=True)
+    g = synth.genome(20_000, 31)
+    ib, io = synth.fixed_reads(g, 1000, 150, 32, sub_rate=0.01)
+    qb, qo = synth.fixed_reads(g, 300, 150, 33, sub_rate=0.01)
+    reads = [bytes(qb[int(qo[i]):int(qo[i + 1])]) for i in range(qo.size - 1)]
+    for unit in (b"A", b"AC", b"ACGT", b"AAC", b"CAGCAG"):
+        reads.append((unit * 200)[:150])
+    chunks = [r for r in reads for _ in range(5) if len(r) >= k]
+    cfg = scb.SuperCountingBloomConfig(k=k, m=m, s=s, n_hashes=h, counter_bits=bits, counter_slots_exponent=cse,
+                                       block_slots_exponent=bse, threads=1, queue=4)
+    b = scb.SuperCountingBloomBuilder(cfg)
+    b.add_sequences((ib, io))
+    b.add_sequence(b"".join(reads[-5:]))
+    idx = b.into_index()
+    rows, queried, distinct = idx.spectrum(chunks)
+    of = oracle.Filter(oracle.make_cfg(k, m, s, h, bits, cse, bse))
+    oracle.insert_records(of, ib, io)
+    of.insert_ascii(b"".join(reads[-5:]))
+    sp = oracle.Spectrum()
+    oq = sum(sp.add_chunk(of, c) for c in chunks)
+    assert queried == oq and distinct == sp.distinct()
+    assert rows == sp.rows()
+
+
 def test_counter_histogram_after_load_matches_oracle(gpu, oracle, tmp_path):
     scb = gpu
     cfg_o = oracle.make_cfg(21, 15, 17, 3, 32, 14, 8, 1, 1)
