@@ -1265,16 +1265,23 @@ __device__ __forceinline__ void order_by_hi(const uint32_t (&hi)[R], uint32_t n,
         }
         __syncthreads();
         if (pass == 0) {
-            if (tid == 0) {
-                uint32_t run = 0, smers = 0;
-                for (int c = 0; c < 32; ++c) {
-                    const uint32_t v = s_cls[c];
-                    s_cls[c] = run;
-                    run += v;
-                    smers += v * (32 - c + k_minus_s);  // class c holds records of 32 - c k-mers
+            if (tid < 64) {  // the first wave: class c in lane c, prefix by shuffles
+                const uint32_t v = lane < 32 ? s_cls[lane] : 0u;
+                uint32_t run = v;
+#pragma unroll
+                for (int o = 1; o < 32; o <<= 1) {
+                    const uint32_t y = __shfl_up(run, o, 64);
+                    run += lane >= uint32_t(o) ? y : 0u;
                 }
-                if (s_smers) *s_smers = smers;
-                if (s_total) *s_total = run;
+                if (lane < 32) s_cls[lane] = run - v;
+                uint32_t smers = lane < 32 ? v * (32 - lane + k_minus_s) : 0u;  // class c: records of 32 - c k-mers
+#pragma unroll
+                for (int o = 16; o > 0; o >>= 1) smers += __shfl_xor(smers, o, 64);
+                const uint32_t total = __shfl(run, 31, 64);
+                if (lane == 0) {
+                    if (s_smers) *s_smers = smers;
+                    if (s_total) *s_total = total;
+                }
             }
             __syncthreads();
         }
