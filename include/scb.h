@@ -291,10 +291,13 @@ int scb_server_submit_abundances(scb_server *srv, const uint8_t *bases, const ui
 int scb_server_submit_spectrum(scb_server *srv, const uint8_t *bases, const uint64_t *offsets,
                                size_t n, uint64_t *values, uint64_t *counts, size_t cap,
                                uint64_t *ticket);
-/* Blocks until the request has finished; each ticket is waited for once. */
+/* Blocks until the request has finished; each ticket is waited for once. A
+ * ticket that another scb_server_wait call is blocked on is refused
+ * (SCB_ERR_INTERNAL_STATE), and scb_server_wait_any never returns it. */
 int scb_server_wait(scb_server *srv, uint64_t ticket, scb_served *out);
-/* Waits up to timeout_us for any finished request not yet waited for; *ticket
- * is 0 when none finished in time. */
+/* Waits up to timeout_us for any finished request not yet waited for and not
+ * claimed by a blocked scb_server_wait call; *ticket is 0 when none finished in
+ * time. */
 int scb_server_wait_any(scb_server *srv, uint64_t timeout_us, uint64_t *ticket, scb_served *out);
 /* Page-locked host memory for request buffers (faster host<->HBM copies). */
 void *scb_host_alloc(size_t bytes);

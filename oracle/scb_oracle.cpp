@@ -322,14 +322,16 @@ struct FilterBase {
     virtual uint64_t insert_packed(const Packed &seq) = 0;
     virtual void estimate_abundances(const Packed &seq, std::vector<uint64_t> &out) const = 0;
     virtual void stream(const Packed &seq, uint64_t &queried, uint64_t &checksum) const = 0;
-    virtual uint64_t estimate_map(const Packed &seq,
-                                  std::unordered_map<uint64_t, uint64_t> &map) const = 0;
+    // (canonical k-mer, estimate) of every k-mer of seq in position order
+    virtual uint64_t estimate_pairs(const Packed &seq,
+                                    std::vector<std::pair<uint64_t, uint64_t>> &out) const = 0;
     virtual uint64_t get(uint64_t block, uint64_t addr) const = 0;
     virtual uint64_t max_value() const = 0;
     virtual void to_u32(uint32_t *out) const = 0;
     virtual void from_u32(const uint32_t *in) = 0;
     virtual size_t slots() const = 0;
     virtual void histogram(std::unordered_map<uint64_t, uint64_t> &h) const = 0;
+    virtual const void *data(size_t &bytes) const = 0;
     virtual void write_le(FILE *fp) const = 0;
     virtual bool read_le(FILE *fp) = 0;
     uint64_t phys_block(uint64_t block) const {  // filter.rs:217-219 -> .scb order
@@ -461,14 +463,16 @@ struct Filter final : FilterBase {
         });
     }
 
-    // filter.rs:302-335 + 407-460: map[canonical k-mer] = estimate, first wins
-    uint64_t estimate_map(const Packed &seq,
-                          std::unordered_map<uint64_t, uint64_t> &map) const override {
+    // filter.rs:302-335 + 407-460: the (canonical k-mer, estimate) pairs that
+    // estimate_super_kmer_in_shard offers to the map in position order; the
+    // caller inserts them first-wins (filter.rs:454)
+    uint64_t estimate_pairs(const Packed &seq,
+                            std::vector<std::pair<uint64_t, uint64_t>> &out) const override {
         if (seq.len < cfg.k) return 0;
         uint64_t queried = 0;
         for_each_super_kmer(seq, [&](const C *blk, size_t a, size_t b) {
             super_kmer_window_min(blk, seq, a, b, [&](size_t kpos, C e) {
-                map.try_emplace(canonical_kmer(seq, cfg.k, kpos), uint64_t(e));
+                out.emplace_back(canonical_kmer(seq, cfg.k, kpos), uint64_t(e));
                 ++queried;
             });
         });
@@ -481,16 +485,34 @@ struct Filter final : FilterBase {
     void from_u32(const uint32_t *in) override {
         for (size_t i = 0; i < counters.size(); ++i) counters[i] = C(in[i]);
     }
-    // filter.rs:50-55
+    // filter.rs:50-55: every counter value is counted. Values below 2^16 are
+    // counted densely, by worker threads over slices of the table; the rest
+    // (32-bit counters only) go to a map.
     void histogram(std::unordered_map<uint64_t, uint64_t> &h) const override {
-        if (sizeof(C) <= 2) {
-            std::vector<uint64_t> dense(size_t(kMax) + 1, 0);
-            for (C c : counters) ++dense[c];
-            for (size_t v = 0; v < dense.size(); ++v)
-                if (dense[v]) h[v] += dense[v];
-        } else {
-            for (C c : counters) ++h[c];
+        const size_t n = counters.size();
+        const size_t nt = std::max<size_t>(1, std::min<size_t>(16, n >> 24));
+        std::vector<std::vector<uint64_t>> dense(nt, std::vector<uint64_t>(65536, 0));
+        std::vector<std::unordered_map<uint64_t, uint64_t>> wide(nt);
+        std::vector<std::thread> pool;
+        for (size_t t = 0; t < nt; ++t)
+            pool.emplace_back([&, t] {
+                const size_t a = n * t / nt, b = n * (t + 1) / nt;
+                for (size_t i = a; i < b; ++i) {
+                    const uint64_t c = counters[i];
+                    if (c < 65536) ++dense[t][c];
+                    else ++wide[t][c];
+                }
+            });
+        for (auto &th : pool) th.join();
+        for (size_t t = 0; t < nt; ++t) {
+            for (size_t v = 0; v < 65536; ++v)
+                if (dense[t][v]) h[v] += dense[t][v];
+            for (auto &kv : wide[t]) h[kv.first] += kv.second;
         }
+    }
+    const void *data(size_t &bytes) const override {
+        bytes = counters.size() * sizeof(C);
+        return counters.data();
     }
     // counter.rs:31-135: little-endian, shard-major (filter.rs:575-580)
     void write_le(FILE *fp) const override {
@@ -532,8 +554,18 @@ Packed pack_run(const uint8_t *seq, size_t a, size_t b) {
     return Packed(seq + a, b - a);
 }
 
+// The distinct-k-mer map of fastx.rs:255-308, split by a hash of the key into
+// kParts maps so that merges run in parallel; each key lives in exactly one of
+// them, so first-wins per map is first-wins over the whole key set.
 struct Spectrum {
-    std::unordered_map<uint64_t, uint64_t> map;
+    static constexpr int kParts = 64;
+    std::unordered_map<uint64_t, uint64_t> map[kParts];
+    static int part(uint64_t key) { return int((key * 0x9E3779B97F4A7C15ull) >> 58); }
+    size_t size() const {
+        size_t n = 0;
+        for (auto &m : map) n += m.size();
+        return n;
+    }
 };
 
 // Bounded MPMC channel of owned records (crossbeam_channel::bounded semantics).
@@ -614,6 +646,25 @@ std::vector<double> deconvolved_counts(const uint64_t *values, const uint64_t *c
     for (size_t a = 1; a <= max_abundance; ++a)
         out.push_back(std::fmax(double(total_counters) * log_coefficients[a] / hash_count, 0.0));
     return out;
+}
+
+// Record-parallel forms of the per-record calls above, for inputs of millions
+// of records: records [0, n) are dealt to `threads` workers in contiguous
+// ranges, as add_fasta_typed's and query_fasta's worker pools share records
+// (api.rs:278-313, 621-666). Inserts lock the shard of every super-k-mer
+// (filter.rs:220-222) and saturating increments commute, so the table equals
+// the one-record-at-a-time result.
+template <class Fn>
+void for_record_ranges(size_t n, size_t threads, Fn &&fn) {
+    threads = std::max<size_t>(1, std::min(threads, n));
+    if (threads == 1) {
+        fn(size_t(0), size_t(0), n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (size_t t = 0; t < threads; ++t)
+        pool.emplace_back([&, t] { fn(t, n * t / threads, n * (t + 1) / threads); });
+    for (auto &th : pool) th.join();
 }
 
 }  // namespace
@@ -699,6 +750,9 @@ void oracle_filter_free(oracle_filter *f) { delete f; }
 size_t oracle_filter_counter_slots(const oracle_filter *f) { return f->f->slots(); }
 void oracle_filter_counters_u32(const oracle_filter *f, uint32_t *out) { f->f->to_u32(out); }
 void oracle_filter_set_counters_u32(oracle_filter *f, const uint32_t *in) { f->f->from_u32(in); }
+// The counter table itself (physical order, native width); valid until the
+// filter is freed.
+const void *oracle_filter_data(const oracle_filter *f, size_t *bytes) { return f->f->data(*bytes); }
 
 uint64_t oracle_filter_insert_packed(oracle_filter *f, const uint8_t *ascii, size_t len) {
     return f->f->insert_packed(Packed(ascii, len));
@@ -782,6 +836,41 @@ size_t oracle_filter_estimate_ascii_brute(const oracle_filter *f, const uint8_t 
     return n;
 }
 
+void oracle_filter_insert_records(oracle_filter *f, const uint8_t *bases, const uint64_t *offsets,
+                                  size_t n, size_t threads, uint64_t *kmers, uint64_t *smers) {
+    std::vector<uint64_t> ik(std::max<size_t>(threads, 1), 0), is(std::max<size_t>(threads, 1), 0);
+    for_record_ranges(n, threads, [&](size_t t, size_t a, size_t b) {
+        for (size_t r = a; r < b; ++r)
+            for_each_valid_run(bases + offsets[r], size_t(offsets[r + 1] - offsets[r]), f->f->cfg.k,
+                               [&](size_t x, size_t y) {
+                                   const uint8_t *rec = bases + offsets[r];
+                                   ik[t] += y - x + 1 - f->f->cfg.k;  // api.rs:723
+                                   is[t] += f->f->insert_packed(pack_run(rec, x, y));
+                               });
+    });
+    uint64_t tk = 0, ts = 0;
+    for (size_t t = 0; t < ik.size(); ++t) tk += ik[t], ts += is[t];
+    if (kmers) *kmers = tk;
+    if (smers) *smers = ts;
+}
+
+// api.rs:799-853 per record; values/valid are indexed by position
+// (offsets[r] + window), entries of positions where no window starts are 0.
+void oracle_filter_estimate_records(const oracle_filter *f, const uint8_t *bases,
+                                    const uint64_t *offsets, size_t n, size_t threads,
+                                    uint64_t *values, uint8_t *valid) {
+    memset(values, 0, size_t(offsets[n]) * sizeof(uint64_t));
+    memset(valid, 0, size_t(offsets[n]));
+    for_record_ranges(n, threads, [&](size_t, size_t a, size_t b) {
+        for (size_t r = a; r < b; ++r) {
+            const size_t len = size_t(offsets[r + 1] - offsets[r]);
+            if (len >= f->f->cfg.k)
+                oracle_filter_estimate_ascii(f, bases + offsets[r], len, values + offsets[r],
+                                             valid + offsets[r]);
+        }
+    });
+}
+
 size_t oracle_filter_counter_histogram(const oracle_filter *f, uint64_t *values, uint64_t *counts,
                                        size_t cap) {
     std::unordered_map<uint64_t, uint64_t> h;
@@ -797,13 +886,53 @@ void oracle_spectrum_free(oracle_spectrum *sp) { delete sp; }
 uint64_t oracle_spectrum_add_chunk(oracle_spectrum *sp, const oracle_filter *f, const uint8_t *ascii,
                                    size_t len) {
     if (len < f->f->cfg.k) return 0;  // fastx.rs:284-286
-    return f->f->estimate_map(Packed(ascii, len), sp->s.map);
+    std::vector<std::pair<uint64_t, uint64_t>> pairs;
+    uint64_t q = f->f->estimate_pairs(Packed(ascii, len), pairs);
+    for (auto &kv : pairs) sp->s.map[Spectrum::part(kv.first)].try_emplace(kv.first, kv.second);
+    return q;
 }
-size_t oracle_spectrum_distinct(const oracle_spectrum *sp) { return sp->s.map.size(); }
+size_t oracle_spectrum_distinct(const oracle_spectrum *sp) { return sp->s.size(); }
+// estimate_distinct_kmers (fastx.rs:255-308) over records split into their
+// DnaChunks (maximal ACGT runs of at least k bases, fastx.rs:14-19, 234-236).
+// Worker t walks its records in order and keeps its pairs per key part; then
+// each part's map takes the pairs of worker 0, 1, ... in order, first wins.
+// That is the result of per-worker first-wins maps (filter.rs:454) merged
+// first-wins in worker order (fastx.rs:392-400). Returns the queried k-mers.
+uint64_t oracle_spectrum_add_records(oracle_spectrum *sp, const oracle_filter *f, const uint8_t *bases,
+                                     const uint64_t *offsets, size_t n, size_t threads) {
+    constexpr int P = Spectrum::kParts;
+    const size_t T = std::max<size_t>(1, std::min(threads, std::max<size_t>(n, 1)));
+    std::vector<std::vector<std::vector<std::pair<uint64_t, uint64_t>>>> parts(
+        T, std::vector<std::vector<std::pair<uint64_t, uint64_t>>>(P));
+    std::vector<uint64_t> q(T, 0);
+    for_record_ranges(n, T, [&](size_t t, size_t a, size_t b) {
+        std::vector<std::pair<uint64_t, uint64_t>> pairs;
+        for (size_t r = a; r < b; ++r) {
+            const uint8_t *rec = bases + offsets[r];
+            for_each_valid_run(rec, size_t(offsets[r + 1] - offsets[r]), f->f->cfg.k,
+                               [&](size_t x, size_t y) {
+                                   pairs.clear();
+                                   q[t] += f->f->estimate_pairs(pack_run(rec, x, y), pairs);
+                                   for (auto &kv : pairs) parts[t][Spectrum::part(kv.first)].push_back(kv);
+                               });
+        }
+    });
+    for_record_ranges(P, T, [&](size_t, size_t a, size_t b) {
+        for (size_t p = a; p < b; ++p)
+            for (size_t t = 0; t < T; ++t) {
+                for (auto &kv : parts[t][p]) sp->s.map[p].try_emplace(kv.first, kv.second);
+                std::vector<std::pair<uint64_t, uint64_t>>().swap(parts[t][p]);
+            }
+    });
+    uint64_t total = 0;
+    for (uint64_t x : q) total += x;
+    return total;
+}
 // fastx.rs:402-410
 size_t oracle_spectrum_rows(const oracle_spectrum *sp, uint64_t *values, uint64_t *counts, size_t cap) {
     std::unordered_map<uint64_t, uint64_t> h;
-    for (auto &kv : sp->s.map) ++h[kv.second];
+    for (auto &m : sp->s.map)
+        for (auto &kv : m) ++h[kv.second];
     std::vector<std::pair<uint64_t, uint64_t>> rows(h.begin(), h.end());
     std::sort(rows.begin(), rows.end());
     for (size_t i = 0; i < rows.size() && i < cap; ++i) values[i] = rows[i].first, counts[i] = rows[i].second;

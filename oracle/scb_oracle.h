@@ -82,6 +82,18 @@ size_t oracle_filter_estimate_ascii(const oracle_filter *f, const uint8_t *seq, 
  * k-s+1 s-mers x n_hashes counters of the window's block); same output shape */
 size_t oracle_filter_estimate_ascii_brute(const oracle_filter *f, const uint8_t *seq, size_t len,
                                           uint64_t *values, uint8_t *valid);
+/* Record-parallel insert / abundances over n records (bases + n+1 offsets) on
+ * `threads` workers, as the worker pools of add_fasta_typed / query_fasta
+ * (api.rs:278-313, 621-666): per record, the run split of api.rs:676-726 /
+ * 799-853. Abundances are indexed by position (offsets[r] + window); values
+ * and valid hold offsets[n] entries. */
+void oracle_filter_insert_records(oracle_filter *f, const uint8_t *bases, const uint64_t *offsets,
+                                  size_t n, size_t threads, uint64_t *kmers, uint64_t *smers);
+void oracle_filter_estimate_records(const oracle_filter *f, const uint8_t *bases,
+                                    const uint64_t *offsets, size_t n, size_t threads,
+                                    uint64_t *values, uint8_t *valid);
+/* the counter table in physical order at its native width (bytes receives its size) */
+const void *oracle_filter_data(const oracle_filter *f, size_t *bytes);
 /* counter_histogram (filter.rs:267-275): rows sorted by value; returns row count */
 size_t oracle_filter_counter_histogram(const oracle_filter *f, uint64_t *values, uint64_t *counts,
                                        size_t cap);
@@ -93,6 +105,10 @@ void oracle_spectrum_free(oracle_spectrum *sp);
 /* queries one DnaChunk (all-ACGT run); returns queried k-mers */
 uint64_t oracle_spectrum_add_chunk(oracle_spectrum *sp, const oracle_filter *f,
                                    const uint8_t *ascii, size_t len);
+/* estimate_distinct_kmers (fastx.rs:255-308) over the DnaChunks of n records on
+ * `threads` workers, merged first-wins (fastx.rs:392-400); returns queried k-mers */
+uint64_t oracle_spectrum_add_records(oracle_spectrum *sp, const oracle_filter *f, const uint8_t *bases,
+                                     const uint64_t *offsets, size_t n, size_t threads);
 size_t oracle_spectrum_distinct(const oracle_spectrum *sp);
 size_t oracle_spectrum_rows(const oracle_spectrum *sp, uint64_t *values, uint64_t *counts, size_t cap);
 

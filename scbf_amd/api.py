@@ -644,7 +644,7 @@ class SuperCountingBloomServer:
     def wait(self, ticket: int) -> Served:
         r = ScbServed()
         rc = lib().scb_server_wait(self._h, ticket, C.byref(r))
-        if rc != 0 and ticket in self._live:
+        if rc != 0 and r.done_ns and ticket in self._live:  # a failed request was taken
             self._live.pop(ticket)
         _check(rc)
         return self._result(ticket, r)
@@ -653,7 +653,7 @@ class SuperCountingBloomServer:
         r = ScbServed()
         t = C.c_uint64()
         rc = lib().scb_server_wait_any(self._h, int(timeout_s * 1e6), C.byref(t), C.byref(r))
-        if rc != 0 and t.value in self._live:
+        if rc != 0 and t.value and t.value in self._live:
             self._live.pop(t.value)
         _check(rc)
         return self._result(t.value, r) if t.value else None

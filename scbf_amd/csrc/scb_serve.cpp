@@ -52,6 +52,7 @@ struct Request {
     uint64_t *rows_v = nullptr, *rows_c = nullptr;
     size_t cap = 0;
     bool done = false;
+    bool claimed = false;  // a scb_server_wait call is blocked on this ticket
     scb_served res{};
 };
 using ReqPtr = std::shared_ptr<Request>;
@@ -443,6 +444,8 @@ namespace {
 
 int take(scb_server *srv, uint64_t ticket, scb_served *out) {
     auto it = srv->live.find(ticket);
+    if (it == srv->live.end())
+        return fail(SCB_ERR_INTERNAL_STATE, "internal state error: unknown or already waited ticket");
     ReqPtr r = it->second;
     srv->live.erase(it);
     auto f = std::find(srv->finished.begin(), srv->finished.end(), ticket);
@@ -461,6 +464,10 @@ int scb_server_wait(scb_server *srv, uint64_t ticket, scb_served *out) {
     if (it == srv->live.end())
         return fail(SCB_ERR_INTERNAL_STATE, "internal state error: unknown or already waited ticket");
     ReqPtr r = it->second;
+    if (r->claimed)
+        return fail(SCB_ERR_INTERNAL_STATE, "internal state error: the ticket is already being waited for");
+    // scb_server_wait_any skips claimed tickets, so this call alone takes it
+    r->claimed = true;
     srv->cv_done.wait(lk, [&] { return r->done; });
     return take(srv, ticket, out);
 }
@@ -469,9 +476,19 @@ int scb_server_wait_any(scb_server *srv, uint64_t timeout_us, uint64_t *ticket, 
     if (!srv || !ticket) return fail(SCB_ERR_INTERNAL_STATE, "internal state error: null argument");
     std::unique_lock<std::mutex> lk(srv->mu);
     *ticket = 0;
-    if (!srv->cv_done.wait_for(lk, std::chrono::microseconds(timeout_us), [&] { return !srv->finished.empty(); }))
-        return SCB_OK;
-    uint64_t t = srv->finished.front();
+    // the oldest finished request that no scb_server_wait call has claimed
+    uint64_t t = 0;
+    auto pick = [&] {
+        for (uint64_t f : srv->finished) {
+            auto it = srv->live.find(f);
+            if (it != srv->live.end() && !it->second->claimed) {
+                t = f;
+                return true;
+            }
+        }
+        return false;
+    };
+    if (!srv->cv_done.wait_for(lk, std::chrono::microseconds(timeout_us), pick)) return SCB_OK;
     *ticket = t;
     return take(srv, t, out);
 }

@@ -144,3 +144,53 @@ def serving_requests(g, count, seed, n_range=(1, 10_000), len_range=(31, 20_000)
         offsets[1:] = np.cumsum(lens)
         out.append((bases, offsets))
     return out
+
+
+def concat_sets(parts):
+    """Concatenates (bases, offsets) read sets into one."""
+    bases = np.concatenate([b for b, _ in parts])
+    offs = [np.zeros(1, dtype=np.uint64)]
+    for _, o in parts:
+        offs.append(o[1:].astype(np.uint64) + offs[-1][-1])
+    return np.ascontiguousarray(bases), np.concatenate(offs).astype(np.uint64)
+
+
+def default_geometry_workload(seed=1, insert_coverage=50, fresh_coverage=10):
+    """Short-read workload of the default geometry: 5 Mbp genome; insert 150 bp
+    reads at 50x (1% substitutions, 0.1% N); query = the inserted reads followed
+    by a fresh 10x set. Returns (genome, insert bases, insert offsets, query
+    bases, query offsets)."""
+    g = genome(5_000_000, seed)
+    ib, io = coverage_reads(g, insert_coverage, 150, seed + 1, sub_rate=0.01, n_rate=0.001)
+    nb, no = coverage_reads(g, fresh_coverage, 150, seed + 2, sub_rate=0.01, n_rate=0.001)
+    qb, qo = concat_sets([(ib, io), (nb, no)])
+    return g, ib, io, qb, qo
+
+
+def repeat_genome(seed=3, length=20_000_000, repeat_len=300, copies=10_000):
+    """20 Mbp uniform genome with one 300 bp random element inserted 10,000 times."""
+    return insert_repeats(genome(length, seed), repeat_len, copies, seed)
+
+
+def long_reads(g, coverage, seed):
+    """Log-normal read lengths (median 8 kbp, sigma 0.7, clipped to 500 bp-60 kbp)
+    at the given coverage with 3% substitutions, 1% insertions, 1% deletions."""
+    return variable_reads(g, int(g.size * coverage), 8000, 0.7, 500, 60_000, seed,
+                          sub=0.03, ins=0.01, dele=0.01)
+
+
+def mixture_reads(n_genomes, genome_len, total_bases, seed, read_len=150, sigma=2.0, sub_rate=0.005):
+    """Reads of a mixture of n_genomes random genomes whose relative abundances are
+    exp(N(0, sigma^2)); 150 bp reads, total_bases in all, 0.5% substitutions."""
+    rng = np.random.default_rng([seed, 7])
+    w = np.exp(rng.normal(0.0, sigma, size=n_genomes))
+    w /= w.sum()
+    n_reads = int(total_bases // read_len)
+    counts = rng.multinomial(n_reads, w)
+    parts = []
+    for i, n in enumerate(counts):
+        if n == 0:
+            continue
+        g = genome(genome_len, seed * 1000 + i)
+        parts.append(fixed_reads(g, int(n), read_len, seed * 1000 + 500 + i, sub_rate=sub_rate))
+    return concat_sets(parts)

@@ -66,6 +66,11 @@ def lib():
             "oracle_filter_estimate_ascii": (C.c_size_t, [C.c_void_p, u8p, C.c_size_t, u64p, u8p]),
             "oracle_filter_estimate_ascii_brute": (C.c_size_t, [C.c_void_p, u8p, C.c_size_t, u64p, u8p]),
             "oracle_filter_counter_histogram": (C.c_size_t, [C.c_void_p, u64p, u64p, C.c_size_t]),
+            "oracle_filter_insert_records": (None, [C.c_void_p, u8p, u64p, C.c_size_t, C.c_size_t, u64p, u64p]),
+            "oracle_filter_estimate_records": (None, [C.c_void_p, u8p, u64p, C.c_size_t, C.c_size_t, u64p, u8p]),
+            "oracle_filter_data": (C.c_void_p, [C.c_void_p, C.POINTER(C.c_size_t)]),
+            "oracle_spectrum_add_records": (C.c_uint64, [C.c_void_p, C.c_void_p, u8p, u64p, C.c_size_t,
+                                                         C.c_size_t]),
             "oracle_spectrum_new": (C.c_void_p, []),
             "oracle_spectrum_free": (None, [C.c_void_p]),
             "oracle_spectrum_add_chunk": (C.c_uint64, [C.c_void_p, C.c_void_p, u8p, C.c_size_t]),
@@ -89,6 +94,15 @@ def lib():
             fn.restype, fn.argtypes = res, args
         _L = L
     return _L
+
+
+def worker_threads():
+    """Oracle worker threads: the CPUs this process may use, at most 16."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
 
 
 def _b(seq):
@@ -206,6 +220,35 @@ class Filter:
         got = fn(self.h, _p(a), a.size, _p(v, u64p), _p(ok))
         return v[:got], ok[:got].astype(bool)
 
+    def insert_records(self, bases, offsets, threads=None):
+        """Record-parallel insert of n records; returns (inserted_kmers, inserted_smers)."""
+        b = _b(bases)
+        o = np.ascontiguousarray(offsets, dtype=np.uint64)
+        k, s = C.c_uint64(), C.c_uint64()
+        lib().oracle_filter_insert_records(self.h, _p(b), _p(o, u64p), o.size - 1, threads or worker_threads(),
+                                           C.byref(k), C.byref(s))
+        return k.value, s.value
+
+    def estimate_records(self, bases, offsets, threads=None):
+        """Position-indexed abundances (u64) and validity (bool) of n records."""
+        b = _b(bases)
+        o = np.ascontiguousarray(offsets, dtype=np.uint64)
+        n = max(1, int(o[-1]))
+        v = np.empty(n, dtype=np.uint64)
+        ok = np.empty(n, dtype=np.uint8)
+        lib().oracle_filter_estimate_records(self.h, _p(b), _p(o, u64p), o.size - 1, threads or worker_threads(),
+                                             _p(v, u64p), _p(ok))
+        return v[:int(o[-1])], ok[:int(o[-1])].view(bool)
+
+    def table(self):
+        """Zero-copy view of the counter table (physical order, native width);
+        valid while this filter lives."""
+        nbytes = C.c_size_t()
+        ptr = lib().oracle_filter_data(self.h, C.byref(nbytes))
+        dt = {8: np.uint8, 16: np.uint16, 32: np.uint32}[self.cfg.counter_bits]
+        buf = (C.c_uint8 * nbytes.value).from_address(ptr)
+        return np.frombuffer(buf, dtype=dt)
+
     def counters(self):
         n = lib().oracle_filter_counter_slots(self.h)
         out = np.zeros(n, dtype=np.uint32)
@@ -251,6 +294,13 @@ class Spectrum:
     def add_chunk(self, filt, seq):
         a = _b(seq)
         return lib().oracle_spectrum_add_chunk(self.h, filt.h, _p(a), a.size)
+
+    def add_records(self, filt, bases, offsets, threads=None):
+        """estimate_distinct_kmers over the DnaChunks of n records; returns queried k-mers."""
+        b = _b(bases)
+        o = np.ascontiguousarray(offsets, dtype=np.uint64)
+        return lib().oracle_spectrum_add_records(self.h, filt.h, _p(b), _p(o, u64p), o.size - 1,
+                                                 threads or worker_threads())
 
     def distinct(self):
         return lib().oracle_spectrum_distinct(self.h)

@@ -146,3 +146,75 @@ def test_overlong_record_is_refused(gpu):
         rows, q, d = idx.spectrum((rec, np.array([0, rec.size], dtype=np.uint64)))
         s = srv.wait(t)
         assert (s.rows, s.queried, s.distinct) == (rows, q, d)
+
+
+def test_wait_and_wait_any_from_several_threads(gpu):
+    """Threads that wait for their own tickets and threads that take any finished
+    ticket share one server: every ticket is delivered exactly once with the
+    direct call's result. A ticket taken by wait_any before its owner waits is
+    refused to the owner (InternalState); a ticket whose owner is blocked in
+    wait is never handed to wait_any, and a second wait on it is refused."""
+    import threading
+    import time
+    scb = gpu
+    g, idx, _ = _index(scb)
+    reqs = _requests(g, 48, 4242, max_records=30, max_len=2000)
+    big_b, big_o = synth.fixed_reads(g, 100_000, 150, 6)
+    with scb.SuperCountingBloomServer(idx, scb.ServeConfig(workers=2)) as srv:
+        big = srv.submit_abundances(big_b, big_o)
+        owner = {}
+
+        def own_wait():
+            owner["r"] = srv.wait(big)
+
+        th = threading.Thread(target=own_wait)
+        th.start()
+        time.sleep(0.2)
+        with pytest.raises(scb.SuperCountingBloomError) as e:
+            srv.wait(big)  # claimed by the owner thread (or already taken by it)
+        assert e.value.kind == "InternalState"
+        th.join()
+        assert owner["r"].ticket == big
+
+        tickets = [srv.submit_abundances(b, o) for b, o in reqs]
+        delivered, refused, errors = {}, [], []
+        lock = threading.Lock()
+
+        def waiter(ts):
+            for t in ts:
+                try:
+                    r = srv.wait(t)
+                    with lock:
+                        assert t not in delivered
+                        delivered[t] = r
+                except scb.SuperCountingBloomError as ex:
+                    with lock:
+                        if ex.kind == "InternalState":
+                            refused.append(t)
+                        else:
+                            errors.append(ex)
+
+        def any_waiter():
+            while True:
+                with lock:
+                    if len(delivered) == len(tickets):
+                        return
+                r = srv.wait_any(0.05)
+                if r is not None:
+                    with lock:
+                        if r.ticket in delivered:
+                            errors.append(AssertionError(f"ticket {r.ticket} delivered twice"))
+                        delivered[r.ticket] = r
+
+        threads = [threading.Thread(target=waiter, args=(tickets[i::4],)) for i in range(2)]
+        threads += [threading.Thread(target=any_waiter) for _ in range(2)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join(timeout=60)
+            assert not t.is_alive(), "a waiting thread did not finish"
+        assert not errors, errors
+        assert sorted(delivered) == sorted(tickets)
+        assert all(t in delivered for t in refused)
+        for t, r in zip(tickets, reqs):
+            _check_abundances(idx, r, delivered[t])
