@@ -191,6 +191,23 @@ int scb_index_estimate_sequences_device(const scb_index *idx, const uint8_t *d_b
                                         const uint64_t *d_offsets, size_t n,
                                         uint64_t total_bases, void *d_values,
                                         uint32_t *d_valid_bits, scb_query_report *report);
+/* No counterpart in the reference. scb_index_estimate_sequences_device for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc. Same arguments and
+ * results; its kernels take every per-call buffer from a fixed device array of
+ * SCB_NOALLOC_SCRATCH_BYTES that each GPU holds from its first call until the
+ * process ends, and the call makes no hipMalloc or hipFree. total_bases may be
+ * at most SCB_NOALLOC_MAX_BASES (larger batches are refused with
+ * SCB_ERR_INTERNAL_STATE and a message). Calls on one GPU share that GPU's
+ * array: the library serializes them with a lock (one call uses the array at a
+ * time), each runs on the null stream and returns after it has synchronized,
+ * and the array keeps the last call's contents. */
+#define SCB_NOALLOC_SCRATCH_BYTES ((uint64_t)96 << 20)
+#define SCB_NOALLOC_MAX_BASES ((uint64_t)1 << 22)
+int scb_index_estimate_sequences_device_noalloc(const scb_index *idx, const uint8_t *d_bases,
+                                                const uint64_t *d_offsets, size_t n,
+                                                uint64_t total_bases, void *d_values,
+                                                uint32_t *d_valid_bits, scb_query_report *report);
 /* SuperCountingBloomIndex::summarize_sequence (api.rs:264-271, 855-891) */
 int scb_index_summarize_sequence(const scb_index *idx, const uint8_t *seq, size_t len,
                                  scb_summary *out);

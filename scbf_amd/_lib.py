@@ -107,6 +107,9 @@ SIGNATURES = {
     "scb_index_estimate_sequences_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
                                                       C.c_uint64, C.c_void_p, C.c_void_p,
                                                       C.POINTER(ScbQueryReport)]),
+    "scb_index_estimate_sequences_device_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                                              C.c_uint64, C.c_void_p, C.c_void_p,
+                                                              C.POINTER(ScbQueryReport)]),
     "scb_index_summarize_sequence": (C.c_int, [C.c_void_p, u8p, C.c_size_t, C.POINTER(ScbSummary)]),
     "scb_index_query_fasta": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(ScbQueryReport)]),
     "scb_index_counter_histogram": (C.c_int, [C.c_void_p, u64p, u64p, C.c_size_t, szp]),

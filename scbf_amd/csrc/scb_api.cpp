@@ -764,6 +764,26 @@ int scb_index_estimate_sequences_device(const scb_index *idx, const uint8_t *d_b
     return SCB_OK;
 }
 
+// No counterpart in the reference: scb_index_estimate_sequences_device for
+// callers that cannot allocate device memory.
+int scb_index_estimate_sequences_device_noalloc(const scb_index *idx, const uint8_t *d_bases,
+                                                const uint64_t *d_offsets, size_t n, uint64_t total_bases,
+                                                void *d_values, uint32_t *d_valid_bits,
+                                                scb_query_report *report) {
+    if (!idx) return fail(SCB_ERR_INTERNAL_STATE, "internal state error: null index");
+    scb::DeviceBatch db{d_bases, d_offsets, n, total_bases};
+    scb::QueryResult r;
+    Status s = idx->engine->query_noalloc(db, idx->method, d_values, d_valid_bits, r);
+    if (!s.good()) return fail(s);
+    if (report) {
+        *report = scb_query_report{};
+        report->valid_kmers = r.queried;
+        report->positive_estimates = r.positive;
+        report->estimate_checksum = r.checksum;
+    }
+    return SCB_OK;
+}
+
 // api.rs:264-271, 855-891
 int scb_index_summarize_sequence(const scb_index *idx, const uint8_t *seq, size_t len, scb_summary *out) {
     size_t k = size_t(idx->cfg.k);

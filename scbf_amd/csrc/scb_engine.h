@@ -86,6 +86,12 @@ class Engine {
     // n_bases bits rounded up to u32 words (or null).
     Status query(const DeviceBatch &b, Method method, void *d_values, uint32_t *d_valid_bits,
                  QueryResult &r) const;
+    // query with every per-call device buffer taken from a fixed scratch array
+    // (kNoAllocScratchBytes per device, held until the process ends): no
+    // hipMalloc or hipFree; b.n_bases <= kNoAllocMaxBases; runs on the null
+    // stream; calls on one device are serialized.
+    Status query_noalloc(const DeviceBatch &b, Method method, void *d_values, uint32_t *d_valid_bits,
+                         QueryResult &r) const;
     Status spectrum(const DeviceBatch &b, SpectrumResult &r) const;
     Status counter_histogram(std::vector<std::pair<uint64_t, uint64_t>> &rows) const;
 
@@ -117,6 +123,8 @@ class Engine {
     mutable bool zero_pending_ = false;
     std::unique_ptr<Workspace> acquire() const;
     void release(std::unique_ptr<Workspace> ws) const;
+    Status run_query(Workspace &ws, const DeviceBatch &b, Method method, void *d_values, uint32_t *d_valid_bits,
+                     QueryResult &r) const;
 };
 
 // Device time of the stages of the calling thread's last insert/query call,
@@ -144,5 +152,10 @@ int device_count();
 
 // Maximum bases per device batch (positions are 32-bit inside the kernels).
 constexpr uint64_t kMaxBatchBases = (uint64_t(1) << 32) - 4096;
+
+// Allocation-free query: the fixed per-device scratch array and the largest
+// batch it serves (about 16.3 bytes per base plus the bin plan).
+constexpr uint64_t kNoAllocScratchBytes = uint64_t(96) << 20;
+constexpr uint64_t kNoAllocMaxBases = uint64_t(1) << 22;
 
 }  // namespace scb

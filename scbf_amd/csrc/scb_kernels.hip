@@ -34,6 +34,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <thread>
+#include <type_traits>
 
 #include "scb_bases.h"
 #include "scb_engine.h"
@@ -1995,8 +1996,10 @@ template <class T>
 struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
+    bool owned = true;  // false: a view of memory this buffer does not own; it never grows
     hipError_t reserve(size_t want) {
         if (want <= n) return hipSuccess;
+        if (!owned) return hipErrorOutOfMemory;
         if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;
@@ -2005,8 +2008,9 @@ struct DevBuf {
         return e;
     }
     ~DevBuf() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
     }
+    void view(T *at, size_t count) { p = at, n = count, owned = false; }
 };
 
 // Page-locked host buffer that only grows (host side of batch copies).
@@ -2051,10 +2055,23 @@ class Workspace {
     DevBuf<unsigned long long> spec_gset;
     uint64_t *h_stats = nullptr;   // pinned
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // A view workspace runs on the null stream, has no events, and its device
+    // buffers point into memory it does not own (none of them can grow); its
+    // h_stats belongs to the caller.
+    bool is_view = false;
+    void make_view() {
+        is_view = true;
+        for (auto *b : {&pk, &valid, &bin_count, &bin_off, &cursor, &ccursor, &coarse_hist, &seg_count, &tile_r0,
+                        &out_bits, &hist_big_keys, &spec_bin_kmers})
+            b->owned = false;
+        for (auto *b : {&rec, &sorted, &work, &stats, &plan_blk, &h_offsets_dev}) b->owned = false;
+        h_bases_dev.owned = out_dev.owned = false;
+        hist_dense.owned = hist_big_counts.owned = spec_gset.owned = false;
+    }
     ~Workspace() {
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
-        if (h_stats) (void)hipHostFree(h_stats);
+        if (h_stats && !is_view) (void)hipHostFree(h_stats);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -2074,7 +2091,9 @@ Status make_workspace(std::unique_ptr<Workspace> &ws) {
 
 thread_local StageTimes g_stage_times;
 
-void mark(Workspace &ws, int i) { (void)hipEventRecord(ws.ev[i], ws.stream); }
+void mark(Workspace &ws, int i) {
+    if (ws.ev[i]) (void)hipEventRecord(ws.ev[i], ws.stream);
+}
 
 // Elapsed ms between recorded events (after the stream has been synchronized).
 void collect_times(Workspace &ws, bool batched) {
@@ -2595,6 +2614,17 @@ Status Engine::query(const DeviceBatch &b, Method method, void *d_values, uint32
     SCB_HIP(hipSetDevice(device_));
     auto ws = acquire();
     if (!ws) return Status{8, "device error: cannot create a workspace"};
+    s = run_query(*ws, b, method, d_values, d_valid_bits, r);
+    release(std::move(ws));
+    return s;
+}
+
+// The query pipeline on one workspace (pooled, or a view of the fixed scratch
+// array); the stream is synchronized before it returns.
+Status Engine::run_query(Workspace &wsr, const DeviceBatch &b, Method method, void *d_values,
+                         uint32_t *d_valid_bits, QueryResult &r) const {
+    Status s;
+    Workspace *ws = &wsr;
     Launch L{geo_, dev_geo(geo_), *ws, d_table_, device_};
     hipStream_t st = ws->stream;
     if (d_valid_bits) fill(d_valid_bits, (b.n_bases + 31) / 32, 0, st);
@@ -2662,13 +2692,116 @@ Status Engine::query(const DeviceBatch &b, Method method, void *d_values, uint32
     }
     if (s.good()) s = finish(*ws);
     if (s.good()) {
-        if (b.n_bases >= geo_.k) collect_times(*ws, batched);
+        if (b.n_bases >= geo_.k && ws->ev[0]) collect_times(*ws, batched);
         r.queried = ws->h_stats[kStQueried];
         r.positive = ws->h_stats[kStPositive];
         r.checksum = ws->h_stats[kStChecksum];
         r.super_kmers = ws->h_stats[kStRecords];
     }
-    release(std::move(ws));
+    return s;
+}
+
+// ---- allocation-free query ---------------------------------------------------
+// query_noalloc runs the same kernels as query with every per-call device
+// buffer carved from `scratch`, a fixed array that each GPU holds from the
+// first call until the process ends. Host code never uses the array's name as
+// an address: hipGetSymbolAddress gives each device's own copy once, with that
+// device current, and that pointer is what the kernels receive.
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kNoAllocDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t *g_scratch_ptr[kNoAllocDevices] = {};
+std::mutex g_noalloc_call_mu[kNoAllocDevices];  // one call at a time uses a device's array
+
+// Address of the current device's copy of `scratch`, looked up on the
+// device's first call.
+Status scratch_address(int dev, uint8_t *&out) {
+    if (dev < 0 || dev >= kNoAllocDevices)
+        return Status{8, "device error: device index beyond the allocation-free entry point's table"};
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return Status{8, std::string("device error: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)};
+        g_scratch_ptr[dev] = static_cast<uint8_t *>(p);
+    }
+    out = g_scratch_ptr[dev];
+    return Status::ok();
+}
+
+// Points the buffers run_query uses at 256-byte aligned pieces of the scratch
+// array, sized as batched_front and launch_scan size them for this batch.
+// Returns the bytes needed (which may exceed cap: then nothing may run).
+uint64_t carve_query(uint8_t *base, const Geometry &g, uint64_t n_bases, uint32_t bin_log, uint32_t per_item,
+                     bool batched, Workspace &ws) {
+    uint64_t off = 0;
+    auto take = [&](auto &buf, uint64_t count) {
+        using T = std::remove_reference_t<decltype(*buf.p)>;
+        off = (off + 255) & ~uint64_t(255);
+        buf.view(reinterpret_cast<T *>(base + off), count);
+        off += std::max<uint64_t>(count, 1) * sizeof(T);
+    };
+    const uint64_t tiles = (n_bases + kTile - 1) / kTile;
+    take(ws.stats, kStatsLen);
+    take(ws.seg_count, kSeg + 1);
+    take(ws.tile_r0, std::max<uint64_t>(tiles, 1));
+    if (batched) {
+        const uint64_t nbins = uint64_t(1) << (g.cse - bin_log);
+        const uint64_t words = tiles * (kTile / 16) + kPkPadWords;
+        const uint64_t seg_cap = (tiles + kSeg - 1) / kSeg * kTile;
+        const uint64_t cap = seg_cap * kSeg;
+        take(ws.pk, words);
+        take(ws.rec, cap);
+        take(ws.sorted, cap);
+        take(ws.bin_count, nbins);
+        take(ws.bin_off, nbins);
+        take(ws.cursor, nbins);
+        take(ws.work, nbins + cap / per_item + 2);
+        take(ws.plan_blk, 2 * ((nbins + kPlanThreads - 1) / kPlanThreads));
+        if (nbins > kPartRange) {
+            const uint32_t lb = uint32_t(__builtin_ctzll(nbins));
+            const uint32_t fine_bits = std::max<uint32_t>(7, lb > 11 ? lb - 11 : 0);
+            const uint64_t ncoarse = nbins >> fine_bits;
+            take(ws.coarse_hist, ncoarse * 32);
+            take(ws.ccursor, ncoarse * 32);
+        }
+    }
+    return (off + 255) & ~uint64_t(255);
+}
+}  // namespace
+
+Status Engine::query_noalloc(const DeviceBatch &b, Method method, void *d_values, uint32_t *d_valid_bits,
+                             QueryResult &r) const {
+    r = QueryResult{};
+    if (b.n_bases > kNoAllocMaxBases)
+        return Status{6, "internal state error: the allocation-free query takes at most " +
+                             std::to_string(kNoAllocMaxBases) + " bases per call (" + std::to_string(b.n_bases) +
+                             " given)"};
+    Status s = ensure_zeroed();
+    if (!s.good()) return s;
+    if (b.n_bases == 0 || b.n_reads == 0) return Status::ok();
+    SCB_HIP(hipSetDevice(device_));
+    uint8_t *base = nullptr;
+    s = scratch_address(device_, base);
+    if (!s.good()) return s;
+    std::lock_guard<std::mutex> lk(g_noalloc_call_mu[device_]);
+    Workspace ws;  // null stream, no events, no owned buffers
+    ws.make_view();
+    uint64_t host_stats[kStatsLen];
+    ws.h_stats = host_stats;
+    const bool batched = method == kBatched && batched_ok(geo_, tune_.query_bin_log) && b.n_bases >= geo_.k;
+    const uint32_t per_item =
+        std::max<uint32_t>(1, std::min<uint32_t>(tune_.max_records_per_item, kQueryItemRecords));
+    const uint64_t need = carve_query(base, geo_, b.n_bases, tune_.query_bin_log, per_item, batched, ws);
+    if (need > kNoAllocScratchBytes)
+        return Status{6, "internal state error: the batch needs " + std::to_string(need) +
+                             " bytes of scratch, more than the " + std::to_string(kNoAllocScratchBytes) +
+                             " bytes of the allocation-free query"};
+    s = run_query(ws, b, method, d_values, d_valid_bits, r);
+    if (!s.good()) (void)hipStreamSynchronize(nullptr);  // nothing may still use the array
     return s;
 }
 

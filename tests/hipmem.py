@@ -27,6 +27,7 @@ def _hip():
         h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         h.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
         h.hipDeviceSynchronize.argtypes = []
+        h.hipMemGetInfo.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
         _H = h
     return _H
 
@@ -67,3 +68,10 @@ class DeviceBuffer:
         if out.nbytes:
             _ok(_hip().hipMemcpy(C.c_void_p(out.ctypes.data), C.c_void_p(self.ptr), out.nbytes, 2), "hipMemcpy D2H")
         return out
+
+
+def device_free_bytes():
+    """Free device memory of the current device (hipMemGetInfo)."""
+    free, total = C.c_size_t(), C.c_size_t()
+    _ok(_hip().hipMemGetInfo(C.byref(free), C.byref(total)), "hipMemGetInfo")
+    return free.value
