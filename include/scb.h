@@ -134,8 +134,11 @@ int scb_builder_add_sequence(scb_builder *b, const uint8_t *seq, size_t len,
  * AddReport fields as add_fasta_typed sums them, api.rs:633-662) */
 int scb_builder_add_sequences(scb_builder *b, const uint8_t *bases, const uint64_t *offsets,
                               size_t n, scb_add_report *report);
-/* Same with HBM-resident inputs (d_offsets: n+1 device u64; total bases must
- * be < 2^32). records_* fields of the report are not computed (left 0). */
+/* Same with HBM-resident inputs: d_offsets holds n+1 device u64 with
+ * d_offsets[0] = 0, never decreasing, d_offsets[n] = total_bases < 2^32 (checked
+ * on the device: a violation returns SCB_ERR_INTERNAL_STATE and the batch's
+ * counts are not defined). records_* fields of the report are not computed
+ * (left 0). */
 int scb_builder_add_sequences_device(scb_builder *b, const uint8_t *d_bases,
                                      const uint64_t *d_offsets, size_t n, uint64_t total_bases,
                                      scb_add_report *report);
@@ -182,8 +185,9 @@ int scb_index_estimate_sequence_abundances(const scb_index *idx, const uint8_t *
 int scb_index_estimate_sequences(const scb_index *idx, const uint8_t *bases,
                                  const uint64_t *offsets, size_t n, uint64_t *values,
                                  uint8_t *valid, scb_query_report *report);
-/* Same with HBM-resident inputs and outputs: d_values holds total_bases
- * counters of counter_bits/8 bytes each, d_valid_bits total_bases bits
+/* Same with HBM-resident inputs and outputs (d_offsets as for
+ * scb_builder_add_sequences_device, checked the same way): d_values holds
+ * total_bases counters of counter_bits/8 bytes each, d_valid_bits total_bases bits
  * (rounded up to 32-bit words); either may be NULL to skip it (then only the
  * report is produced: the stream path of fastx.rs:310-369). report's
  * records_processed/total_windows are not computed (left 0). */
@@ -221,7 +225,9 @@ int scb_index_counter_histogram(const scb_index *idx, uint64_t *values, uint64_t
 /* Distinct-k-mer spectrum of a batch of query records against the index
  * (fastx.rs:255-308, 392-410 with the DnaChunk split of fastx.rs:14-19):
  * rows (estimate, #distinct canonical k-mers) sorted by estimate. *queried
- * receives the number of queried k-mers, *distinct the number of keys. */
+ * receives the number of queried k-mers, *distinct the number of keys. Host
+ * input of any size (beyond one device batch it is counted chunk by chunk in
+ * a key set); the _device form takes one batch of fewer than 2^32 bases. */
 int scb_index_spectrum(const scb_index *idx, const uint8_t *bases, const uint64_t *offsets,
                        size_t n, uint64_t *values, uint64_t *counts, size_t cap, size_t *n_rows,
                        uint64_t *queried, uint64_t *distinct);
@@ -229,6 +235,30 @@ int scb_index_spectrum_device(const scb_index *idx, const uint8_t *d_bases,
                               const uint64_t *d_offsets, size_t n, uint64_t total_bases,
                               uint64_t *values, uint64_t *counts, size_t cap, size_t *n_rows,
                               uint64_t *queried, uint64_t *distinct);
+/* FrozenSuperCountingBloom::estimate_packed_sequence (filter.rs:302-335, 407-460)
+ * over the DnaChunks of n host records (maximal ACGT runs of at least k bases):
+ * every distinct canonical k-mer with its estimate (the HashMap<KmerKey, C> of
+ * the reference, first occurrence wins; all occurrences of a key have the same
+ * estimate), as pairs sorted by key. *n_keys receives the number of keys (pairs
+ * beyond cap are not written), *queried the number of k-mers queried. */
+int scb_index_estimate_kmer_map(const scb_index *idx, const uint8_t *bases, const uint64_t *offsets,
+                                size_t n, uint64_t *keys, uint64_t *values, size_t cap,
+                                size_t *n_keys, uint64_t *queried);
+
+/* Distinct-k-mer spectrum over many batches (estimate_distinct_kmers,
+ * fastx.rs:255-308, with its per-worker maps merged first-wins, 392-400): a
+ * device key set that grows as needed; a key is counted once over all batches.
+ * Rows are (estimate, #distinct keys) sorted by estimate, as scb_index_spectrum.
+ * The index must outlive the accumulator; calls on one accumulator are
+ * serialized by the caller. */
+typedef struct scb_spectrum_acc scb_spectrum_acc;
+int scb_spectrum_acc_new(const scb_index *idx, scb_spectrum_acc **out);
+void scb_spectrum_acc_free(scb_spectrum_acc *acc);
+int scb_spectrum_acc_add(scb_spectrum_acc *acc, const uint8_t *bases, const uint64_t *offsets, size_t n);
+int scb_spectrum_acc_add_device(scb_spectrum_acc *acc, const uint8_t *d_bases, const uint64_t *d_offsets,
+                                size_t n, uint64_t total_bases);
+int scb_spectrum_acc_rows(const scb_spectrum_acc *acc, uint64_t *values, uint64_t *counts, size_t cap,
+                          size_t *n_rows, uint64_t *queried, uint64_t *distinct);
 /* estimate_filter_only_spectrum(_from_counter_histogram) (api.rs:337-368,
  * 933-1033); writes rows 1..=max_abundance to counts (counts[a-1]). When
  * hist_values is NULL the index's counter_histogram is used. */
@@ -241,6 +271,19 @@ int scb_index_filter_only_spectrum(const scb_index *idx, const uint64_t *hist_va
 int scb_index_filter_only_count_hint(const scb_index *idx, const uint64_t *hist_values,
                                      const uint64_t *hist_counts, size_t hist_rows,
                                      uint64_t max_abundance, uint64_t *hint);
+
+/* The same estimators on a given histogram and n_hashes, with no index and no
+ * device (extension; host arithmetic only): api.rs:933-1033 and 375-385. */
+int scb_filter_only_spectrum_from_histogram(uint64_t n_hashes, const uint64_t *hist_values,
+                                            const uint64_t *hist_counts, size_t hist_rows,
+                                            uint64_t max_abundance, uint64_t kmer_count_hint,
+                                            uint64_t *counts);
+int scb_filter_only_count_hint_from_histogram(uint64_t n_hashes, const uint64_t *hist_values,
+                                              const uint64_t *hist_counts, size_t hist_rows,
+                                              uint64_t max_abundance, uint64_t *hint);
+/* summarize_abundances (api.rs:855-891) over n windows given as values + validity
+ * (valid[i] == 0 is None); sequence_len is left for the caller. Host only. */
+int scb_summarize_abundances(const uint64_t *values, const uint8_t *valid, size_t n, scb_summary *out);
 
 /* ---- FASTX pipelines (fastx.rs:81-204) ------------------------------------ */
 /* estimate_spectrum_from_fastx_pair (fastx.rs:88-105, 133-170); query_path may

@@ -928,6 +928,14 @@ uint64_t oracle_spectrum_add_records(oracle_spectrum *sp, const oracle_filter *f
     for (uint64_t x : q) total += x;
     return total;
 }
+// The merged map itself: (key, estimate) pairs sorted by key.
+size_t oracle_spectrum_pairs(const oracle_spectrum *sp, uint64_t *keys, uint64_t *values, size_t cap) {
+    std::vector<std::pair<uint64_t, uint64_t>> all;
+    for (auto &m : sp->s.map) all.insert(all.end(), m.begin(), m.end());
+    std::sort(all.begin(), all.end());
+    for (size_t i = 0; i < all.size() && i < cap; ++i) keys[i] = all[i].first, values[i] = all[i].second;
+    return all.size();
+}
 // fastx.rs:402-410
 size_t oracle_spectrum_rows(const oracle_spectrum *sp, uint64_t *values, uint64_t *counts, size_t cap) {
     std::unordered_map<uint64_t, uint64_t> h;

@@ -111,6 +111,8 @@ uint64_t oracle_spectrum_add_records(oracle_spectrum *sp, const oracle_filter *f
                                      const uint64_t *offsets, size_t n, size_t threads);
 size_t oracle_spectrum_distinct(const oracle_spectrum *sp);
 size_t oracle_spectrum_rows(const oracle_spectrum *sp, uint64_t *values, uint64_t *counts, size_t cap);
+/* the merged map as (key, estimate) pairs sorted by key; returns the pair count */
+size_t oracle_spectrum_pairs(const oracle_spectrum *sp, uint64_t *keys, uint64_t *values, size_t cap);
 
 /* ---- .scb serialization (api.rs:388-608) --------------------------------- */
 /* returns 0 on success; message written on error */

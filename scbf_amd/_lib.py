@@ -111,6 +111,18 @@ SIGNATURES = {
                                                               C.c_uint64, C.c_void_p, C.c_void_p,
                                                               C.POINTER(ScbQueryReport)]),
     "scb_index_summarize_sequence": (C.c_int, [C.c_void_p, u8p, C.c_size_t, C.POINTER(ScbSummary)]),
+    "scb_summarize_abundances": (C.c_int, [u64p, u8p, C.c_size_t, C.POINTER(ScbSummary)]),
+    "scb_index_estimate_kmer_map": (C.c_int, [C.c_void_p, u8p, u64p, C.c_size_t, u64p, u64p, C.c_size_t, szp,
+                                              u64p]),
+    "scb_spectrum_acc_new": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "scb_spectrum_acc_free": (None, [C.c_void_p]),
+    "scb_spectrum_acc_add": (C.c_int, [C.c_void_p, u8p, u64p, C.c_size_t]),
+    "scb_spectrum_acc_add_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_uint64]),
+    "scb_spectrum_acc_rows": (C.c_int, [C.c_void_p, u64p, u64p, C.c_size_t, szp, u64p, u64p]),
+    "scb_filter_only_spectrum_from_histogram": (C.c_int, [C.c_uint64, u64p, u64p, C.c_size_t, C.c_uint64,
+                                                          C.c_uint64, u64p]),
+    "scb_filter_only_count_hint_from_histogram": (C.c_int, [C.c_uint64, u64p, u64p, C.c_size_t, C.c_uint64,
+                                                            u64p]),
     "scb_index_query_fasta": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(ScbQueryReport)]),
     "scb_index_counter_histogram": (C.c_int, [C.c_void_p, u64p, u64p, C.c_size_t, szp]),
     "scb_index_spectrum": (C.c_int, [C.c_void_p, u8p, u64p, C.c_size_t, u64p, u64p, C.c_size_t, szp,

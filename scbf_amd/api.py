@@ -405,6 +405,26 @@ class SuperCountingBloomIndex:  # api.rs:50-54, 210-386
         rows = [(int(a), int(b)) for a, b in zip(v[:min(cap, n.value)], c[:min(cap, n.value)])]
         return rows, q.value, d.value
 
+    def estimate_kmer_map(self, records) -> Tuple[dict, int]:
+        """filter.rs:302-335: {canonical k-mer: estimate} of every distinct k-mer
+        of the records' DnaChunks, and the number of k-mers queried."""
+        keys, vals, queried = self.estimate_kmer_map_np(records)
+        return dict(zip(keys.tolist(), vals.tolist())), queried
+
+    def estimate_kmer_map_np(self, records):
+        """The same as sorted key and estimate arrays."""
+        bases, offsets = concat_records(records) if not isinstance(records, tuple) else records
+        cap = max(1, int(offsets[-1]))
+        keys = np.zeros(cap, dtype=np.uint64)
+        vals = np.zeros(cap, dtype=np.uint64)
+        n, q = C.c_size_t(), C.c_uint64()
+        _check(lib().scb_index_estimate_kmer_map(self._h, _ptr(bases), _ptr(offsets, u64p), offsets.size - 1,
+                                                 _ptr(keys, u64p), _ptr(vals, u64p), cap, C.byref(n), C.byref(q)))
+        return keys[:n.value], vals[:n.value], q.value
+
+    def spectrum_accumulator(self) -> "SpectrumAccumulator":
+        return SpectrumAccumulator(self)
+
     def estimate_filter_only_spectrum(self, max_abundance, kmer_count_hint) -> FilterOnlySpectrumEstimate:
         return self.estimate_filter_only_spectrum_from_counter_histogram(None, max_abundance,
                                                                          kmer_count_hint)
@@ -440,12 +460,80 @@ class SuperCountingBloomIndex:  # api.rs:50-54, 210-386
         return out
 
 
+class SpectrumAccumulator:
+    """Distinct-k-mer spectrum over many batches against one index
+    (fastx.rs:255-308 with merged per-worker maps): add() batches, then rows()."""
+
+    def __init__(self, index: "SuperCountingBloomIndex"):
+        self._index = index
+        self._h = C.c_void_p()
+        _check(lib().scb_spectrum_acc_new(index._h, C.byref(self._h)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().scb_spectrum_acc_free(h)
+            self._h = C.c_void_p()
+
+    def add(self, records):
+        bases, offsets = concat_records(records) if not isinstance(records, tuple) else records
+        _check(lib().scb_spectrum_acc_add(self._h, _ptr(bases), _ptr(offsets, u64p), offsets.size - 1))
+
+    def add_device(self, d_bases: int, d_offsets: int, n: int, total_bases: int):
+        _check(lib().scb_spectrum_acc_add_device(self._h, C.c_void_p(d_bases), C.c_void_p(d_offsets), n,
+                                                 total_bases))
+
+    def rows(self) -> Tuple[ApproxSpectrum, int, int]:
+        cap = 1 << 17
+        while True:
+            v = np.zeros(cap, dtype=np.uint64)
+            c = np.zeros(cap, dtype=np.uint64)
+            n, q, d = C.c_size_t(), C.c_uint64(), C.c_uint64()
+            _check(lib().scb_spectrum_acc_rows(self._h, _ptr(v, u64p), _ptr(c, u64p), cap, C.byref(n), C.byref(q),
+                                               C.byref(d)))
+            if n.value <= cap:
+                return [(int(a), int(b)) for a, b in zip(v[:n.value], c[:n.value])], q.value, d.value
+            cap = n.value
+
+
 def _hist_arrays(h):
     if h is None:
         return None, None, 0
     hv = np.array([a for a, _ in h], dtype=np.uint64)
     hc = np.array([b for _, b in h], dtype=np.uint64)
     return hv, hc, len(h)
+
+
+def filter_only_spectrum_from_histogram(counter_histogram, n_hashes: int, max_abundance: int,
+                                        kmer_count_hint: int) -> ApproxSpectrum:
+    """Rows (a, estimated k-mers) for a = 1..=max_abundance of a given counter
+    histogram (api.rs:933-1033), host arithmetic only."""
+    hv, hc, hr = _hist_arrays(counter_histogram)
+    out = np.zeros(max(1, max_abundance), dtype=np.uint64)
+    _check(lib().scb_filter_only_spectrum_from_histogram(n_hashes, _ptr(hv, u64p), _ptr(hc, u64p), hr,
+                                                         max_abundance, kmer_count_hint, _ptr(out, u64p)))
+    return [(a, int(out[a - 1])) for a in range(1, max_abundance + 1)]
+
+
+def filter_only_count_hint_from_histogram(counter_histogram, n_hashes: int, max_abundance: int = 255) -> int:
+    """api.rs:375-385 on a given counter histogram, host arithmetic only."""
+    hv, hc, hr = _hist_arrays(counter_histogram)
+    out = C.c_uint64()
+    _check(lib().scb_filter_only_count_hint_from_histogram(n_hashes, _ptr(hv, u64p), _ptr(hc, u64p), hr,
+                                                           max_abundance, C.byref(out)))
+    return out.value
+
+
+def summarize_abundances(values, valid):
+    """(total_windows, valid_kmers, mean or None, median or None) of per-window
+    values and validity (api.rs:855-891)."""
+    v = np.ascontiguousarray(values, dtype=np.uint64)
+    ok = np.ascontiguousarray(valid, dtype=np.uint8)
+    s = ScbSummary()
+    _check(lib().scb_summarize_abundances(_ptr(v, u64p), _ptr(ok), v.size, C.byref(s)))
+    mean = None if math.isnan(s.mean_abundance) else s.mean_abundance
+    med = None if math.isnan(s.median_abundance) else s.median_abundance
+    return s.total_windows, s.valid_kmers, mean, med
 
 
 def _qrep(r):

@@ -12,6 +12,7 @@
 #include <condition_variable>
 #include <mutex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -139,7 +140,17 @@ struct HostBatch {
     size_t n() const { return offsets.size() - 1; }
 };
 
-constexpr size_t kFileBatchBytes = size_t(256) << 20;
+// Bases per batch of a FASTX file: 256 MiB, or SCB_FILE_BATCH_BYTES (at least
+// 4096) from the environment for hosts that should hold less.
+size_t file_batch_bytes() {
+    const char *e = std::getenv("SCB_FILE_BATCH_BYTES");
+    if (e && *e) {
+        char *end = nullptr;
+        unsigned long long v = std::strtoull(e, &end, 10);
+        if (end && *end == 0 && v >= 4096) return size_t(v);
+    }
+    return size_t(256) << 20;
+}
 
 // Walks a FASTX file in batches of records. split=true yields the DnaChunks of
 // split_non_actg (fastx.rs:14-19: maximal ACGT runs, those shorter than k
@@ -158,6 +169,7 @@ int for_each_batch(const char *path, Fn &&fn) {
     int parse_rc = SCB_OK;
     std::string parse_msg;
     for (int i = 0; i < 2; ++i) free_list.emplace_back(new HostBatch);
+    const size_t batch_bytes = file_batch_bytes();
     std::thread parser([&] {
         std::string hdr;
         std::vector<uint8_t> seq;
@@ -180,7 +192,7 @@ int for_each_batch(const char *path, Fn &&fn) {
             // whole records: the device scan drops every window with a non-ACGT
             // base, so a record gives the k-mers of its ACGT runs (fastx.rs:14-19)
             hb->add(seq.data(), seq.size());
-            if (hb->bases.size() >= kFileBatchBytes) {
+            if (hb->bases.size() >= batch_bytes) {
                 publish();
                 cv.notify_all();
                 if (!take_free()) return;
@@ -784,7 +796,31 @@ int scb_index_estimate_sequences_device_noalloc(const scb_index *idx, const uint
     return SCB_OK;
 }
 
-// api.rs:264-271, 855-891
+// api.rs:855-891 (summarize_abundances): mean of the valid values as u64 sum /
+// count, median of the sorted values (mean of the middle two for an even count).
+int scb_summarize_abundances(const uint64_t *values, const uint8_t *valid, size_t n, scb_summary *out) {
+    if (!out || (n && (!values || !valid))) return fail(SCB_ERR_INTERNAL_STATE, "internal state error: null argument");
+    std::vector<uint64_t> vals;
+    for (size_t i = 0; i < n; ++i)
+        if (valid[i]) vals.push_back(values[i]);
+    out->total_windows = n;
+    out->valid_kmers = vals.size();
+    if (vals.empty()) {
+        out->mean_abundance = NAN;
+        out->median_abundance = NAN;
+        return SCB_OK;
+    }
+    uint64_t sum = 0;
+    for (uint64_t x : vals) sum += x;  // wrapping, as u64 sum in a release build
+    out->mean_abundance = double(sum) / double(vals.size());
+    std::sort(vals.begin(), vals.end());
+    size_t mid = vals.size() / 2;
+    out->median_abundance =
+        vals.size() % 2 == 0 ? (double(vals[mid - 1]) + double(vals[mid])) / 2.0 : double(vals[mid]);
+    return SCB_OK;
+}
+
+// api.rs:264-271
 int scb_index_summarize_sequence(const scb_index *idx, const uint8_t *seq, size_t len, scb_summary *out) {
     size_t k = size_t(idx->cfg.k);
     size_t nw = len < k ? 0 : len + 1 - k;
@@ -793,25 +829,9 @@ int scb_index_summarize_sequence(const scb_index *idx, const uint8_t *seq, size_
     size_t got = 0;
     int rc = scb_index_estimate_sequence_abundances(idx, seq, len, v.data(), ok.data(), &got);
     if (rc) return rc;
-    std::vector<uint64_t> vals;
-    for (size_t i = 0; i < nw; ++i)
-        if (ok[i]) vals.push_back(v[i]);
+    rc = scb_summarize_abundances(v.data(), ok.data(), nw, out);
     out->sequence_len = len;
-    out->total_windows = nw;
-    out->valid_kmers = vals.size();
-    if (vals.empty()) {
-        out->mean_abundance = NAN;
-        out->median_abundance = NAN;
-        return SCB_OK;
-    }
-    uint64_t sum = 0;
-    for (uint64_t x : vals) sum += x;
-    out->mean_abundance = double(sum) / double(vals.size());
-    std::sort(vals.begin(), vals.end());
-    size_t mid = vals.size() / 2;
-    out->median_abundance =
-        vals.size() % 2 == 0 ? (double(vals[mid - 1]) + double(vals[mid])) / 2.0 : double(vals[mid]);
-    return SCB_OK;
+    return rc;
 }
 
 // api.rs:273-327
@@ -877,18 +897,68 @@ int scb_index_spectrum_device(const scb_index *idx, const uint8_t *d_bases, cons
     return SCB_OK;
 }
 
-// api.rs:337-368, 933-956, 1013-1033
-int scb_index_filter_only_spectrum(const scb_index *idx, const uint64_t *hv, const uint64_t *hc,
-                                   size_t hr, uint64_t max_abundance, uint64_t hint, uint64_t *counts) {
-    std::vector<std::pair<uint64_t, uint64_t>> rows;
-    std::vector<uint64_t> vv, cc;
-    if (!hv) {
-        int rc = index_histogram(idx, rows);
-        if (rc) return rc;
-        for (auto &r : rows) vv.push_back(r.first), cc.push_back(r.second);
-        hv = vv.data(), hc = cc.data(), hr = vv.size();
-    }
-    auto d = deconvolved_counts(hv, hc, hr, idx->cfg.n_hashes, max_abundance);
+// filter.rs:302-335 (estimate_packed_sequence) over the DnaChunks of n records:
+// every distinct canonical k-mer with its estimate, sorted by key.
+int scb_index_estimate_kmer_map(const scb_index *idx, const uint8_t *bases, const uint64_t *offsets, size_t n,
+                                uint64_t *keys, uint64_t *values, size_t cap, size_t *n_keys, uint64_t *queried) {
+    int rc = check_offsets(offsets, n);
+    if (rc) return rc;
+    std::vector<std::pair<uint64_t, uint32_t>> pairs;
+    uint64_t q = 0;
+    Status s = idx->engine->kmer_map_host(bases, offsets, n, pairs, q);
+    if (!s.good()) return fail(s);
+    if (cap && (!keys || !values)) return fail(SCB_ERR_INTERNAL_STATE, "internal state error: null output buffers");
+    for (size_t i = 0; i < pairs.size() && i < cap; ++i) keys[i] = pairs[i].first, values[i] = pairs[i].second;
+    if (n_keys) *n_keys = pairs.size();
+    if (queried) *queried = q;
+    return SCB_OK;
+}
+
+int scb_spectrum_acc_new(const scb_index *idx, scb_spectrum_acc **out) {
+    if (!idx || !out) return fail(SCB_ERR_INTERNAL_STATE, "internal state error: null argument");
+    std::unique_ptr<scb_spectrum_acc> a(new scb_spectrum_acc);
+    a->idx = idx;
+    Status s = idx->engine->keyset_create(a->ks);
+    if (!s.good()) return fail(s);
+    *out = a.release();
+    return SCB_OK;
+}
+
+void scb_spectrum_acc_free(scb_spectrum_acc *acc) { delete acc; }
+
+int scb_spectrum_acc_add(scb_spectrum_acc *acc, const uint8_t *bases, const uint64_t *offsets, size_t n) {
+    int rc = check_offsets(offsets, n);
+    if (rc) return rc;
+    Status s = acc->idx->engine->spectrum_accumulate_host(bases, offsets, n, *acc->ks);
+    return s.good() ? SCB_OK : fail(s);
+}
+
+int scb_spectrum_acc_add_device(scb_spectrum_acc *acc, const uint8_t *d_bases, const uint64_t *d_offsets,
+                                size_t n, uint64_t total_bases) {
+    if (total_bases >= scb::kMaxBatchBases)
+        return fail(SCB_ERR_INTERNAL_STATE, "internal state error: device batch exceeds 2^32 bases");
+    Status s = acc->idx->engine->spectrum_accumulate(scb::DeviceBatch{d_bases, d_offsets, n, total_bases}, *acc->ks);
+    return s.good() ? SCB_OK : fail(s);
+}
+
+int scb_spectrum_acc_rows(const scb_spectrum_acc *acc, uint64_t *values, uint64_t *counts, size_t cap,
+                          size_t *n_rows, uint64_t *queried, uint64_t *distinct) {
+    scb::SpectrumResult r;
+    Status s = acc->idx->engine->keyset_rows(*acc->ks, r);
+    if (!s.good()) return fail(s);
+    for (size_t i = 0; i < r.rows.size() && i < cap; ++i) values[i] = r.rows[i].first, counts[i] = r.rows[i].second;
+    if (n_rows) *n_rows = r.rows.size();
+    if (queried) *queried = r.queried;
+    if (distinct) *distinct = r.distinct;
+    return SCB_OK;
+}
+
+// api.rs:933-956, 1013-1033 on a given histogram
+int scb_filter_only_spectrum_from_histogram(uint64_t n_hashes, const uint64_t *hv, const uint64_t *hc, size_t hr,
+                                            uint64_t max_abundance, uint64_t hint, uint64_t *counts) {
+    if ((hr && (!hv || !hc)) || (max_abundance && !counts))
+        return fail(SCB_ERR_INTERNAL_STATE, "internal state error: null argument");
+    auto d = deconvolved_counts(hv, hc, hr, n_hashes, max_abundance);
     double total = 0.0;
     for (double x : d) total += x;
     if (total <= 0.0) {
@@ -909,6 +979,31 @@ int scb_index_filter_only_spectrum(const scb_index *idx, const uint64_t *hv, con
     return SCB_OK;
 }
 
+// api.rs:375-385 on a given histogram
+int scb_filter_only_count_hint_from_histogram(uint64_t n_hashes, const uint64_t *hv, const uint64_t *hc, size_t hr,
+                                              uint64_t max_abundance, uint64_t *hint) {
+    if ((hr && (!hv || !hc)) || !hint) return fail(SCB_ERR_INTERNAL_STATE, "internal state error: null argument");
+    auto d = deconvolved_counts(hv, hc, hr, n_hashes, max_abundance);
+    double total = 0.0;
+    for (double x : d) total += x;
+    *hint = f64_as_u64(std::fmax(std::round(total), 1.0));
+    return SCB_OK;
+}
+
+// api.rs:337-368: the index's own counter histogram unless one is given
+int scb_index_filter_only_spectrum(const scb_index *idx, const uint64_t *hv, const uint64_t *hc,
+                                   size_t hr, uint64_t max_abundance, uint64_t hint, uint64_t *counts) {
+    std::vector<std::pair<uint64_t, uint64_t>> rows;
+    std::vector<uint64_t> vv, cc;
+    if (!hv) {
+        int rc = index_histogram(idx, rows);
+        if (rc) return rc;
+        for (auto &r : rows) vv.push_back(r.first), cc.push_back(r.second);
+        hv = vv.data(), hc = cc.data(), hr = vv.size();
+    }
+    return scb_filter_only_spectrum_from_histogram(idx->cfg.n_hashes, hv, hc, hr, max_abundance, hint, counts);
+}
+
 // api.rs:370-385
 int scb_index_filter_only_count_hint(const scb_index *idx, const uint64_t *hv, const uint64_t *hc,
                                      size_t hr, uint64_t max_abundance, uint64_t *hint) {
@@ -920,11 +1015,7 @@ int scb_index_filter_only_count_hint(const scb_index *idx, const uint64_t *hv, c
         for (auto &r : rows) vv.push_back(r.first), cc.push_back(r.second);
         hv = vv.data(), hc = cc.data(), hr = vv.size();
     }
-    auto d = deconvolved_counts(hv, hc, hr, idx->cfg.n_hashes, max_abundance);
-    double total = 0.0;
-    for (double x : d) total += x;
-    *hint = f64_as_u64(std::fmax(std::round(total), 1.0));
-    return SCB_OK;
+    return scb_filter_only_count_hint_from_histogram(idx->cfg.n_hashes, hv, hc, hr, max_abundance, hint);
 }
 
 // ---- FASTX pipelines ----------------------------------------------------------
@@ -941,20 +1032,20 @@ int scb_estimate_spectrum_from_fastx_pair(const char *index_path, const char *qu
     std::unique_ptr<scb_index, void (*)(scb_index *)> guard(idx, scb_index_free);
     double tq = now_s();
     uint64_t queried = 0, distinct = 0;
-    std::vector<std::pair<uint64_t, uint64_t>> rows;
-    // Distinct keys are merged across batches by running every query chunk in
-    // one spectrum call per batch of the file; keys shared between batches are
-    // only counted once because the spectrum engine keeps a per-call key set.
-    HostBatch all;
+    // fastx.rs:255-308: the query file is read batch by batch; one device key
+    // set spans all batches, so a key is counted once however many batches
+    // hold it (the merged per-worker maps of fastx.rs:392-400)
+    scb::KeySetPtr ks;
+    Status s = idx->engine->keyset_create(ks);
+    if (!s.good()) return fail(s);
     rc = for_each_batch(query_path, [&](HostBatch &hb) -> int {
-        all.bases.insert(all.bases.end(), hb.bases.begin(), hb.bases.end());
-        uint64_t base = all.offsets.back();
-        for (size_t i = 1; i < hb.offsets.size(); ++i) all.offsets.push_back(base + hb.offsets[i]);
-        return SCB_OK;
+        Status bs = idx->engine->spectrum_accumulate_host(hb.bases.data(), hb.offsets.data(), hb.n(), *ks);
+        return bs.good() ? SCB_OK : fail(bs);
     });
     if (rc) return rc;
+    double tqs = now_s();
     scb::SpectrumResult r;
-    Status s = idx->engine->spectrum_host(all.bases.data(), all.offsets.data(), all.n(), r);
+    s = idx->engine->keyset_rows(*ks, r);
     if (!s.good()) return fail(s);
     queried = r.queried;
     distinct = r.distinct;
@@ -971,8 +1062,8 @@ int scb_estimate_spectrum_from_fastx_pair(const char *index_path, const char *qu
         stats->counter_slots = br.counter_slots;
         stats->blocks = br.blocks;
         stats->build_seconds = br.build_seconds;
-        stats->query_seconds = tqe - tq;
-        stats->spectrum_seconds = 0.0;
+        stats->query_seconds = tqs - tq;
+        stats->spectrum_seconds = tqe - tqs;
         stats->total_seconds = now_s() - t_total;
     }
     return SCB_OK;

@@ -70,6 +70,15 @@ struct Tuning {
 
 class Workspace;  // device scratch buffers + stream, one per concurrent call
 
+// Distinct canonical k-mers accumulated over several spectrum calls (the
+// per-worker maps of fastx.rs:255-308 merged as fastx.rs:392-400), held in
+// device memory; created by Engine::keyset_create.
+class KeySet;
+struct KeySetDeleter {
+    void operator()(KeySet *k) const;
+};
+using KeySetPtr = std::unique_ptr<KeySet, KeySetDeleter>;
+
 // Counter table resident in HBM in physical (.scb payload) block order
 // P(b) = (b & (shard_count-1)) * subblocks + (b >> log2(shard_count)).
 class Engine {
@@ -93,6 +102,16 @@ class Engine {
     Status query_noalloc(const DeviceBatch &b, Method method, void *d_values, uint32_t *d_valid_bits,
                          QueryResult &r) const;
     Status spectrum(const DeviceBatch &b, SpectrumResult &r) const;
+    // Spectrum over several batches: each call adds the batch's distinct keys
+    // to ks (keys already in it are not counted again); keyset_rows gives the
+    // rows, the distinct keys and the k-mers queried so far.
+    Status keyset_create(KeySetPtr &out) const;
+    Status spectrum_accumulate(const DeviceBatch &b, KeySet &ks) const;
+    Status keyset_rows(const KeySet &ks, SpectrumResult &r) const;
+    // filter.rs:302-335: (canonical k-mer, estimate) of every distinct k-mer of
+    // the batch (first occurrence; every occurrence has the same estimate)
+    Status kmer_map(const DeviceBatch &b, std::vector<std::pair<uint64_t, uint32_t>> &pairs,
+                    uint64_t &queried) const;
     Status counter_histogram(std::vector<std::pair<uint64_t, uint64_t>> &rows) const;
 
     // Host <-> HBM copies of the whole table (physical order), and of batches.
@@ -106,8 +125,13 @@ class Engine {
     // values/valid indexed by position; either may be null.
     Status query_host(const uint8_t *bases, const uint64_t *offsets, uint64_t n, Method method,
                       uint64_t *values, uint8_t *valid, QueryResult &r) const;
+    // any size: inputs beyond one device batch are counted chunk by chunk in a key set
     Status spectrum_host(const uint8_t *bases, const uint64_t *offsets, uint64_t n,
                          SpectrumResult &r) const;
+    Status spectrum_accumulate_host(const uint8_t *bases, const uint64_t *offsets, uint64_t n, KeySet &ks) const;
+    // pairs sorted by key
+    Status kmer_map_host(const uint8_t *bases, const uint64_t *offsets, uint64_t n,
+                         std::vector<std::pair<uint64_t, uint32_t>> &pairs, uint64_t &queried) const;
 
    private:
     Engine() = default;
@@ -125,6 +149,9 @@ class Engine {
     void release(std::unique_ptr<Workspace> ws) const;
     Status run_query(Workspace &ws, const DeviceBatch &b, Method method, void *d_values, uint32_t *d_valid_bits,
                      QueryResult &r) const;
+    Status spectrum_checks(const DeviceBatch &b) const;
+    // hist: the kernel-side histogram targets (HistOut) or null with emit
+    Status spectrum_pass(Workspace &ws, const DeviceBatch &b, const void *hist, bool emit) const;
 };
 
 // Device time of the stages of the calling thread's last insert/query call,

@@ -22,6 +22,11 @@ struct scb_index {
     scb::Method method = scb::kBatched;
 };
 
+struct scb_spectrum_acc {
+    const scb_index *idx = nullptr;
+    scb::KeySetPtr ks;
+};
+
 namespace scb_host {
 // Records msg as the calling thread's last error (scb_last_error) and returns code.
 int fail(int code, const std::string &msg);

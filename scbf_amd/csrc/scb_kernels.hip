@@ -72,6 +72,8 @@ enum StatSlot {
     kStOverflow = 8,
     kStDistinct = 9,
     kStGlobalKmers = 10,
+    kStBadOffsets = 11,
+    kStEmitted = 12,
     kStCount = 16
 };
 
@@ -362,9 +364,16 @@ struct ScanArgs {
 };
 
 // For every scan tile, the first record whose start lies after the tile's left
-// edge t*kTile - kLeft (a binary search per tile, all tiles in parallel).
-__global__ void k_tile_r0(const uint64_t *offsets, uint64_t n_reads, uint32_t ntiles, uint32_t *r0) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+// edge t*kTile - kLeft (a binary search per tile, all tiles in parallel); and
+// the offsets contract of a batch (offsets[0] = 0, non-decreasing, offsets[n] =
+// n_bases), checked by one thread per record: a violation sets
+// stats[kStBadOffsets] and the call returns an error.
+__global__ void k_tile_r0(const uint64_t *offsets, uint64_t n_reads, uint64_t n_bases, uint32_t ntiles,
+                          uint32_t *r0, uint64_t *stats) {
+    const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    bool bad = t < n_reads && offsets[t] > offsets[t + 1];
+    if (t == 0) bad |= offsets[0] != 0 || offsets[n_reads] != n_bases;
+    if (bad) atomicOr(reinterpret_cast<unsigned long long *>(stats) + kStBadOffsets, 1ull);
     if (t >= ntiles) return;
     int64_t lb = int64_t(t) * kTile - kLeft;
     uint64_t lo = 0, hi = n_reads;
@@ -532,7 +541,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
             int64_t o = ofirst;
             while (r < a.n_reads && o < LB + kSpan) {
                 const int li = int(o - LB);
-                atomicOr(&s_brk[li >> 5], 1u << (li & 31));
+                if (li >= 0) atomicOr(&s_brk[li >> 5], 1u << (li & 31));  // (offsets that decrease: an error)
                 r += kTileThreads;
                 if (r < a.n_reads) o = int64_t(a.offsets[r]);
             }
@@ -1798,20 +1807,33 @@ __global__ __launch_bounds__(1024) void k_spectrum_plan(const uint32_t *bin_coun
     if (threadIdx.x == 0) stats[kStGlobalKmers] = s_sum;
 }
 
+// Histogram of estimates: values below kHistSmall are staged in LDS by the
+// caller, values below dense_n go to a dense global array, larger (32-bit)
+// values to an open-addressing table of kHistBigCap entries.
+struct HistOut {
+    unsigned long long *dense;
+    uint32_t dense_n;
+    uint32_t *big_keys;
+    unsigned long long *big_counts;
+    uint64_t *stats;  // stats[kStOverflow] is raised when big_keys is full
+};
+
 struct SpecArgs {
     BinArgs b;
     const uint32_t *bin_kmers;
     unsigned long long *gset;
     uint32_t glog2;
-    unsigned long long *dense;   // histogram of estimates < dense_n
-    uint32_t dense_n;
-    uint32_t *big_keys;          // 32-bit estimates >= dense_n
-    unsigned long long *big_counts;
+    HistOut h;
     uint32_t stage;              // counters staged in LDS
+    // non-null: the first occurrence of each key in this call is appended as
+    // (key, estimate) at the slot counted in stats[kStEmitted], instead of
+    // being counted into the histogram
+    uint64_t *emit_keys;
+    uint32_t *emit_vals;
+    uint64_t emit_cap;
 };
 
-template <int BITS>
-__device__ __forceinline__ void spec_hist_add(const SpecArgs &a, uint32_t *s_hist, uint32_t v) {
+__device__ __forceinline__ void spec_hist_add(const HistOut &a, uint32_t *s_hist, uint32_t v) {
     if (v < kHistSmall) {
         atomicAdd(&s_hist[v], 1u);
     } else if (v < a.dense_n) {
@@ -1834,7 +1856,25 @@ __device__ __forceinline__ void spec_hist_add(const SpecArgs &a, uint32_t *s_his
             }
             hsh = (hsh + 1) & (kHistBigCap - 1);
         }
-        atomicAdd((unsigned long long *)&a.b.stats[kStOverflow], 1ull);
+        atomicAdd((unsigned long long *)&a.stats[kStOverflow], 1ull);
+    }
+}
+
+// Appends (key, v) to the emit arrays: the emitting lanes of a wave take
+// consecutive slots with one atomic add by the lowest of them.
+__device__ __forceinline__ void emit_pair(const SpecArgs &a, uint64_t key, uint32_t v) {
+    const uint64_t m = __ballot(1);
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = uint32_t(__ffsll((unsigned long long)m) - 1);
+    const uint32_t rank = uint32_t(__popcll(m & ((uint64_t(1) << lane) - 1)));
+    unsigned long long base = 0;
+    if (lane == leader)
+        base = atomicAdd((unsigned long long *)&a.b.stats[kStEmitted], (unsigned long long)__popcll(m));
+    base = __shfl(base, int(leader), 64);
+    const uint64_t i = base + rank;
+    if (i < a.emit_cap) {
+        a.emit_keys[i] = key;
+        a.emit_vals[i] = v;
     }
 }
 
@@ -1907,7 +1947,8 @@ __global__ __launch_bounds__(kSpectrumThreads) void k_bin_spectrum(SpecArgs sa, 
                         ++queried;
                         if (fresh) {
                             ++distinct;
-                            spec_hist_add<BITS>(sa, s_hist, ab);
+                            if (sa.emit_keys) emit_pair(sa, key, ab);
+                            else spec_hist_add(sa.h, s_hist, ab);
                         }
                     }
                 },
@@ -1916,7 +1957,7 @@ __global__ __launch_bounds__(kSpectrumThreads) void k_bin_spectrum(SpecArgs sa, 
         __syncthreads();
     }
     for (uint32_t i = tid; i < kHistSmall; i += nthr)
-        if (s_hist[i]) atomicAdd(&sa.dense[i], (unsigned long long)s_hist[i]);
+        if (s_hist[i]) atomicAdd(&sa.h.dense[i], (unsigned long long)s_hist[i]);
     uint64_t vals[2] = {queried, distinct};
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
@@ -1932,6 +1973,42 @@ __global__ __launch_bounds__(kSpectrumThreads) void k_bin_spectrum(SpecArgs sa, 
             for (uint32_t i = 0; i < nthr / 64; ++i) t[q] += s_red[q][i];
         if (t[0]) atomicAdd(&st[kStQueried], t[0]);
         if (t[1]) atomicAdd(&st[kStDistinct], t[1]);
+    }
+}
+
+// Key sets kept across spectrum calls (the per-worker maps of fastx.rs:255-308
+// merged first-wins, fastx.rs:392-400): the pairs one call emitted go into a
+// global open-addressing set; a key the set did not hold yet is counted into
+// the set's histogram. A key's estimate is the same in every call (the index
+// is frozen), so first-wins needs no order.
+__global__ __launch_bounds__(256) void k_keyset_merge(const uint64_t *keys, const uint32_t *vals, uint64_t n,
+                                                      unsigned long long *set, uint32_t log2cap, HistOut h,
+                                                      unsigned long long *distinct) {
+    __shared__ uint32_t s_hist[kHistSmall];
+    __shared__ unsigned long long s_new;
+    for (uint32_t i = threadIdx.x; i < kHistSmall; i += blockDim.x) s_hist[i] = 0;
+    if (threadIdx.x == 0) s_new = 0;
+    __syncthreads();
+    uint32_t fresh = 0;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        if (set_insert(set, log2cap, keys[i])) {
+            ++fresh;
+            spec_hist_add(h, s_hist, vals[i]);
+        }
+    if (fresh) atomicAdd(&s_new, (unsigned long long)fresh);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kHistSmall; i += blockDim.x)
+        if (s_hist[i]) atomicAdd(&h.dense[i], (unsigned long long)s_hist[i]);
+    if (threadIdx.x == 0 && s_new) atomicAdd(distinct, s_new);
+}
+
+// Moves every key of a full set into a larger one.
+__global__ __launch_bounds__(256) void k_keyset_rehash(const unsigned long long *old, uint64_t old_cap,
+                                                       unsigned long long *set, uint32_t log2cap) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < old_cap;
+         i += uint64_t(gridDim.x) * blockDim.x) {
+        const unsigned long long k = old[i];
+        if (k != kEmptyKey) (void)set_insert(set, log2cap, k);
     }
 }
 
@@ -2035,6 +2112,55 @@ struct HostBuf {
 
 }  // namespace
 
+namespace {
+void fill(uint32_t *p, uint64_t n, uint32_t v, hipStream_t st);
+
+uint32_t spectrum_dense_n(const Geometry &g) { return g.bits == 8 ? 256 : (g.bits == 16 ? 65536 : kHistSmall); }
+
+// Device histogram tables of one spectrum (per call, or kept by a KeySet).
+struct HistTables {
+    DevBuf<unsigned long long> dense, big_counts;
+    DevBuf<uint32_t> big_keys;
+    uint32_t dense_n = 0;
+    hipError_t reserve(const Geometry &g) {
+        dense_n = spectrum_dense_n(g);
+        hipError_t e;
+        if ((e = dense.reserve(dense_n)) != hipSuccess) return e;
+        if ((e = big_keys.reserve(g.bits == 32 ? kHistBigCap : 1)) != hipSuccess) return e;
+        return big_counts.reserve(g.bits == 32 ? kHistBigCap : 1);
+    }
+    void clear(const Geometry &g, hipStream_t st) {
+        fill(reinterpret_cast<uint32_t *>(dense.p), uint64_t(dense_n) * 2, 0, st);
+        if (g.bits == 32) {
+            fill(big_keys.p, kHistBigCap, 0, st);
+            fill(reinterpret_cast<uint32_t *>(big_counts.p), 2 * kHistBigCap, 0, st);
+        }
+    }
+    HistOut out(uint64_t *stats) const { return HistOut{dense.p, dense_n, big_keys.p, big_counts.p, stats}; }
+    // rows (value, count) sorted by value; synchronizes the stream
+    Status rows(const Geometry &g, hipStream_t st, std::vector<std::pair<uint64_t, uint64_t>> &out) const {
+        std::vector<unsigned long long> d(dense_n);
+        std::vector<uint32_t> keys;
+        std::vector<unsigned long long> cnts;
+        SCB_HIP(hipMemcpyAsync(d.data(), dense.p, uint64_t(dense_n) * 8, hipMemcpyDeviceToHost, st));
+        if (g.bits == 32) {
+            keys.resize(kHistBigCap);
+            cnts.resize(kHistBigCap);
+            SCB_HIP(hipMemcpyAsync(keys.data(), big_keys.p, kHistBigCap * 4, hipMemcpyDeviceToHost, st));
+            SCB_HIP(hipMemcpyAsync(cnts.data(), big_counts.p, kHistBigCap * 8, hipMemcpyDeviceToHost, st));
+        }
+        SCB_HIP(hipStreamSynchronize(st));
+        out.clear();
+        for (uint64_t v = 0; v < dense_n; ++v)
+            if (d[v]) out.emplace_back(v, d[v]);
+        for (uint32_t i = 0; i < keys.size(); ++i)
+            if (keys[i]) out.emplace_back(keys[i], cnts[i]);
+        std::sort(out.begin(), out.end());
+        return Status::ok();
+    }
+};
+}  // namespace
+
 class Workspace {
    public:
     hipStream_t stream = nullptr;
@@ -2053,6 +2179,9 @@ class Workspace {
     DevBuf<uint32_t> hist_big_keys;
     DevBuf<uint32_t> spec_bin_kmers;
     DevBuf<unsigned long long> spec_gset;
+    DevBuf<uint64_t> emit_keys;    // (key, estimate) pairs a spectrum pass emits
+    DevBuf<uint32_t> emit_vals;
+    HistTables hist;               // spectrum histogram
     uint64_t *h_stats = nullptr;   // pinned
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     // A view workspace runs on the null stream, has no events, and its device
@@ -2067,6 +2196,8 @@ class Workspace {
         for (auto *b : {&rec, &sorted, &work, &stats, &plan_blk, &h_offsets_dev}) b->owned = false;
         h_bases_dev.owned = out_dev.owned = false;
         hist_dense.owned = hist_big_counts.owned = spec_gset.owned = false;
+        emit_keys.owned = emit_vals.owned = false;
+        hist.dense.owned = hist.big_counts.owned = hist.big_keys.owned = false;
     }
     ~Workspace() {
         for (auto &e : ev)
@@ -2257,7 +2388,7 @@ Status Engine::create(const Geometry &g, std::unique_ptr<Engine> &out) {
     e->zero_pending_ = true;
     // keep query bins within LDS at every width (u32 counters: 2^15 slots = 128 KiB)
     e->tune_.insert_bin_log = std::min<uint32_t>(15, g.cse);
-    e->tune_.query_bin_log = std::min<uint32_t>(g.bits == 8 ? 15 : 15, g.cse);
+    e->tune_.query_bin_log = std::min<uint32_t>(15, g.cse);
     out = std::move(e);
     return Status::ok();
 }
@@ -2363,8 +2494,8 @@ Status launch_scan(Launch &L, const DeviceBatch &b, uint64_t seg_cap, uint32_t *
     bool wide = L.geo.win > 8;
     hipStream_t st = L.ws.stream;
     SCB_HIP(L.ws.tile_r0.reserve(tiles));
-    hipLaunchKernelGGL(k_tile_r0, dim3((tiles + 255) / 256), dim3(256), 0, st, b.d_offsets, b.n_reads, tiles,
-                       L.ws.tile_r0.p);
+    hipLaunchKernelGGL(k_tile_r0, dim3(unsigned((std::max<uint64_t>(tiles, b.n_reads) + 255) / 256)), dim3(256), 0,
+                       st, b.d_offsets, b.n_reads, b.n_bases, tiles, L.ws.tile_r0.p, L.ws.stats.p);
     a.tile_r0 = L.ws.tile_r0.p;
     a.ntiles = tiles;
     // persistent tiles: a few resident workgroups per CU, each prefetching its next tile
@@ -2487,6 +2618,9 @@ Status finish(Workspace &ws) {
     const int direct[kSpCount] = {kStValid, kStSmers, kStQueried, kStPositive, kStChecksum};
     for (int q = 0; q < kSpCount; ++q)
         for (int i = 0; i < kSeg; ++i) ws.h_stats[direct[q]] += ws.h_stats[kStSpread + q * kSeg + i];
+    if (ws.h_stats[kStBadOffsets])
+        return Status{6, "internal state error: batch offsets must start at 0, never decrease and end at the "
+                         "batch's base count"};
     return Status::ok();
 }
 
@@ -2805,75 +2939,104 @@ Status Engine::query_noalloc(const DeviceBatch &b, Method method, void *d_values
     return s;
 }
 
-Status Engine::spectrum(const DeviceBatch &b, SpectrumResult &r) const {
-    r = SpectrumResult{};
+
+// Distinct canonical k-mers accumulated over several spectrum calls: a device
+// open-addressing set of the keys (grown and rehashed before it would pass
+// half full) and the histogram of their estimates.
+class KeySet {
+   public:
+    int device = 0;
+    DevBuf<unsigned long long> set;
+    uint32_t log2 = 0;
+    uint64_t size = 0;     // keys held
+    uint64_t queried = 0;  // k-mers queried over all calls
+    HistTables hist;
+    DevBuf<uint64_t> stats;  // [0] keys held, [kStOverflow] big-value table overflow
+};
+
+void KeySetDeleter::operator()(KeySet *k) const { delete k; }
+
+Status Engine::keyset_create(KeySetPtr &out) const {
+    SCB_HIP(hipSetDevice(device_));
+    KeySetPtr ks(new KeySet);
+    ks->device = device_;
+    ks->log2 = 16;
+    SCB_HIP(ks->set.reserve(uint64_t(1) << ks->log2));
+    SCB_HIP(ks->hist.reserve(geo_));
+    SCB_HIP(ks->stats.reserve(kStatsLen));
+    fill(reinterpret_cast<uint32_t *>(ks->set.p), (uint64_t(1) << ks->log2) * 2, 0xFFFFFFFFu, nullptr);
+    ks->hist.clear(geo_, nullptr);
+    fill(reinterpret_cast<uint32_t *>(ks->stats.p), 2 * kStatsLen, 0, nullptr);
+    SCB_HIP(hipGetLastError());
+    SCB_HIP(hipStreamSynchronize(nullptr));
+    out = std::move(ks);
+    return Status::ok();
+}
+
+Status Engine::spectrum_checks(const DeviceBatch &b) const {
     Status s = check_batch(geo_, b);
     if (!s.good()) return s;
     s = ensure_zeroed();
     if (!s.good()) return s;
-    if (b.n_bases < geo_.k || b.n_reads == 0) return Status::ok();
     if (geo_.nb > (uint64_t(1) << 27))
         return Status{6, "internal state error: the GPU spectrum supports at most 2^27 counter blocks"};
-    SCB_HIP(hipSetDevice(device_));
-    auto wsp = acquire();
-    if (!wsp) return Status{8, "device error: cannot create a workspace"};
-    Workspace &ws = *wsp;
+    if (geo_.bse >= 32)  // slot addresses inside a block are 32-bit in the spectrum kernel
+        return Status{6, "internal state error: the GPU spectrum supports blocks of at most 2^31 slots"};
+    return Status::ok();
+}
+
+// One spectrum pass over a batch (filter.rs:407-460 per record): records
+// binned by counter bin, each bin's keys deduplicated in LDS (or, for split or
+// oversized bins, in a per-call global set); first occurrences are counted
+// into `hist` or, with emit, appended as (key, estimate) to ws.emit_*. The
+// stream is synchronized on return; h_stats hold the counts.
+Status Engine::spectrum_pass(Workspace &ws, const DeviceBatch &b, const void *hist, bool emit) const {
     Launch L{geo_, dev_geo(geo_), ws, d_table_, device_};
     hipStream_t st = ws.stream;
     const uint32_t bin_log = geo_.bse <= 14 ? std::min<uint32_t>(14, geo_.cse) : geo_.bse;
     const uint32_t stage = (geo_.bse <= 14 && (uint64_t(1) << bin_log) * geo_.bytes >= 4) ? 1 : 0;
     const uint32_t per_item = std::max<uint32_t>(1, std::min<uint32_t>(tune_.max_records_per_item, kMaxItemRecords));
-    uint64_t nbins = uint64_t(1) << (geo_.cse - bin_log);
-    const uint32_t dense_n = geo_.bits == 8 ? 256 : (geo_.bits == 16 ? 65536 : kHistSmall);
-    do {
-        hipError_t e;
-        if ((e = ws.spec_bin_kmers.reserve(nbins)) != hipSuccess ||
-            (e = ws.hist_dense.reserve(dense_n)) != hipSuccess ||
-            (e = ws.hist_big_keys.reserve(geo_.bits == 32 ? kHistBigCap : 1)) != hipSuccess ||
-            (e = ws.hist_big_counts.reserve(geo_.bits == 32 ? kHistBigCap : 1)) != hipSuccess) {
-            s = Status{8, std::string("device error: ") + hipGetErrorString(e)};
-            break;
-        }
-        uint64_t nb_unused = 0;
-        s = batched_front(L, b, bin_log, per_item, nullptr, nb_unused, ws.spec_bin_kmers.p);
-        if (!s.good()) break;
-        hipLaunchKernelGGL(k_spectrum_plan, dim3(1), dim3(1024), 0, st, ws.bin_count.p, ws.spec_bin_kmers.p,
-                           nbins, per_item, ws.stats.p);
-        if ((s = finish(ws)).code) break;
-        uint64_t gk = ws.h_stats[kStGlobalKmers];
-        uint32_t glog2 = 10;
-        while ((uint64_t(1) << glog2) < 2 * gk + 2) ++glog2;
-        if ((e = ws.spec_gset.reserve(uint64_t(1) << glog2)) != hipSuccess) {
-            s = Status{8, std::string("device error: ") + hipGetErrorString(e)};
-            break;
-        }
-        fill(reinterpret_cast<uint32_t *>(ws.spec_gset.p), (uint64_t(1) << glog2) * 2, 0xFFFFFFFFu, st);
-        fill(reinterpret_cast<uint32_t *>(ws.hist_dense.p), uint64_t(dense_n) * 2, 0, st);
-        if (geo_.bits == 32) {
-            fill(ws.hist_big_keys.p, kHistBigCap, 0, st);
-            fill(reinterpret_cast<uint32_t *>(ws.hist_big_counts.p), 2 * kHistBigCap, 0, st);
-        }
-        SpecArgs sa{};
-        sa.b.sorted = ws.final_sorted;
-        sa.b.bin_off = ws.bin_off.p;
-        sa.b.bin_count = ws.bin_count.p;
-        sa.b.work = ws.work.p;
-        sa.b.stats = ws.stats.p;
-        sa.b.pk = ws.pk.p;
-        sa.b.table = d_table_;
-        sa.b.bin_log = bin_log;
-        sa.b.per_item = per_item;
-        sa.bin_kmers = ws.spec_bin_kmers.p;
-        sa.gset = ws.spec_gset.p;
-        sa.glog2 = glog2;
-        sa.dense = ws.hist_dense.p;
-        sa.dense_n = dense_n;
-        sa.big_keys = ws.hist_big_keys.p;
-        sa.big_counts = ws.hist_big_counts.p;
-        sa.stage = stage;
-        size_t lds = (stage ? (size_t(1) << bin_log) * geo_.bytes : 0) + kSetCap * 8 + kHistSmall * 4;
-        const int thr = kSpectrumThreads;
-        bool wide = geo_.win > 8;
+    const uint64_t nbins = uint64_t(1) << (geo_.cse - bin_log);
+    SCB_HIP(ws.spec_bin_kmers.reserve(nbins));
+    if (emit) {
+        SCB_HIP(ws.emit_keys.reserve(b.n_bases + 1));
+        SCB_HIP(ws.emit_vals.reserve(b.n_bases + 1));
+    }
+    uint64_t nb_unused = 0;
+    Status s = batched_front(L, b, bin_log, per_item, nullptr, nb_unused, ws.spec_bin_kmers.p);
+    if (!s.good()) return s;
+    hipLaunchKernelGGL(k_spectrum_plan, dim3(1), dim3(1024), 0, st, ws.bin_count.p, ws.spec_bin_kmers.p, nbins,
+                       per_item, ws.stats.p);
+    if ((s = finish(ws)).code) return s;
+    const uint64_t gk = ws.h_stats[kStGlobalKmers];
+    uint32_t glog2 = 10;
+    while ((uint64_t(1) << glog2) < 2 * gk + 2) ++glog2;
+    SCB_HIP(ws.spec_gset.reserve(uint64_t(1) << glog2));
+    fill(reinterpret_cast<uint32_t *>(ws.spec_gset.p), (uint64_t(1) << glog2) * 2, 0xFFFFFFFFu, st);
+    SpecArgs sa{};
+    sa.b.sorted = ws.final_sorted;
+    sa.b.bin_off = ws.bin_off.p;
+    sa.b.bin_count = ws.bin_count.p;
+    sa.b.work = ws.work.p;
+    sa.b.stats = ws.stats.p;
+    sa.b.pk = ws.pk.p;
+    sa.b.table = d_table_;
+    sa.b.bin_log = bin_log;
+    sa.b.per_item = per_item;
+    sa.bin_kmers = ws.spec_bin_kmers.p;
+    sa.gset = ws.spec_gset.p;
+    sa.glog2 = glog2;
+    if (hist) sa.h = *static_cast<const HistOut *>(hist);
+    sa.h.stats = ws.stats.p;
+    sa.stage = stage;
+    if (emit) {
+        sa.emit_keys = ws.emit_keys.p;
+        sa.emit_vals = ws.emit_vals.p;
+        sa.emit_cap = b.n_bases + 1;
+    }
+    size_t lds = (stage ? (size_t(1) << bin_log) * geo_.bytes : 0) + kSetCap * 8 + kHistSmall * 4;
+    const int thr = kSpectrumThreads;
+    const bool wide = geo_.win > 8;
 #define SCB_SPEC(BITS, WC)                                                                      \
     do {                                                                                        \
         s = set_lds(k_bin_spectrum<BITS, WC>, lds);                                             \
@@ -2881,41 +3044,136 @@ Status Engine::spectrum(const DeviceBatch &b, SpectrumResult &r) const {
         int grid = persistent_grid(k_bin_spectrum<BITS, WC>, thr, lds, device_);                \
         hipLaunchKernelGGL((k_bin_spectrum<BITS, WC>), dim3(grid), dim3(thr), lds, st, sa, L.dg); \
     } while (0)
-        switch (geo_.bits) {
-            case 8: if (wide) SCB_SPEC(8, 32); else SCB_SPEC(8, 8); break;
-            case 16: if (wide) SCB_SPEC(16, 32); else SCB_SPEC(16, 8); break;
-            default: if (wide) SCB_SPEC(32, 32); else SCB_SPEC(32, 8); break;
-        }
+    switch (geo_.bits) {
+        case 8: if (wide) SCB_SPEC(8, 32); else SCB_SPEC(8, 8); break;
+        case 16: if (wide) SCB_SPEC(16, 32); else SCB_SPEC(16, 8); break;
+        default: if (wide) SCB_SPEC(32, 32); else SCB_SPEC(32, 8); break;
+    }
 #undef SCB_SPEC
-        if (!s.good()) break;
-        if ((e = hipGetLastError()) != hipSuccess) {
-            s = Status{8, std::string("device error: k_bin_spectrum: ") + hipGetErrorString(e)};
+    if (!s.good()) return s;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return Status{8, std::string("device error: k_bin_spectrum: ") + hipGetErrorString(e)};
+    mark(ws, 3);
+    if ((s = finish(ws)).code) return s;
+    collect_times(ws, true);  // the bin stage includes the plan's host round trip
+    if (ws.h_stats[kStOverflow]) return Status{8, "device error: spectrum has more than 2^20 distinct large estimates"};
+    if (emit && ws.h_stats[kStEmitted] > b.n_bases + 1)
+        return Status{6, "internal state error: spectrum emitted more keys than the batch has k-mers"};
+    return Status::ok();
+}
+
+Status Engine::spectrum(const DeviceBatch &b, SpectrumResult &r) const {
+    r = SpectrumResult{};
+    Status s = spectrum_checks(b);
+    if (!s.good()) return s;
+    if (b.n_bases < geo_.k || b.n_reads == 0) return Status::ok();
+    SCB_HIP(hipSetDevice(device_));
+    auto wsp = acquire();
+    if (!wsp) return Status{8, "device error: cannot create a workspace"};
+    Workspace &ws = *wsp;
+    do {
+        hipError_t e = ws.hist.reserve(geo_);
+        if (e != hipSuccess) {
+            s = Status{8, std::string("device error: ") + hipGetErrorString(e)};
             break;
         }
-        mark(ws, 3);
-        std::vector<unsigned long long> dense(dense_n);
-        std::vector<uint32_t> keys;
-        std::vector<unsigned long long> cnts;
-        (void)hipMemcpyAsync(dense.data(), ws.hist_dense.p, uint64_t(dense_n) * 8, hipMemcpyDeviceToHost, st);
-        if (geo_.bits == 32) {
-            keys.resize(kHistBigCap);
-            cnts.resize(kHistBigCap);
-            (void)hipMemcpyAsync(keys.data(), ws.hist_big_keys.p, kHistBigCap * 4, hipMemcpyDeviceToHost, st);
-            (void)hipMemcpyAsync(cnts.data(), ws.hist_big_counts.p, kHistBigCap * 8, hipMemcpyDeviceToHost, st);
-        }
-        if ((s = finish(ws)).code) break;
-        collect_times(ws, true);  // the bin stage includes the plan's host round trip
-        if (ws.h_stats[kStOverflow]) {
-            s = Status{8, "device error: spectrum has more than 2^20 distinct large estimates"};
-            break;
-        }
-        for (uint64_t v = 0; v < dense_n; ++v)
-            if (dense[v]) r.rows.emplace_back(v, dense[v]);
-        for (uint32_t i = 0; i < keys.size(); ++i)
-            if (keys[i]) r.rows.emplace_back(keys[i], cnts[i]);
-        std::sort(r.rows.begin(), r.rows.end());
+        ws.hist.clear(geo_, ws.stream);
+        HistOut h = ws.hist.out(ws.stats.p);
+        if ((s = spectrum_pass(ws, b, &h, false)).code) break;
+        if ((s = ws.hist.rows(geo_, ws.stream, r.rows)).code) break;
         r.queried = ws.h_stats[kStQueried];
         r.distinct = ws.h_stats[kStDistinct];
+    } while (false);
+    release(std::move(wsp));
+    return s;
+}
+
+Status Engine::spectrum_accumulate(const DeviceBatch &b, KeySet &ks) const {
+    Status s = spectrum_checks(b);
+    if (!s.good()) return s;
+    if (ks.device != device_) return Status{6, "internal state error: the key set belongs to another device"};
+    if (b.n_bases < geo_.k || b.n_reads == 0) return Status::ok();
+    SCB_HIP(hipSetDevice(device_));
+    auto wsp = acquire();
+    if (!wsp) return Status{8, "device error: cannot create a workspace"};
+    Workspace &ws = *wsp;
+    hipStream_t st = ws.stream;
+    do {
+        if ((s = spectrum_pass(ws, b, nullptr, true)).code) break;
+        const uint64_t ne = ws.h_stats[kStEmitted];
+        ks.queried += ws.h_stats[kStQueried];
+        if (!ne) break;
+        // keep the set at most half full: grow (powers of two) and rehash
+        uint32_t want = ks.log2;
+        while ((uint64_t(1) << want) < 2 * (ks.size + ne)) ++want;
+        if (want > 31) {
+            s = Status{8, "device error: the spectrum key set would exceed 2^30 distinct keys"};
+            break;
+        }
+        if (want != ks.log2) {
+            DevBuf<unsigned long long> bigger;
+            hipError_t e = bigger.reserve(uint64_t(1) << want);
+            if (e != hipSuccess) {
+                s = Status{8, std::string("device error: growing the spectrum key set: ") + hipGetErrorString(e)};
+                break;
+            }
+            fill(reinterpret_cast<uint32_t *>(bigger.p), (uint64_t(1) << want) * 2, 0xFFFFFFFFu, st);
+            const uint64_t old_cap = uint64_t(1) << ks.log2;
+            hipLaunchKernelGGL(k_keyset_rehash, dim3(unsigned(std::min<uint64_t>((old_cap + 255) / 256, 16384))),
+                               dim3(256), 0, st, ks.set.p, old_cap, bigger.p, want);
+            SCB_HIP(hipStreamSynchronize(st));  // the old set is freed below
+            std::swap(ks.set.p, bigger.p);
+            std::swap(ks.set.n, bigger.n);
+            ks.log2 = want;
+        }
+        hipLaunchKernelGGL(k_keyset_merge, dim3(unsigned(std::min<uint64_t>((ne + 255) / 256, 16384))), dim3(256), 0,
+                           st, ws.emit_keys.p, ws.emit_vals.p, ne, ks.set.p, ks.log2, ks.hist.out(ks.stats.p),
+                           reinterpret_cast<unsigned long long *>(ks.stats.p));
+        SCB_HIP(hipGetLastError());
+        uint64_t head[kStOverflow + 1];
+        SCB_HIP(hipMemcpyAsync(head, ks.stats.p, sizeof(head), hipMemcpyDeviceToHost, st));
+        SCB_HIP(hipStreamSynchronize(st));
+        ks.size = head[0];
+        if (head[kStOverflow]) s = Status{8, "device error: spectrum has more than 2^20 distinct large estimates"};
+    } while (false);
+    release(std::move(wsp));
+    return s;
+}
+
+Status Engine::keyset_rows(const KeySet &ks, SpectrumResult &r) const {
+    r = SpectrumResult{};
+    SCB_HIP(hipSetDevice(device_));
+    Status s = ks.hist.rows(geo_, nullptr, r.rows);
+    r.queried = ks.queried;
+    r.distinct = ks.size;
+    return s;
+}
+
+Status Engine::kmer_map(const DeviceBatch &b, std::vector<std::pair<uint64_t, uint32_t>> &pairs,
+                        uint64_t &queried) const {
+    pairs.clear();
+    queried = 0;
+    Status s = spectrum_checks(b);
+    if (!s.good()) return s;
+    if (b.n_bases < geo_.k || b.n_reads == 0) return Status::ok();
+    SCB_HIP(hipSetDevice(device_));
+    auto wsp = acquire();
+    if (!wsp) return Status{8, "device error: cannot create a workspace"};
+    Workspace &ws = *wsp;
+    do {
+        if ((s = spectrum_pass(ws, b, nullptr, true)).code) break;
+        const uint64_t ne = ws.h_stats[kStEmitted];
+        queried = ws.h_stats[kStQueried];
+        std::vector<uint64_t> keys(ne);
+        std::vector<uint32_t> vals(ne);
+        hipError_t e = hipMemcpy(keys.data(), ws.emit_keys.p, ne * 8, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(vals.data(), ws.emit_vals.p, ne * 4, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            s = Status{8, std::string("device error: copying the k-mer map: ") + hipGetErrorString(e)};
+            break;
+        }
+        pairs.resize(ne);
+        for (uint64_t i = 0; i < ne; ++i) pairs[i] = {keys[i], vals[i]};
     } while (false);
     release(std::move(wsp));
     return s;
@@ -3118,27 +3376,94 @@ Status Engine::query_host(const uint8_t *bases, const uint64_t *offsets, uint64_
     return s;
 }
 
+namespace {
+// Host records staged chunk by chunk (chunks of at most `limit` bases, split at
+// record boundaries) into the staging workspace; fn(DeviceBatch) runs per chunk.
+template <class Fn>
+Status for_each_staged_chunk(Workspace &st, const uint8_t *bases, const uint64_t *offsets, uint64_t n,
+                             uint64_t limit, Fn &&fn) {
+    return for_each_chunk(offsets, n, limit, [&](uint64_t r0, uint64_t r1) -> Status {
+        const uint64_t nb = offsets[r1] - offsets[r0];
+        SCB_HIP(st.h_bases_dev.reserve(nb + 4));
+        SCB_HIP(st.h_offsets_dev.reserve(r1 - r0 + 1));
+        SCB_HIP(st.h_offs.reserve(r1 - r0 + 1));
+        for (uint64_t i = r0; i <= r1; ++i) st.h_offs.p[i - r0] = offsets[i] - offsets[r0];
+        SCB_HIP(hipMemcpy(st.h_bases_dev.p, bases + offsets[r0], nb, hipMemcpyHostToDevice));
+        SCB_HIP(hipMemcpy(st.h_offsets_dev.p, st.h_offs.p, (r1 - r0 + 1) * 8, hipMemcpyHostToDevice));
+        return fn(DeviceBatch{st.h_bases_dev.p, st.h_offsets_dev.p, r1 - r0, nb});
+    });
+}
+// Spectrum chunks of host input: the per-call pass holds about 8 bytes of
+// records and 12 bytes of emitted keys per base.
+constexpr uint64_t kSpectrumChunkBases = uint64_t(1) << 28;
+}  // namespace
+
+// One call when the input fits one device batch; larger inputs go through a
+// key set chunk by chunk, so a key seen in an earlier chunk is not counted again.
 Status Engine::spectrum_host(const uint8_t *bases, const uint64_t *offsets, uint64_t n,
                              SpectrumResult &r) const {
     r = SpectrumResult{};
     SCB_HIP(hipSetDevice(device_));
     const uint64_t nb = offsets[n] - offsets[0];
-    if (nb >= kMaxBatchBases)
-        return Status{6, "internal state error: spectrum query exceeds 2^32 bases in one call"};
     auto st = acquire();
     if (!st) return Status{8, "device error: cannot create a workspace"};
-    Status s = [&]() -> Status {
-        SCB_HIP(st->h_bases_dev.reserve(nb + 4));
-        SCB_HIP(st->h_offsets_dev.reserve(n + 1));
-        SCB_HIP(st->h_offs.reserve(n + 1));
-        for (uint64_t i = 0; i <= n; ++i) st->h_offs.p[i] = offsets[i] - offsets[0];
-        SCB_HIP(hipMemcpy(st->h_bases_dev.p, bases + offsets[0], nb, hipMemcpyHostToDevice));
-        SCB_HIP(hipMemcpy(st->h_offsets_dev.p, st->h_offs.p, (n + 1) * 8, hipMemcpyHostToDevice));
-        DeviceBatch b{st->h_bases_dev.p, st->h_offsets_dev.p, n, nb};
-        return spectrum(b, r);
-    }();
+    Status s;
+    if (nb < kMaxBatchBases) {
+        s = for_each_staged_chunk(*st, bases, offsets, n, kMaxBatchBases,
+                                  [&](const DeviceBatch &b) { return spectrum(b, r); });
+    } else {
+        KeySetPtr ks;
+        s = keyset_create(ks);
+        if (s.good())
+            s = for_each_staged_chunk(*st, bases, offsets, n, kSpectrumChunkBases,
+                                      [&](const DeviceBatch &b) { return spectrum_accumulate(b, *ks); });
+        if (s.good()) s = keyset_rows(*ks, r);
+    }
     release(std::move(st));
     return s;
+}
+
+Status Engine::spectrum_accumulate_host(const uint8_t *bases, const uint64_t *offsets, uint64_t n,
+                                        KeySet &ks) const {
+    SCB_HIP(hipSetDevice(device_));
+    auto st = acquire();
+    if (!st) return Status{8, "device error: cannot create a workspace"};
+    Status s = for_each_staged_chunk(*st, bases, offsets, n, kSpectrumChunkBases,
+                                     [&](const DeviceBatch &b) { return spectrum_accumulate(b, ks); });
+    release(std::move(st));
+    return s;
+}
+
+// Pairs of every chunk, then one pass that keeps the first pair of each key
+// (all pairs of a key carry the same estimate), sorted by key.
+Status Engine::kmer_map_host(const uint8_t *bases, const uint64_t *offsets, uint64_t n,
+                             std::vector<std::pair<uint64_t, uint32_t>> &pairs, uint64_t &queried) const {
+    pairs.clear();
+    queried = 0;
+    SCB_HIP(hipSetDevice(device_));
+    auto st = acquire();
+    if (!st) return Status{8, "device error: cannot create a workspace"};
+    Status s = for_each_staged_chunk(*st, bases, offsets, n, kSpectrumChunkBases, [&](const DeviceBatch &b) {
+        std::vector<std::pair<uint64_t, uint32_t>> part;
+        uint64_t q = 0;
+        Status ps = kmer_map(b, part, q);
+        if (!ps.good()) return ps;
+        queried += q;
+        pairs.insert(pairs.end(), part.begin(), part.end());
+        return Status::ok();
+    });
+    release(std::move(st));
+    if (!s.good()) return s;
+    std::stable_sort(pairs.begin(), pairs.end(),
+                     [](const std::pair<uint64_t, uint32_t> &x, const std::pair<uint64_t, uint32_t> &y) {
+                         return x.first < y.first;
+                     });
+    pairs.erase(std::unique(pairs.begin(), pairs.end(),
+                            [](const std::pair<uint64_t, uint32_t> &x, const std::pair<uint64_t, uint32_t> &y) {
+                                return x.first == y.first;
+                            }),
+                pairs.end());
+    return Status::ok();
 }
 
 StageTimes last_stage_times() { return g_stage_times; }

@@ -76,6 +76,7 @@ def lib():
             "oracle_spectrum_add_chunk": (C.c_uint64, [C.c_void_p, C.c_void_p, u8p, C.c_size_t]),
             "oracle_spectrum_distinct": (C.c_size_t, [C.c_void_p]),
             "oracle_spectrum_rows": (C.c_size_t, [C.c_void_p, u64p, u64p, C.c_size_t]),
+            "oracle_spectrum_pairs": (C.c_size_t, [C.c_void_p, u64p, u64p, C.c_size_t]),
             "oracle_save": (C.c_int, [C.c_void_p, C.c_uint64, C.c_char_p, C.c_char_p, C.c_size_t]),
             "oracle_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(OracleConfig), u64p,
                                       C.c_char_p, C.c_size_t]),
@@ -304,6 +305,14 @@ class Spectrum:
 
     def distinct(self):
         return lib().oracle_spectrum_distinct(self.h)
+
+    def pairs(self):
+        """The merged map as sorted key and estimate arrays."""
+        n = self.distinct()
+        k = np.zeros(max(1, n), dtype=np.uint64)
+        v = np.zeros(max(1, n), dtype=np.uint64)
+        got = lib().oracle_spectrum_pairs(self.h, _p(k, u64p), _p(v, u64p), n)
+        return k[:got], v[:got]
 
     def rows(self):
         cap = 1 << 20

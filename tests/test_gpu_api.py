@@ -284,3 +284,88 @@ def test_fastx_pipeline_with_n_runs_matches_oracle(gpu, oracle, tmp_path, bits):
     assert got == want
     assert (build.chunks, build.indexed_kmers, build.inserted_smers) == want[:3]
     assert st.chunks > len(ins)  # the N runs split records into several chunks
+
+
+@pytest.mark.parametrize("bits", [8, 16, 32])
+def test_host_estimators_and_scb_bytes_equal_oracle(gpu, oracle, tmp_path, bits):
+    """On a real index at each counter width: the filter-only spectrum and count
+    hint of the index's own histogram (api.rs:337-385, 933-1033), the same on
+    given histograms, and summarize_sequence (api.rs:264-271, 855-891) equal the
+    oracle's; scb_index_save writes exactly the oracle's .scb bytes, header
+    (with non-default threads and queue) and payload (api.rs:388-480)."""
+    scb = gpu
+    k, m, s, h, cse, bse = (21, 15, 17, 3, 22, 10) if bits == 32 else (31, 21, 25, 4, 22, 9)
+    g = synth.genome(200_000, 3)
+    ib, io = synth.fixed_reads(g, 20_000, 150, 4, sub_rate=0.01, n_rate=0.001)
+    cfg = scb.SuperCountingBloomConfig(k=k, m=m, s=s, n_hashes=h, counter_bits=bits, counter_slots_exponent=cse,
+                                       block_slots_exponent=bse, threads=3, queue=77)
+    b = scb.SuperCountingBloomBuilder(cfg)
+    b.add_sequences((ib, io))
+    idx = b.into_index()
+    of = oracle.Filter(oracle.make_cfg(k, m, s, h, bits, cse, bse, threads=3, queue=77))
+    kmers, _ = of.insert_records(ib, io)
+    assert idx.inserted_kmers() == kmers
+    ohist = of.counter_histogram()
+    assert idx.counter_histogram() == ohist
+    hint = idx.filter_only_count_hint()
+    assert hint == oracle.filter_only_count_hint(ohist, h, 255)
+    assert idx.estimate_filter_only_spectrum(255, hint).rows == oracle.filter_only_spectrum(ohist, h, 255, hint)
+    for max_ab, given_hint in ((8, 1000), (64, 12345)):
+        got = idx.estimate_filter_only_spectrum_from_counter_histogram(ohist, max_ab, given_hint).rows
+        assert got == oracle.filter_only_spectrum(ohist, h, max_ab, given_hint)
+        assert idx.filter_only_count_hint_from_counter_histogram(ohist, max_ab) == \
+            oracle.filter_only_count_hint(ohist, h, max_ab)
+    for seq in (ib[:1500], ib[:k - 1], b"ACGTNNNN" * 20 + bytes(ib[3000:3400])):
+        summ = idx.summarize_sequence("q", seq)
+        v, ok = of.estimate_ascii(seq)
+        n, mean, med = oracle.summarize(v, ok)
+        assert (summ.total_windows, summ.valid_kmers) == (v.size, n)
+        if n:
+            assert (summ.mean_abundance, summ.median_abundance) == (mean, med)
+        else:
+            assert summ.mean_abundance is None and summ.median_abundance is None
+    ours, theirs = tmp_path / "gpu.scb", tmp_path / "oracle.scb"
+    idx.save(ours)
+    rc, msg = of.save(theirs, kmers)
+    assert rc == 0, msg
+    a, bb = ours.read_bytes(), theirs.read_bytes()
+    assert len(a) == len(bb) == 139 + (1 << cse) * bits // 8
+    assert a[:139] == bb[:139], "header differs"
+    assert a == bb, "payload differs"
+
+
+def test_device_batches_with_bad_offsets_are_refused(gpu):
+    """The offsets contract of the HBM-resident entry points (offsets[0] = 0,
+    never decreasing, offsets[n] = total_bases) is checked on the device."""
+    from hipmem import DeviceBuffer
+    scb = gpu
+    b = scb.SuperCountingBloomBuilder(small_cfg(scb, counter_bits=16))
+    bases = np.frombuffer(b"ACGTACGTTGCAACGT" * 64, dtype=np.uint8).copy()
+    db = DeviceBuffer.from_array(bases)
+    for offs in ([0, 600, 500, 1024], [0, 512, 1000], [8, 512, 1024]):
+        o = np.array(offs, dtype=np.uint64)
+        do = DeviceBuffer.from_array(o)
+        with pytest.raises(scb.SuperCountingBloomError) as e:
+            b.add_sequences_device(db.ptr, do.ptr, o.size - 1, bases.size)
+        assert e.value.kind == "InternalState" and "offsets" in str(e.value)
+    good = DeviceBuffer.from_array(np.array([0, 512, 1024], dtype=np.uint64))
+    assert b.add_sequences_device(db.ptr, good.ptr, 2, bases.size).inserted_kmers == 2 * (512 - 5 + 1)
+    idx = b.into_index()
+    bad = DeviceBuffer.from_array(np.array([0, 700, 300, 1024], dtype=np.uint64))
+    with pytest.raises(scb.SuperCountingBloomError):
+        idx.estimate_sequences_device(db.ptr, bad.ptr, 3, bases.size, None, None)
+    rep = idx.estimate_sequences_device(db.ptr, good.ptr, 2, bases.size, None, None)
+    assert rep.valid_kmers == 2 * (512 - 5 + 1)
+
+
+def test_spectrum_refuses_blocks_beyond_2e31_slots(gpu):
+    """counter_slots_exponent 33 with 2^32-slot blocks is a valid config
+    (config.rs:76) that the GPU spectrum cannot address: a clear error."""
+    scb = gpu
+    b = scb.SuperCountingBloomBuilder(small_cfg(scb, k=9, m=5, s=7, counter_slots_exponent=33,
+                                                block_slots_exponent=32))
+    b.add_sequence(b"ACGTACGTTGCAACGTAGGT")
+    idx = b.into_index()
+    with pytest.raises(scb.SuperCountingBloomError) as e:
+        idx.spectrum([b"ACGTACGTTGCAACGTAGGT"])
+    assert e.value.kind == "InternalState" and "2^31" in str(e.value)
