@@ -304,6 +304,34 @@ int scb_write_spectrum(const uint64_t *values, const uint64_t *counts, size_t n_
 typedef int (*scb_record_visitor)(void *user, const char *name, const uint8_t *seq, size_t len);
 int scb_for_each_fastx_record(const char *path, scb_record_visitor visit, void *user);
 
+/* ---- reservoir-sampled spectrum (bin/sampled_spectrum.rs:150-308) ---------- */
+/* A row of write_scaled_spectrum (sampled_spectrum.rs:223-253). */
+typedef struct scb_sampled_row {
+    uint64_t estimated_count, sampled_kmers, linear_scaled_windows, estimated_distinct_kmers;
+} scb_sampled_row;
+/* The CLI's stderr metrics (sampled_spectrum.rs:113-146). Sampling and
+ * querying are one fused device pass here (sample_seconds covers both). */
+typedef struct scb_sampled_stats {
+    uint64_t indexed_kmers, inserted_smers, total_inserted_kmers, sampled_kmers, queried_kmers,
+        invalid_queries;
+    double build_seconds, sample_seconds;
+} scb_sampled_stats;
+/* sample_inserted_kmers + query_sampled_kmers (sampled_spectrum.rs:150-221)
+ * over n host records against the index: a reservoir of sample_size valid
+ * k-mer windows in input order (SplitMix64 from seed), each sampled k-mer's
+ * estimate, rows sorted by estimate. sample_size 0 is SCB_ERR_INVALID_CONFIG. */
+int scb_index_sampled_spectrum(const scb_index *idx, const uint8_t *bases, const uint64_t *offsets,
+                               size_t n, uint64_t sample_size, uint64_t seed, scb_sampled_row *rows,
+                               size_t cap, size_t *n_rows, scb_sampled_stats *stats);
+/* run() of bin/sampled_spectrum.rs:109-148: build the index from the FASTA/FASTQ
+ * file, then sample and query its valid windows. */
+int scb_sampled_spectrum_from_fastx(const char *path, const scb_config *cfg, uint64_t sample_size,
+                                    uint64_t seed, scb_sampled_row *rows, size_t cap, size_t *n_rows,
+                                    scb_sampled_stats *stats);
+/* CSV "estimated_count,sampled_kmers,linear_scaled_windows,estimated_distinct_kmers"
+ * (sampled_spectrum.rs:228-252); path NULL writes to stdout */
+int scb_write_sampled_spectrum(const scb_sampled_row *rows, size_t n_rows, const char *path);
+
 /* ---- serving loop --------------------------------------------------------
  * No counterpart in the reference. A frozen index stays resident in HBM and
  * answers concurrent requests: worker threads coalesce queued abundance

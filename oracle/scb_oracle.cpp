@@ -947,6 +947,105 @@ size_t oracle_spectrum_rows(const oracle_spectrum *sp, uint64_t *values, uint64_
     return rows.size();
 }
 
+// bin/sampled_spectrum.rs:150-308: reservoir sampling of the valid k-mer
+// windows of the records in input order (SplitMix64 draws, gen_below = next %
+// upper), then each sampled k-mer queried on its own through
+// estimate_sequence_abundances (api.rs:250-262), the first window's estimate
+// counted; rows scaled as write_scaled_spectrum (sampled_spectrum.rs:223-253).
+// rows holds 4 u64 per row: estimate, sampled k-mers, linear scaled windows,
+// estimated distinct k-mers. stats: total windows seen, sampled k-mers,
+// queried k-mers, invalid queries.
+namespace {
+// ReservoirSampler (sampled_spectrum.rs:255-308) over the valid windows of n
+// records in order (sample_sequence_kmers, 168-199): the retained k-mers
+// (uppercased) and, per slot, the 1-based number of the window it holds.
+struct SampledWindows {
+    std::vector<std::vector<uint8_t>> kmers;
+    std::vector<uint64_t> window;
+    uint64_t seen = 0;
+};
+SampledWindows reservoir_sample(const uint8_t *bases, const uint64_t *offsets, size_t n, size_t k,
+                                uint64_t sample_size, uint64_t seed) {
+    SampledWindows out;
+    uint64_t state = seed;
+    auto next_u64 = [&]() {  // sampled_spectrum.rs:289-295
+        state += 0x9E3779B97F4A7C15ull;
+        uint64_t v = state;
+        v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ull;
+        v = (v ^ (v >> 27)) * 0x94D049BB133111EBull;
+        return v ^ (v >> 31);
+    };
+    auto consider = [&](const uint8_t *kmer) {  // sampled_spectrum.rs:262-279
+        ++out.seen;
+        size_t slot;
+        if (out.kmers.size() < sample_size) {
+            slot = out.kmers.size();
+        } else {
+            uint64_t r = next_u64() % out.seen;  // gen_below (297-299)
+            if (r >= sample_size) return;
+            slot = size_t(r);
+        }
+        std::vector<uint8_t> up(kmer, kmer + k);
+        for (auto &c : up)
+            if (c >= 'a' && c <= 'z') c = uint8_t(c - 32);
+        if (slot == out.kmers.size()) {
+            out.kmers.push_back(std::move(up));
+            out.window.push_back(out.seen);
+        } else {
+            out.kmers[slot] = std::move(up);
+            out.window[slot] = out.seen;
+        }
+    };
+    for (size_t r = 0; r < n; ++r) {
+        const uint8_t *rec = bases + offsets[r];
+        for_each_valid_run(rec, size_t(offsets[r + 1] - offsets[r]), k, [&](size_t a, size_t b) {
+            for (size_t o = a; o + k <= b; ++o) consider(rec + o);
+        });
+    }
+    return out;
+}
+}  // namespace
+
+// The sampler alone: window numbers held by slots 0..min(seen, sample_size);
+// returns the number of slots filled, *seen the windows considered.
+size_t oracle_reservoir_windows(const uint8_t *bases, const uint64_t *offsets, size_t n, size_t k,
+                                uint64_t sample_size, uint64_t seed, uint64_t *windows, uint64_t *seen) {
+    SampledWindows sw = reservoir_sample(bases, offsets, n, k, sample_size, seed);
+    std::copy(sw.window.begin(), sw.window.end(), windows);
+    *seen = sw.seen;
+    return sw.window.size();
+}
+
+size_t oracle_sampled_spectrum(const oracle_filter *f, const uint8_t *bases, const uint64_t *offsets, size_t n,
+                               uint64_t sample_size, uint64_t seed, uint64_t *rows, size_t cap, uint64_t *stats) {
+    SampledWindows sw = reservoir_sample(bases, offsets, n, f->f->cfg.k, sample_size, seed);
+    const std::vector<std::vector<uint8_t>> &kmers = sw.kmers;
+    const uint64_t seen = sw.seen;
+    std::unordered_map<uint64_t, uint64_t> hist;  // sampled_spectrum.rs:201-221
+    uint64_t queried = 0, invalid = 0;
+    std::vector<uint64_t> v(1);
+    std::vector<uint8_t> ok(1);
+    for (auto &km : kmers) {
+        ++queried;
+        size_t got = oracle_filter_estimate_ascii(f, km.data(), km.size(), v.data(), ok.data());
+        if (got && ok[0]) ++hist[v[0]];
+        else ++invalid;
+    }
+    std::vector<std::pair<uint64_t, uint64_t>> h(hist.begin(), hist.end());
+    std::sort(h.begin(), h.end());
+    const double sample_count = double(std::max<size_t>(kmers.size(), 1));
+    const double scale = double(seen) / sample_count;
+    for (size_t i = 0; i < h.size() && i < cap; ++i) {
+        const uint64_t lin = f64_as_u64(std::round(double(h[i].second) * scale));
+        rows[4 * i] = h[i].first;
+        rows[4 * i + 1] = h[i].second;
+        rows[4 * i + 2] = lin;
+        rows[4 * i + 3] = h[i].first == 0 ? 0 : f64_as_u64(std::round(double(lin) / double(h[i].first)));
+    }
+    stats[0] = seen, stats[1] = kmers.size(), stats[2] = queried, stats[3] = invalid;
+    return h.size();
+}
+
 // api.rs:388-414, 466-480, 515-529, 545-559
 int oracle_save(const oracle_filter *f, uint64_t inserted_kmers, const char *path, char *msg,
                 size_t cap) {

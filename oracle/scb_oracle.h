@@ -114,6 +114,18 @@ size_t oracle_spectrum_rows(const oracle_spectrum *sp, uint64_t *values, uint64_
 /* the merged map as (key, estimate) pairs sorted by key; returns the pair count */
 size_t oracle_spectrum_pairs(const oracle_spectrum *sp, uint64_t *keys, uint64_t *values, size_t cap);
 
+/* ---- reservoir-sampled spectrum (bin/sampled_spectrum.rs:150-308) -------- */
+/* Sequential reservoir over the valid windows of n records, sampled k-mers
+ * queried one by one; rows: 4 u64 each (estimate, sampled, linear scaled
+ * windows, estimated distinct); stats: seen, sampled, queried, invalid.
+ * Returns the row count. */
+size_t oracle_sampled_spectrum(const oracle_filter *f, const uint8_t *bases, const uint64_t *offsets, size_t n,
+                               uint64_t sample_size, uint64_t seed, uint64_t *rows, size_t cap, uint64_t *stats);
+/* The sampler alone: the 1-based window number each slot holds (windows has
+ * min(seen, sample_size) entries); returns the slot count, *seen the windows. */
+size_t oracle_reservoir_windows(const uint8_t *bases, const uint64_t *offsets, size_t n, size_t k,
+                                uint64_t sample_size, uint64_t seed, uint64_t *windows, uint64_t *seen);
+
 /* ---- .scb serialization (api.rs:388-608) --------------------------------- */
 /* returns 0 on success; message written on error */
 int oracle_save(const oracle_filter *f, uint64_t inserted_kmers, const char *path, char *msg,

@@ -55,6 +55,17 @@ class ScbSummary(C.Structure):
                 ("mean_abundance", C.c_double), ("median_abundance", C.c_double)]
 
 
+class ScbSampledRow(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("estimated_count", "sampled_kmers", "linear_scaled_windows",
+                                          "estimated_distinct_kmers")]
+
+
+class ScbSampledStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("indexed_kmers", "inserted_smers", "total_inserted_kmers",
+                                          "sampled_kmers", "queried_kmers", "invalid_queries")] + \
+               [("build_seconds", C.c_double), ("sample_seconds", C.c_double)]
+
+
 class ScbServeConfig(C.Structure):
     _fields_ = [("max_batch_bases", C.c_uint64), ("max_batch_requests", C.c_uint64),
                 ("hbm_budget_bytes", C.c_uint64), ("workers", C.c_uint32), ("max_pending", C.c_uint32)]
@@ -115,6 +126,12 @@ SIGNATURES = {
     "scb_index_estimate_kmer_map": (C.c_int, [C.c_void_p, u8p, u64p, C.c_size_t, u64p, u64p, C.c_size_t, szp,
                                               u64p]),
     "scb_spectrum_acc_new": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "scb_index_sampled_spectrum": (C.c_int, [C.c_void_p, u8p, u64p, C.c_size_t, C.c_uint64, C.c_uint64,
+                                             C.POINTER(ScbSampledRow), C.c_size_t, szp, C.POINTER(ScbSampledStats)]),
+    "scb_sampled_spectrum_from_fastx": (C.c_int, [C.c_char_p, C.POINTER(ScbConfig), C.c_uint64, C.c_uint64,
+                                                  C.POINTER(ScbSampledRow), C.c_size_t, szp,
+                                                  C.POINTER(ScbSampledStats)]),
+    "scb_write_sampled_spectrum": (C.c_int, [C.POINTER(ScbSampledRow), C.c_size_t, C.c_char_p]),
     "scb_spectrum_acc_free": (None, [C.c_void_p]),
     "scb_spectrum_acc_add": (C.c_int, [C.c_void_p, u8p, u64p, C.c_size_t]),
     "scb_spectrum_acc_add_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_uint64]),

@@ -10,9 +10,9 @@ from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
-from ._lib import (VISITOR, ScbAddReport, ScbBuildReport, ScbConfig, ScbQueryReport, ScbServeConfig,
-                   ScbServed, ScbSpectrumStats, ScbStreamStats, ScbSummary, last_error, lib, u8p,
-                   u32p, u64p)
+from ._lib import (VISITOR, ScbAddReport, ScbBuildReport, ScbConfig, ScbQueryReport, ScbSampledRow,
+                   ScbSampledStats, ScbServeConfig, ScbServed, ScbSpectrumStats, ScbStreamStats, ScbSummary,
+                   last_error, lib, u8p, u32p, u64p)
 
 ApproxSpectrum = List[Tuple[int, int]]
 
@@ -422,6 +422,17 @@ class SuperCountingBloomIndex:  # api.rs:50-54, 210-386
                                                  _ptr(keys, u64p), _ptr(vals, u64p), cap, C.byref(n), C.byref(q)))
         return keys[:n.value], vals[:n.value], q.value
 
+    def sampled_spectrum(self, records, sample_size: int = 1_000_000, seed: int = 1) -> "SampledSpectrum":
+        """bin/sampled_spectrum.rs:150-253 over the records against this index."""
+        bases, offsets = concat_records(records) if not isinstance(records, tuple) else records
+        cap = 1 << 17
+        rows = (ScbSampledRow * cap)()
+        n = C.c_size_t()
+        st = ScbSampledStats()
+        _check(lib().scb_index_sampled_spectrum(self._h, _ptr(bases), _ptr(offsets, u64p), offsets.size - 1,
+                                                sample_size, seed, rows, cap, C.byref(n), C.byref(st)))
+        return _sampled(rows, n.value, st)
+
     def spectrum_accumulator(self) -> "SpectrumAccumulator":
         return SpectrumAccumulator(self)
 
@@ -494,6 +505,47 @@ class SpectrumAccumulator:
             if n.value <= cap:
                 return [(int(a), int(b)) for a, b in zip(v[:n.value], c[:n.value])], q.value, d.value
             cap = n.value
+
+
+@dataclass
+class SampledSpectrum:
+    """write_scaled_spectrum's rows (estimated_count, sampled_kmers,
+    linear_scaled_windows, estimated_distinct_kmers) and the CLI's metrics."""
+    rows: List[Tuple[int, int, int, int]]
+    total_inserted_kmers: int
+    sampled_kmers: int
+    queried_kmers: int
+    invalid_queries: int
+    indexed_kmers: int = 0
+    inserted_smers: int = 0
+    build_seconds: float = 0.0
+    sample_seconds: float = 0.0
+
+    def write(self, output=None):
+        arr = (ScbSampledRow * max(1, len(self.rows)))()
+        for i, r in enumerate(self.rows):
+            arr[i] = ScbSampledRow(*r)
+        _check(lib().scb_write_sampled_spectrum(arr, len(self.rows), None if output is None else _path(output)))
+
+
+def _sampled(rows, n, st):
+    out = [(rows[i].estimated_count, rows[i].sampled_kmers, rows[i].linear_scaled_windows,
+            rows[i].estimated_distinct_kmers) for i in range(min(n, len(rows)))]
+    return SampledSpectrum(out, st.total_inserted_kmers, st.sampled_kmers, st.queried_kmers, st.invalid_queries,
+                           st.indexed_kmers, st.inserted_smers, st.build_seconds, st.sample_seconds)
+
+
+def sampled_spectrum_from_fastx(path, cfg: SuperCountingBloomConfig, sample_size: int = 1_000_000,
+                                seed: int = 1) -> SampledSpectrum:
+    """run() of bin/sampled_spectrum.rs: index the file, then sample and query its windows."""
+    cap = 1 << 17
+    rows = (ScbSampledRow * cap)()
+    n = C.c_size_t()
+    st = ScbSampledStats()
+    c = cfg._c()
+    _check(lib().scb_sampled_spectrum_from_fastx(_path(path), C.byref(c), sample_size, seed, rows, cap,
+                                                 C.byref(n), C.byref(st)))
+    return _sampled(rows, n.value, st)
 
 
 def _hist_arrays(h):

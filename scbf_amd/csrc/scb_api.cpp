@@ -1104,6 +1104,121 @@ int scb_stream_estimates_from_fastx_pair(const char *index_path, const char *que
     return SCB_OK;
 }
 
+// ---- reservoir-sampled spectrum (bin/sampled_spectrum.rs) ----------------------
+namespace {
+// query_sampled_kmers' histogram (sampled_spectrum.rs:201-221) of the sampled
+// estimates and write_scaled_spectrum's scaling (223-253), in f64 as there.
+std::vector<scb_sampled_row> sampled_rows(const std::vector<uint32_t> &vals, uint64_t seen) {
+    std::vector<std::pair<uint64_t, uint64_t>> h;
+    {
+        std::vector<uint32_t> v(vals);
+        std::sort(v.begin(), v.end());
+        for (size_t i = 0; i < v.size();) {
+            size_t j = i;
+            while (j < v.size() && v[j] == v[i]) ++j;
+            h.emplace_back(v[i], j - i);
+            i = j;
+        }
+    }
+    const double sample_count = double(std::max<size_t>(vals.size(), 1));
+    const double scale = double(seen) / sample_count;
+    std::vector<scb_sampled_row> rows;
+    for (auto &r : h) {
+        scb_sampled_row o{};
+        o.estimated_count = r.first;
+        o.sampled_kmers = r.second;
+        o.linear_scaled_windows = f64_as_u64(std::round(double(r.second) * scale));
+        o.estimated_distinct_kmers =
+            r.first == 0 ? 0 : f64_as_u64(std::round(double(o.linear_scaled_windows) / double(r.first)));
+        rows.push_back(o);
+    }
+    return rows;
+}
+
+int sampled_out(const scb_index *idx, const scb::Reservoir &rs, scb_sampled_row *rows, size_t cap, size_t *n_rows,
+                scb_sampled_stats *stats) {
+    std::vector<uint32_t> vals;
+    uint64_t seen = 0;
+    Status s = idx->engine->reservoir_values(rs, vals, seen);
+    if (!s.good()) return fail(s);
+    auto out = sampled_rows(vals, seen);
+    if (cap && !rows) return fail(SCB_ERR_INTERNAL_STATE, "internal state error: null row buffer");
+    for (size_t i = 0; i < out.size() && i < cap; ++i) rows[i] = out[i];
+    if (n_rows) *n_rows = out.size();
+    if (stats) {
+        stats->total_inserted_kmers = seen;
+        stats->sampled_kmers = vals.size();
+        stats->queried_kmers = vals.size();
+        stats->invalid_queries = 0;  // every sampled window lies in an ACGT run
+    }
+    return SCB_OK;
+}
+}  // namespace
+
+int scb_index_sampled_spectrum(const scb_index *idx, const uint8_t *bases, const uint64_t *offsets, size_t n,
+                               uint64_t sample_size, uint64_t seed, scb_sampled_row *rows, size_t cap,
+                               size_t *n_rows, scb_sampled_stats *stats) {
+    if (sample_size == 0) return fail(SCB_ERR_INVALID_CONFIG, "invalid config: sample_size must be positive");
+    int rc = check_offsets(offsets, n);
+    if (rc) return rc;
+    double t0 = now_s();
+    scb::ReservoirPtr rs;
+    Status s = idx->engine->reservoir_create(sample_size, seed, rs);
+    if (s.good()) s = idx->engine->sample_accumulate_host(bases, offsets, n, *rs);
+    if (!s.good()) return fail(s);
+    if (stats) *stats = scb_sampled_stats{};
+    rc = sampled_out(idx, *rs, rows, cap, n_rows, stats);
+    if (stats) stats->sample_seconds = now_s() - t0;
+    return rc;
+}
+
+// bin/sampled_spectrum.rs:109-148 (run): build the index from the file, then
+// sample its valid windows and query them, in one pass of the file's records.
+int scb_sampled_spectrum_from_fastx(const char *path, const scb_config *cfg, uint64_t sample_size, uint64_t seed,
+                                    scb_sampled_row *rows, size_t cap, size_t *n_rows, scb_sampled_stats *stats) {
+    if (sample_size == 0) return fail(SCB_ERR_INVALID_CONFIG, "invalid config: sample_size must be positive");
+    int rc = full_check(*cfg);
+    if (rc) return rc;
+    scb_index *idx = nullptr;
+    scb_build_report br{};
+    if ((rc = scb_index_build_from_fastx(path, cfg, &idx, &br))) return rc;
+    std::unique_ptr<scb_index, void (*)(scb_index *)> guard(idx, scb_index_free);
+    double t0 = now_s();
+    scb::ReservoirPtr rs;
+    Status s = idx->engine->reservoir_create(sample_size, seed, rs);
+    if (!s.good()) return fail(s);
+    rc = for_each_batch(path, [&](HostBatch &hb) -> int {
+        Status bs = idx->engine->sample_accumulate_host(hb.bases.data(), hb.offsets.data(), hb.n(), *rs);
+        return bs.good() ? SCB_OK : fail(bs);
+    });
+    if (rc) return rc;
+    scb_sampled_stats st{};
+    rc = sampled_out(idx, *rs, rows, cap, n_rows, &st);
+    if (rc) return rc;
+    st.indexed_kmers = br.indexed_kmers;
+    st.inserted_smers = br.inserted_smers;
+    st.build_seconds = br.build_seconds;
+    st.sample_seconds = now_s() - t0;
+    if (stats) *stats = st;
+    return SCB_OK;
+}
+
+// write_scaled_spectrum (sampled_spectrum.rs:223-253): CSV with its header
+int scb_write_sampled_spectrum(const scb_sampled_row *rows, size_t n, const char *path) {
+    FILE *fp = path ? std::fopen(path, "wb") : stdout;
+    if (!fp) return fail(SCB_ERR_IO, std::string("I/O error for '") + path + "': " + std::strerror(errno));
+    std::fprintf(fp, "estimated_count,sampled_kmers,linear_scaled_windows,estimated_distinct_kmers\n");
+    for (size_t i = 0; i < n; ++i)
+        std::fprintf(fp, "%llu,%llu,%llu,%llu\n", (unsigned long long)rows[i].estimated_count,
+                     (unsigned long long)rows[i].sampled_kmers, (unsigned long long)rows[i].linear_scaled_windows,
+                     (unsigned long long)rows[i].estimated_distinct_kmers);
+    int bad = std::ferror(fp);
+    if (path) bad |= std::fclose(fp);
+    else std::fflush(fp);
+    if (bad) return fail(SCB_ERR_IO, std::string("I/O error for '") + (path ? path : "<stream>") + "': write failed");
+    return SCB_OK;
+}
+
 // spectrum.rs:9-31
 int scb_write_spectrum(const uint64_t *values, const uint64_t *counts, size_t n, const char *path) {
     FILE *fp = path ? fopen(path, "wb") : stdout;

@@ -79,6 +79,14 @@ struct KeySetDeleter {
 };
 using KeySetPtr = std::unique_ptr<KeySet, KeySetDeleter>;
 
+// Reservoir of sampled valid k-mer windows and their estimates
+// (bin/sampled_spectrum.rs:150-308), held in device memory.
+class Reservoir;
+struct ReservoirDeleter {
+    void operator()(Reservoir *r) const;
+};
+using ReservoirPtr = std::unique_ptr<Reservoir, ReservoirDeleter>;
+
 // Counter table resident in HBM in physical (.scb payload) block order
 // P(b) = (b & (shard_count-1)) * subblocks + (b >> log2(shard_count)).
 class Engine {
@@ -112,6 +120,14 @@ class Engine {
     // the batch (first occurrence; every occurrence has the same estimate)
     Status kmer_map(const DeviceBatch &b, std::vector<std::pair<uint64_t, uint32_t>> &pairs,
                     uint64_t &queried) const;
+    // Reservoir sampling of the valid windows of batches in input order (a
+    // capacity of cap windows, SplitMix64 draws from seed) with each sampled
+    // window's estimate; reservoir_values gives the min(seen, cap) estimates by
+    // slot and the number of windows seen.
+    Status reservoir_create(uint64_t cap, uint64_t seed, ReservoirPtr &out) const;
+    Status sample_accumulate(const DeviceBatch &b, Reservoir &rs) const;
+    Status sample_accumulate_host(const uint8_t *bases, const uint64_t *offsets, uint64_t n, Reservoir &rs) const;
+    Status reservoir_values(const Reservoir &rs, std::vector<uint32_t> &vals, uint64_t &seen) const;
     Status counter_histogram(std::vector<std::pair<uint64_t, uint64_t>> &rows) const;
 
     // Host <-> HBM copies of the whole table (physical order), and of batches.

@@ -2012,6 +2012,124 @@ __global__ __launch_bounds__(256) void k_keyset_rehash(const unsigned long long 
     }
 }
 
+// ---- reservoir-sampled spectrum (bin/sampled_spectrum.rs:150-308) ------------
+// The reference keeps the first `cap` valid windows of the input in a
+// reservoir; the w-th valid window (w > cap, windows counted from 1 in input
+// order) then draws r = next_u64() % w from one SplitMix64 stream and replaces
+// slot r when r < cap. The stream's t-th draw is the SplitMix64 output of
+// seed + t * 0x9E3779B97F4A7C15, i.e. xs64(seed + (t - 1) * 0x9E3779B97F4A7C15),
+// and window w makes draw t = w - cap, so every window's target slot is a
+// function of w alone: slot s ends up holding the largest w that targeted it.
+// Windows are numbered from a per-word rank of the query's validity bitmap;
+// one pass takes, per slot, the largest w (atomic max), a second pass stores
+// that window's estimate.
+constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
+constexpr int kRankThreads = 1024;
+
+// Reservoir slot written by the w-th valid window, or cap for none.
+__device__ __forceinline__ uint64_t reservoir_slot(uint64_t w, uint64_t cap, uint64_t seed) {
+    if (w <= cap) return w - 1;
+    const uint64_t r = xs64(seed + (w - cap - 1) * kGolden) % w;
+    return r < cap ? r : cap;
+}
+
+// Valid windows per block of kRankThreads bitmap words.
+__global__ __launch_bounds__(kRankThreads) void k_rank_sums(const uint32_t *bits, uint64_t nwords,
+                                                            uint32_t *block_sums) {
+    __shared__ uint32_t s_w[kRankThreads / 64];
+    const uint64_t i = uint64_t(blockIdx.x) * kRankThreads + threadIdx.x;
+    uint32_t c = i < nwords ? uint32_t(__popc(bits[i])) : 0u;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int q = 0; q < kRankThreads / 64; ++q) t += s_w[q];
+        block_sums[blockIdx.x] = t;
+    }
+}
+
+// Exclusive scan of the block sums in place (one workgroup, chunks of 1024).
+__global__ __launch_bounds__(kRankThreads) void k_rank_scan_blocks(uint32_t *sums, uint64_t n) {
+    __shared__ uint32_t s_w[kRankThreads / 64];
+    __shared__ uint32_t s_carry;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (uint64_t base = 0; base < n; base += kRankThreads) {
+        const uint64_t i = base + threadIdx.x;
+        const uint32_t v = i < n ? sums[i] : 0u;
+        uint32_t incl = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off, 64);
+            if (lane >= uint32_t(off)) incl += y;
+        }
+        if (lane == 63) s_w[wid] = incl;
+        __syncthreads();
+        uint32_t before = s_carry, total = 0;
+        for (uint32_t q = 0; q < kRankThreads / 64; ++q) {
+            if (q < wid) before += s_w[q];
+            total += s_w[q];
+        }
+        if (i < n) sums[i] = before + incl - v;
+        __syncthreads();
+        if (threadIdx.x == 0) s_carry += total;
+        __syncthreads();
+    }
+}
+
+// rank[i] = valid windows in words before word i.
+__global__ __launch_bounds__(kRankThreads) void k_rank_words(const uint32_t *bits, uint64_t nwords,
+                                                             const uint32_t *block_base, uint32_t *rank) {
+    __shared__ uint32_t s_w[kRankThreads / 64];
+    const uint64_t i = uint64_t(blockIdx.x) * kRankThreads + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint32_t v = i < nwords ? uint32_t(__popc(bits[i])) : 0u;
+    uint32_t incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off, 64);
+        if (lane >= uint32_t(off)) incl += y;
+    }
+    if (lane == 63) s_w[wid] = incl;
+    __syncthreads();
+    uint32_t before = block_base[blockIdx.x];
+    for (uint32_t q = 0; q < wid; ++q) before += s_w[q];
+    if (i < nwords) rank[i] = before + incl - v;
+}
+
+// Per slot, the largest window number that targets it (one thread per word).
+__global__ __launch_bounds__(256) void k_reservoir_hits(const uint32_t *bits, uint64_t nwords, const uint32_t *rank,
+                                                        uint64_t w0, uint64_t cap, uint64_t seed,
+                                                        unsigned long long *slot_win) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nwords;
+         i += uint64_t(gridDim.x) * blockDim.x) {
+        uint64_t w = w0 + rank[i];
+        for (uint32_t b = bits[i]; b; b &= b - 1) {
+            ++w;
+            const uint64_t slot = reservoir_slot(w, cap, seed);
+            if (slot < cap) atomicMax(&slot_win[slot], (unsigned long long)w);
+        }
+    }
+}
+
+// The estimate of each slot's final window (when that window is in this batch).
+template <int BITS>
+__global__ __launch_bounds__(256) void k_reservoir_take(const uint32_t *bits, uint64_t nwords, const uint32_t *rank,
+                                                        uint64_t w0, uint64_t cap, uint64_t seed,
+                                                        const unsigned long long *slot_win, const void *values,
+                                                        uint32_t *slot_val) {
+    const typename Ctr<BITS>::T *v = reinterpret_cast<const typename Ctr<BITS>::T *>(values);
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nwords;
+         i += uint64_t(gridDim.x) * blockDim.x) {
+        uint64_t w = w0 + rank[i];
+        for (uint32_t b = bits[i]; b; b &= b - 1) {
+            ++w;
+            const uint64_t slot = reservoir_slot(w, cap, seed);
+            if (slot < cap && slot_win[slot] == w) slot_val[slot] = uint32_t(v[i * 32 + uint32_t(__ffs(b) - 1)]);
+        }
+    }
+}
+
 // filter.rs:50-55, 267-275: histogram of every counter value. Values below
 // kSmall are counted in LDS; larger 8/16-bit values go straight to a dense
 // global array; larger 32-bit values go to an open-addressing table.
@@ -3463,6 +3581,97 @@ Status Engine::kmer_map_host(const uint8_t *bases, const uint64_t *offsets, uint
                                 return x.first == y.first;
                             }),
                 pairs.end());
+    return Status::ok();
+}
+
+// ---- reservoir-sampled spectrum -------------------------------------------------
+class Reservoir {
+   public:
+    uint64_t cap = 0, seed = 0, seen = 0;
+    DevBuf<unsigned long long> slot_win;  // largest window number per slot
+    DevBuf<uint32_t> slot_val;            // estimate of that window
+    DevBuf<uint8_t> values;               // per-batch query outputs
+    DevBuf<uint32_t> bits, rank, block_sums;
+};
+
+void ReservoirDeleter::operator()(Reservoir *r) const { delete r; }
+
+Status Engine::reservoir_create(uint64_t cap, uint64_t seed, ReservoirPtr &out) const {
+    if (cap == 0) return Status{1, "invalid config: sample_size must be positive"};
+    SCB_HIP(hipSetDevice(device_));
+    ReservoirPtr r(new Reservoir);
+    r->cap = cap, r->seed = seed;
+    SCB_HIP(r->slot_win.reserve(cap));
+    SCB_HIP(r->slot_val.reserve(cap));
+    fill(reinterpret_cast<uint32_t *>(r->slot_win.p), 2 * cap, 0, nullptr);
+    fill(r->slot_val.p, cap, 0, nullptr);
+    SCB_HIP(hipGetLastError());
+    SCB_HIP(hipStreamSynchronize(nullptr));
+    out = std::move(r);
+    return Status::ok();
+}
+
+// Queries the batch (every window's estimate and validity), numbers its valid
+// windows after the ones seen before, and updates the reservoir.
+Status Engine::sample_accumulate(const DeviceBatch &b, Reservoir &rs) const {
+    const uint64_t npos = b.n_bases;
+    if (npos == 0 || b.n_reads == 0) return Status::ok();
+    SCB_HIP(hipSetDevice(device_));
+    const uint64_t nwords = (npos + 31) / 32;
+    const uint64_t nblocks = (nwords + kRankThreads - 1) / kRankThreads;
+    SCB_HIP(rs.values.reserve(npos * geo_.bytes + 4));
+    SCB_HIP(rs.bits.reserve(nwords + 1));
+    SCB_HIP(rs.rank.reserve(nwords + 1));
+    SCB_HIP(rs.block_sums.reserve(nblocks + 1));
+    QueryResult qr;
+    Status s = query(b, kBatched, rs.values.p, rs.bits.p, qr);
+    if (!s.good()) return s;
+    hipStream_t st = nullptr;
+    hipLaunchKernelGGL(k_rank_sums, dim3(unsigned(nblocks)), dim3(kRankThreads), 0, st, rs.bits.p, nwords,
+                       rs.block_sums.p);
+    hipLaunchKernelGGL(k_rank_scan_blocks, dim3(1), dim3(kRankThreads), 0, st, rs.block_sums.p, nblocks);
+    hipLaunchKernelGGL(k_rank_words, dim3(unsigned(nblocks)), dim3(kRankThreads), 0, st, rs.bits.p, nwords,
+                       rs.block_sums.p, rs.rank.p);
+    const unsigned grid = unsigned(std::min<uint64_t>((nwords + 255) / 256, 65536));
+    hipLaunchKernelGGL(k_reservoir_hits, dim3(grid), dim3(256), 0, st, rs.bits.p, nwords, rs.rank.p, rs.seen, rs.cap,
+                       rs.seed, rs.slot_win.p);
+    switch (geo_.bits) {
+        case 8:
+            hipLaunchKernelGGL(k_reservoir_take<8>, dim3(grid), dim3(256), 0, st, rs.bits.p, nwords, rs.rank.p,
+                               rs.seen, rs.cap, rs.seed, rs.slot_win.p, rs.values.p, rs.slot_val.p);
+            break;
+        case 16:
+            hipLaunchKernelGGL(k_reservoir_take<16>, dim3(grid), dim3(256), 0, st, rs.bits.p, nwords, rs.rank.p,
+                               rs.seen, rs.cap, rs.seed, rs.slot_win.p, rs.values.p, rs.slot_val.p);
+            break;
+        default:
+            hipLaunchKernelGGL(k_reservoir_take<32>, dim3(grid), dim3(256), 0, st, rs.bits.p, nwords, rs.rank.p,
+                               rs.seen, rs.cap, rs.seed, rs.slot_win.p, rs.values.p, rs.slot_val.p);
+            break;
+    }
+    SCB_HIP(hipGetLastError());
+    SCB_HIP(hipStreamSynchronize(st));
+    rs.seen += qr.queried;
+    return Status::ok();
+}
+
+Status Engine::sample_accumulate_host(const uint8_t *bases, const uint64_t *offsets, uint64_t n,
+                                      Reservoir &rs) const {
+    SCB_HIP(hipSetDevice(device_));
+    auto st = acquire();
+    if (!st) return Status{8, "device error: cannot create a workspace"};
+    Status s = for_each_staged_chunk(*st, bases, offsets, n, kSpectrumChunkBases,
+                                     [&](const DeviceBatch &b) { return sample_accumulate(b, rs); });
+    release(std::move(st));
+    return s;
+}
+
+Status Engine::reservoir_values(const Reservoir &rs, std::vector<uint32_t> &vals, uint64_t &seen) const {
+    SCB_HIP(hipSetDevice(device_));
+    const uint64_t n = std::min(rs.seen, rs.cap);
+    vals.resize(n);
+    if (n) SCB_HIP(hipMemcpy(vals.data(), rs.slot_val.p, n * 4, hipMemcpyDeviceToHost));
+    seen = rs.seen;
     return Status::ok();
 }
 

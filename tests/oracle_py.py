@@ -77,6 +77,10 @@ def lib():
             "oracle_spectrum_distinct": (C.c_size_t, [C.c_void_p]),
             "oracle_spectrum_rows": (C.c_size_t, [C.c_void_p, u64p, u64p, C.c_size_t]),
             "oracle_spectrum_pairs": (C.c_size_t, [C.c_void_p, u64p, u64p, C.c_size_t]),
+            "oracle_reservoir_windows": (C.c_size_t, [u8p, u64p, C.c_size_t, C.c_size_t, C.c_uint64, C.c_uint64,
+                                                      u64p, u64p]),
+            "oracle_sampled_spectrum": (C.c_size_t, [C.c_void_p, u8p, u64p, C.c_size_t, C.c_uint64, C.c_uint64,
+                                                     u64p, C.c_size_t, u64p]),
             "oracle_save": (C.c_int, [C.c_void_p, C.c_uint64, C.c_char_p, C.c_char_p, C.c_size_t]),
             "oracle_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(OracleConfig), u64p,
                                       C.c_char_p, C.c_size_t]),
@@ -250,6 +254,19 @@ class Filter:
         buf = (C.c_uint8 * nbytes.value).from_address(ptr)
         return np.frombuffer(buf, dtype=dt)
 
+    def sampled_spectrum(self, bases, offsets, sample_size, seed):
+        """bin/sampled_spectrum.rs: rows [(estimate, sampled, linear_scaled, distinct)]
+        and (seen, sampled, queried, invalid)."""
+        b = _b(bases)
+        o = np.ascontiguousarray(offsets, dtype=np.uint64)
+        cap = 1 << 16
+        rows = np.zeros(4 * cap, dtype=np.uint64)
+        st = np.zeros(4, dtype=np.uint64)
+        n = lib().oracle_sampled_spectrum(self.h, _p(b), _p(o, u64p), o.size - 1, sample_size, seed,
+                                          _p(rows, u64p), cap, _p(st, u64p))
+        r = rows[:4 * min(n, cap)].reshape(-1, 4)
+        return [tuple(int(x) for x in row) for row in r], tuple(int(x) for x in st)
+
     def counters(self):
         n = lib().oracle_filter_counter_slots(self.h)
         out = np.zeros(n, dtype=np.uint32)
@@ -364,6 +381,17 @@ def cpu_baseline(cfg, ib, io, qb, qo):
     if rc:
         raise ValueError("invalid config")
     return r
+
+
+def reservoir_windows(bases, offsets, k, sample_size, seed):
+    """bin/sampled_spectrum.rs's reservoir: window number per slot, and windows seen."""
+    b = _b(bases)
+    o = np.ascontiguousarray(offsets, dtype=np.uint64)
+    out = np.zeros(max(1, min(int(o[-1]) + 1, sample_size)), dtype=np.uint64)
+    seen = C.c_uint64()
+    n = lib().oracle_reservoir_windows(_p(b), _p(o, u64p), o.size - 1, k, sample_size, seed, _p(out, u64p),
+                                       C.byref(seen))
+    return out[:n], seen.value
 
 
 def insert_records(filt, bases, offsets):

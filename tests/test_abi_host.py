@@ -218,3 +218,17 @@ def test_primitives_match_oracle():
     for L in (1, 7, 21, 25, 31, 32):
         for pos in (0, 3):
             assert scb.canonical_kmer(seq, L, pos) == O.canonical_kmer(seq, L, pos)
+
+
+def test_sampled_spectrum_host_checks_and_csv(tmp_path):
+    """bin/sampled_spectrum.rs:100-105: a zero sample size is an InvalidConfig
+    before any work; write_scaled_spectrum's CSV (223-253)."""
+    L = scb.api.lib()
+    n = C.c_size_t()
+    rc = L.scb_index_sampled_spectrum(None, None, None, 0, 0, 1, None, 0, C.byref(n), None)
+    assert rc == 1 and _lib.last_error() == "invalid config: sample_size must be positive"
+    s = scb.SampledSpectrum([(1, 5, 50, 50), (3, 2, 20, 7)], 70, 7, 7, 0)
+    p = tmp_path / "s.csv"
+    s.write(p)
+    assert p.read_text() == ("estimated_count,sampled_kmers,linear_scaled_windows,estimated_distinct_kmers\n"
+                             "1,5,50,50\n3,2,20,7\n")
