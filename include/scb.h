@@ -122,6 +122,26 @@ int scb_device_count(void);
  * method reports its single fused kernel in ms[0] and ms[3]). */
 void scb_last_stage_times(double *ms);
 
+/* extension: tunables of the batched pipeline (no counterpart in the
+ * reference; every valid setting gives identical results). An engine starts
+ * from defaults for its geometry, replaced by the matching entry of the tuning
+ * cache: the JSON file named by SCB_TUNING_CACHE, else tuning.json beside
+ * libscb.so, or one loaded with scb_tuning_cache_load (written by the
+ * autotuner scbf_amd/tune.py). */
+typedef struct scb_tuning {
+    uint32_t insert_bin_log;        /* slots per insert bin, log2: [max(bse,1), min(15,cse)] */
+    uint32_t query_bin_log;         /* slots per query bin, log2: [bse, cse], bin <= 128 KiB */
+    uint32_t spectrum_bin_log;      /* slots per spectrum bin, log2 (used when bse <= 14) */
+    uint32_t insert_item_records;   /* records per insert work item: [1, 2048] */
+    uint32_t query_item_records;    /* records per query work item: [1, 3584] */
+    uint32_t spectrum_item_records; /* records per spectrum work item: [1, 4096] */
+    uint32_t use_graph;             /* 1: replay repeated device-batch insert/query pipelines
+                                       from a captured HIP graph */
+    uint32_t _pad;
+} scb_tuning;
+/* Replaces the process-wide tuning cache (engines created afterwards use it). */
+int scb_tuning_cache_load(const char *path);
+
 /* ---- builder ------------------------------------------------------------- */
 /* SuperCountingBloomBuilder::new (api.rs:102-121): allocates and zeroes the
  * 2^counter_slots_exponent-counter table in HBM. */
@@ -158,6 +178,9 @@ void scb_builder_config(const scb_builder *b, scb_config *out);
 int scb_builder_set_method(scb_builder *b, int method);
 /* extension: copy the counter table (physical = .scb payload order) to host */
 int scb_builder_counters(const scb_builder *b, void *out, size_t bytes);
+/* extension: the engine's tuning (an invalid setting is SCB_ERR_INVALID_CONFIG) */
+int scb_builder_set_tuning(scb_builder *b, const scb_tuning *t);
+void scb_builder_tuning(const scb_builder *b, scb_tuning *out);
 
 /* ---- index --------------------------------------------------------------- */
 /* SuperCountingBloomIndex::build_from_fastx (api.rs:211-223; fastx.rs:206-253,
@@ -169,6 +192,8 @@ void scb_index_config(const scb_index *idx, scb_config *out);   /* api.rs:225-22
 uint64_t scb_index_inserted_kmers(const scb_index *idx);         /* api.rs:229-231 */
 int scb_index_set_method(scb_index *idx, int method);            /* extension */
 int scb_index_counters(const scb_index *idx, void *out, size_t bytes); /* extension */
+int scb_index_set_tuning(scb_index *idx, const scb_tuning *t);       /* extension */
+void scb_index_tuning(const scb_index *idx, scb_tuning *out);         /* extension */
 /* SuperCountingBloomIndex::save / load (api.rs:233-248, 388-608): byte-exact
  * .scb files (139-byte header, then shard-major little-endian counters) */
 int scb_index_save(const scb_index *idx, const char *path);

@@ -66,6 +66,12 @@ class ScbSampledStats(C.Structure):
                [("build_seconds", C.c_double), ("sample_seconds", C.c_double)]
 
 
+class ScbTuning(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("insert_bin_log", "query_bin_log", "spectrum_bin_log",
+                                          "insert_item_records", "query_item_records", "spectrum_item_records",
+                                          "use_graph", "_pad")]
+
+
 class ScbServeConfig(C.Structure):
     _fields_ = [("max_batch_bases", C.c_uint64), ("max_batch_requests", C.c_uint64),
                 ("hbm_budget_bytes", C.c_uint64), ("workers", C.c_uint32), ("max_pending", C.c_uint32)]
@@ -109,6 +115,11 @@ SIGNATURES = {
     "scb_index_config": (None, [C.c_void_p, C.POINTER(ScbConfig)]),
     "scb_index_inserted_kmers": (C.c_uint64, [C.c_void_p]),
     "scb_index_set_method": (C.c_int, [C.c_void_p, C.c_int]),
+    "scb_index_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(ScbTuning)]),
+    "scb_index_tuning": (None, [C.c_void_p, C.POINTER(ScbTuning)]),
+    "scb_builder_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(ScbTuning)]),
+    "scb_builder_tuning": (None, [C.c_void_p, C.POINTER(ScbTuning)]),
+    "scb_tuning_cache_load": (C.c_int, [C.c_char_p]),
     "scb_index_counters": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "scb_index_save": (C.c_int, [C.c_void_p, C.c_char_p]),
     "scb_index_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),

@@ -12,7 +12,7 @@ import numpy as np
 
 from ._lib import (VISITOR, ScbAddReport, ScbBuildReport, ScbConfig, ScbQueryReport, ScbSampledRow,
                    ScbSampledStats, ScbServeConfig, ScbServed, ScbSpectrumStats, ScbStreamStats, ScbSummary,
-                   last_error, lib, u8p, u32p, u64p)
+                   ScbTuning, last_error, lib, u8p, u32p, u64p)
 
 ApproxSpectrum = List[Tuple[int, int]]
 
@@ -203,6 +203,33 @@ def _path(p):
     return os.fspath(p).encode()
 
 
+@dataclass
+class Tuning:
+    """Tunables of the batched pipeline (include/scb.h scb_tuning); results are
+    identical for every valid setting."""
+    insert_bin_log: int = 15
+    query_bin_log: int = 15
+    spectrum_bin_log: int = 14
+    insert_item_records: int = 2048
+    query_item_records: int = 3584
+    spectrum_item_records: int = 4096
+    use_graph: int = 0
+
+    def _c(self):
+        return ScbTuning(self.insert_bin_log, self.query_bin_log, self.spectrum_bin_log, self.insert_item_records,
+                         self.query_item_records, self.spectrum_item_records, self.use_graph, 0)
+
+    @staticmethod
+    def _from_c(c):
+        return Tuning(c.insert_bin_log, c.query_bin_log, c.spectrum_bin_log, c.insert_item_records,
+                      c.query_item_records, c.spectrum_item_records, c.use_graph)
+
+
+def tuning_cache_load(path):
+    """Replaces the process-wide tuning cache; engines created afterwards use it."""
+    _check(lib().scb_tuning_cache_load(_path(path)))
+
+
 class SuperCountingBloomBuilder:  # api.rs:43-48, 101-208
     def __init__(self, cfg: SuperCountingBloomConfig):
         self._h = C.c_void_p()
@@ -283,6 +310,15 @@ class SuperCountingBloomBuilder:  # api.rs:43-48, 101-208
     # extensions
     def set_method(self, method: int):
         _check(lib().scb_builder_set_method(self._handle(), method))
+
+    def set_tuning(self, t: "Tuning"):
+        c = t._c()
+        _check(lib().scb_builder_set_tuning(self._handle(), C.byref(c)))
+
+    def tuning(self) -> "Tuning":
+        c = ScbTuning()
+        lib().scb_builder_tuning(self._handle(), C.byref(c))
+        return Tuning._from_c(c)
 
     def counters(self) -> np.ndarray:
         cfg = self.config()
@@ -462,6 +498,15 @@ class SuperCountingBloomIndex:  # api.rs:50-54, 210-386
     # extensions
     def set_method(self, method: int):
         _check(lib().scb_index_set_method(self._h, method))
+
+    def set_tuning(self, t: "Tuning"):
+        c = t._c()
+        _check(lib().scb_index_set_tuning(self._h, C.byref(c)))
+
+    def tuning(self) -> "Tuning":
+        c = ScbTuning()
+        lib().scb_index_tuning(self._h, C.byref(c))
+        return Tuning._from_c(c)
 
     def counters(self) -> np.ndarray:
         cfg = self.config()

@@ -562,6 +562,44 @@ int scb_builder_set_method(scb_builder *b, int method) {
     return SCB_OK;
 }
 
+namespace {
+scb::Tuning from_c(const scb_tuning &c) {
+    scb::Tuning t;
+    t.insert_bin_log = c.insert_bin_log, t.query_bin_log = c.query_bin_log, t.spectrum_bin_log = c.spectrum_bin_log;
+    t.insert_item_records = c.insert_item_records, t.query_item_records = c.query_item_records;
+    t.spectrum_item_records = c.spectrum_item_records, t.use_graph = c.use_graph;
+    return t;
+}
+void to_c(const scb::Tuning &t, scb_tuning *c) {
+    *c = scb_tuning{};
+    c->insert_bin_log = t.insert_bin_log, c->query_bin_log = t.query_bin_log, c->spectrum_bin_log = t.spectrum_bin_log;
+    c->insert_item_records = t.insert_item_records, c->query_item_records = t.query_item_records;
+    c->spectrum_item_records = t.spectrum_item_records, c->use_graph = t.use_graph;
+}
+}  // namespace
+
+int scb_tuning_cache_load(const char *path) {
+    if (!path) return fail(SCB_ERR_INTERNAL_STATE, "internal state error: null path");
+    Status s = scb::load_tuning_cache(path);
+    return s.good() ? SCB_OK : fail(s);
+}
+
+int scb_builder_set_tuning(scb_builder *b, const scb_tuning *t) {
+    if (!t) return fail(SCB_ERR_INTERNAL_STATE, "internal state error: null tuning");
+    Status s = b->engine->set_tuning(from_c(*t));
+    return s.good() ? SCB_OK : fail(s);
+}
+
+void scb_builder_tuning(const scb_builder *b, scb_tuning *out) { to_c(b->engine->tuning(), out); }
+
+int scb_index_set_tuning(scb_index *idx, const scb_tuning *t) {
+    if (!t) return fail(SCB_ERR_INTERNAL_STATE, "internal state error: null tuning");
+    Status s = idx->engine->set_tuning(from_c(*t));
+    return s.good() ? SCB_OK : fail(s);
+}
+
+void scb_index_tuning(const scb_index *idx, scb_tuning *out) { to_c(idx->engine->tuning(), out); }
+
 int scb_builder_counters(const scb_builder *b, void *out, size_t bytes) {
     Status s = b->engine->download(out, bytes);
     return s.good() ? SCB_OK : fail(s);

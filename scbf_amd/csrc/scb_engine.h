@@ -60,13 +60,30 @@ struct SpectrumResult {
 
 enum Method { kBatched = 0, kPlain = 1 };
 
-// Tunables of the batched pipeline (chosen per geometry by the autotuner or the
-// defaults below).
+// Tunables of the batched pipeline, per engine: defaults from the geometry,
+// replaced by a tuning cache entry for the geometry (written by the autotuner,
+// scbf_amd/tune.py) or by set_tuning. Every valid setting gives the same
+// counters and estimates; only the speed differs.
 struct Tuning {
-    uint32_t insert_bin_log = 15;   // slots per insert bin (u32 LDS counts)
-    uint32_t query_bin_log = 15;    // slots per query bin (native-width LDS copy)
-    uint32_t max_records_per_item = 4096;  // hot-bin split size (at most kMaxItemRecords)
+    uint32_t insert_bin_log = 15;         // slots per insert bin (16-bit LDS counts)
+    uint32_t query_bin_log = 15;          // slots per query bin (native-width LDS copy)
+    uint32_t spectrum_bin_log = 14;       // slots per spectrum bin (bse <= 14)
+    uint32_t insert_item_records = 2048;  // hot-bin split sizes (records per work item)
+    uint32_t query_item_records = 3584;
+    uint32_t spectrum_item_records = 4096;
+    uint32_t use_graph = 0;               // replay repeated device-batch pipelines from a HIP graph
 };
+
+// Geometry key of a tuning cache entry.
+struct TuningKey {
+    uint32_t k, m, s, nh, bits, cse, bse;
+    bool operator==(const TuningKey &o) const {
+        return k == o.k && m == o.m && s == o.s && nh == o.nh && bits == o.bits && cse == o.cse && bse == o.bse;
+    }
+};
+// Replaces the process-wide tuning cache with the entries of a JSON file (the
+// format tune.py writes); engines created afterwards use them.
+Status load_tuning_cache(const char *path);
 
 class Workspace;  // device scratch buffers + stream, one per concurrent call
 
@@ -96,7 +113,10 @@ class Engine {
 
     const Geometry &geometry() const { return geo_; }
     int device() const { return device_; }
-    Tuning &tuning() { return tune_; }
+    const Tuning &tuning() const { return tune_; }
+    // checks the setting against the geometry and the kernels' LDS limits
+    Status set_tuning(const Tuning &t);
+    static Tuning default_tuning(const Geometry &g);
 
     Status insert(const DeviceBatch &b, Method method, InsertResult &r);
     // d_values: n_bases counters of geo.bytes each (or null); d_valid_bits:

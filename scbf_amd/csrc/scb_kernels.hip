@@ -26,6 +26,7 @@
 //               read once (query, spectrum)
 // Plain pipeline: k_scan alone, one thread per super-k-mer, global atomics/loads.
 
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,6 +34,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <thread>
 #include <type_traits>
 
@@ -2302,6 +2304,19 @@ class Workspace {
     HistTables hist;               // spectrum histogram
     uint64_t *h_stats = nullptr;   // pinned
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // A captured pipeline (insert or query) and the key of the call it replays.
+    struct GraphSlot {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        uint64_t key = 0, eager_key = 0;
+        void drop() {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            exec = nullptr, graph = nullptr, key = 0;
+        }
+    } graph_insert, graph_query;
+    bool capturing = false;    // launches are being captured (no event records)
+    bool stages_timed = true;  // the last call recorded all four stage events
     // A view workspace runs on the null stream, has no events, and its device
     // buffers point into memory it does not own (none of them can grow); its
     // h_stats belongs to the caller.
@@ -2317,7 +2332,23 @@ class Workspace {
         emit_keys.owned = emit_vals.owned = false;
         hist.dense.owned = hist.big_counts.owned = hist.big_keys.owned = false;
     }
+    // Hash of the device buffers a pipeline uses: their addresses and sizes.
+    uint64_t buffer_signature() const {
+        uint64_t h = 0x243F6A8885A308D3ull;
+        auto mix = [&](const void *p, size_t n) {
+            h = (h ^ reinterpret_cast<uintptr_t>(p)) * 0x100000001B3ull;
+            h = (h ^ n) * 0x100000001B3ull;
+        };
+        mix(pk.p, pk.n), mix(rec.p, rec.n), mix(sorted.p, sorted.n), mix(work.p, work.n);
+        mix(stats.p, stats.n), mix(plan_blk.p, plan_blk.n), mix(bin_count.p, bin_count.n);
+        mix(bin_off.p, bin_off.n), mix(cursor.p, cursor.n), mix(ccursor.p, ccursor.n);
+        mix(coarse_hist.p, coarse_hist.n), mix(seg_count.p, seg_count.n), mix(tile_r0.p, tile_r0.n);
+        mix(spec_bin_kmers.p, spec_bin_kmers.n);
+        return h;
+    }
     ~Workspace() {
+        graph_insert.drop();
+        graph_query.drop();
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
         if (h_stats && !is_view) (void)hipHostFree(h_stats);
@@ -2341,13 +2372,19 @@ Status make_workspace(std::unique_ptr<Workspace> &ws) {
 thread_local StageTimes g_stage_times;
 
 void mark(Workspace &ws, int i) {
-    if (ws.ev[i]) (void)hipEventRecord(ws.ev[i], ws.stream);
+    if (ws.ev[i] && !ws.capturing) (void)hipEventRecord(ws.ev[i], ws.stream);
 }
 
 // Elapsed ms between recorded events (after the stream has been synchronized).
 void collect_times(Workspace &ws, bool batched) {
     StageTimes t{};
     float ms = 0;
+    if (!ws.stages_timed) {  // a replayed graph: only the total is known
+        if (hipEventElapsedTime(&ms, ws.ev[0], ws.ev[3]) == hipSuccess) t.total_ms = ms;
+        t.batched = batched;
+        g_stage_times = t;
+        return;
+    }
     if (hipEventElapsedTime(&ms, ws.ev[0], ws.ev[1]) == hipSuccess) t.scan_ms = ms;
     if (batched) {
         if (hipEventElapsedTime(&ms, ws.ev[1], ws.ev[2]) == hipSuccess) t.plan_scatter_ms = ms;
@@ -2497,6 +2534,177 @@ void table_release(int device, uint64_t bytes, void *p) {
 static_assert(kMaxItemRecords <= 65536, "record order indices are u16");
 static_assert(kQueryItemRecords % kQueryThreads == 0, "query items are walked in whole rounds");
 
+// ---- tuning -------------------------------------------------------------------
+Tuning Engine::default_tuning(const Geometry &g) {
+    Tuning t;
+    // keep query bins within LDS at every width (u32 counters: 2^15 slots = 128 KiB)
+    t.insert_bin_log = std::min<uint32_t>(15, g.cse);
+    t.query_bin_log = std::min<uint32_t>(15, g.cse);
+    t.spectrum_bin_log = std::min<uint32_t>(14, g.cse);
+    t.insert_item_records = kInsertItemRecords;
+    t.query_item_records = kQueryItemRecords;
+    t.spectrum_item_records = kMaxItemRecords;
+    return t;
+}
+
+Status Engine::set_tuning(const Tuning &t) {
+    const Geometry &g = geo_;
+    auto bad = [](const std::string &m) { return Status{1, "invalid config: tuning: " + m}; };
+    const uint32_t lo = std::max<uint32_t>(g.bse, 1);
+    if (t.insert_bin_log < lo || t.insert_bin_log > std::min<uint32_t>(15, g.cse))
+        return bad("insert_bin_log must lie in [max(block_slots_exponent, 1), min(15, counter_slots_exponent)]");
+    if (t.query_bin_log < g.bse || t.query_bin_log > g.cse || (uint64_t(1) << t.query_bin_log) * g.bytes > (128u << 10))
+        return bad("query_bin_log must lie in [block_slots_exponent, counter_slots_exponent] with a bin of at most "
+                   "128 KiB");
+    if (t.spectrum_bin_log < std::min<uint32_t>(g.bse, 14) || t.spectrum_bin_log > std::min<uint32_t>(14, g.cse))
+        return bad("spectrum_bin_log must lie in [min(block_slots_exponent, 14), min(14, counter_slots_exponent)]");
+    if (t.insert_item_records < 1 || t.insert_item_records > kInsertItemRecords)
+        return bad("insert_item_records must lie in [1, " + std::to_string(kInsertItemRecords) + "]");
+    if (t.query_item_records < 1 || t.query_item_records > kQueryItemRecords)
+        return bad("query_item_records must lie in [1, " + std::to_string(kQueryItemRecords) + "]");
+    if (t.spectrum_item_records < 1 || t.spectrum_item_records > kMaxItemRecords)
+        return bad("spectrum_item_records must lie in [1, " + std::to_string(kMaxItemRecords) + "]");
+    if (t.use_graph > 1) return bad("use_graph must be 0 or 1");
+    tune_ = t;
+    return Status::ok();
+}
+
+namespace {
+struct TuningCache {
+    std::mutex mu;
+    bool loaded = false;
+    std::vector<std::pair<TuningKey, Tuning>> entries;
+};
+TuningCache &tuning_cache() {
+    static TuningCache *c = new TuningCache;
+    return *c;
+}
+
+// Integer value of "name": in one JSON object's text (no nesting of the same
+// name); false when absent.
+bool json_int(const std::string &obj, const char *name, uint64_t &out) {
+    const std::string key = std::string("\"") + name + "\"";
+    size_t p = obj.find(key);
+    if (p == std::string::npos) return false;
+    p = obj.find(':', p + key.size());
+    if (p == std::string::npos) return false;
+    ++p;
+    while (p < obj.size() && (obj[p] == ' ' || obj[p] == '\t' || obj[p] == '\n' || obj[p] == '\r')) ++p;
+    if (p >= obj.size() || obj[p] < '0' || obj[p] > '9') return false;
+    out = std::strtoull(obj.c_str() + p, nullptr, 10);
+    return true;
+}
+
+// Entries are the objects at depth 2 of {"entries": [ {...}, ... ]}; an object's
+// nested objects (timings) are part of its text and hold no key of its own.
+Status parse_tuning_json(const std::string &text, std::vector<std::pair<TuningKey, Tuning>> &out) {
+    const size_t arr = text.find("\"entries\"");
+    if (arr == std::string::npos) return Status{1, "invalid config: tuning cache has no \"entries\""};
+    size_t p = text.find('[', arr);
+    if (p == std::string::npos) return Status{1, "invalid config: tuning cache \"entries\" is not a list"};
+    int depth = 0;
+    size_t start = 0;
+    bool in_str = false;
+    for (++p; p < text.size(); ++p) {
+        const char c = text[p];
+        if (in_str) {
+            if (c == '\\') ++p;
+            else if (c == '"') in_str = false;
+            continue;
+        }
+        if (c == '"') in_str = true;
+        else if (c == '{') {
+            if (depth++ == 0) start = p;
+        } else if (c == '}') {
+            if (--depth == 0) {
+                const std::string obj = text.substr(start, p - start + 1);
+                // the entry's own fields precede its nested timing object
+                const size_t nested = obj.find('{', 1);
+                const std::string head = nested == std::string::npos ? obj : obj.substr(0, nested);
+                uint64_t v[14];
+                const char *names[14] = {"k", "m", "s", "n_hashes", "counter_bits", "counter_slots_exponent",
+                                         "block_slots_exponent", "insert_bin_log", "query_bin_log",
+                                         "spectrum_bin_log", "insert_item_records", "query_item_records",
+                                         "spectrum_item_records", "use_graph"};
+                for (int i = 0; i < 14; ++i)
+                    if (!json_int(head, names[i], v[i]))
+                        return Status{1, std::string("invalid config: tuning cache entry without \"") + names[i] +
+                                             "\""};
+                TuningKey key{uint32_t(v[0]), uint32_t(v[1]), uint32_t(v[2]), uint32_t(v[3]), uint32_t(v[4]),
+                              uint32_t(v[5]), uint32_t(v[6])};
+                Tuning t;
+                t.insert_bin_log = uint32_t(v[7]), t.query_bin_log = uint32_t(v[8]);
+                t.spectrum_bin_log = uint32_t(v[9]), t.insert_item_records = uint32_t(v[10]);
+                t.query_item_records = uint32_t(v[11]), t.spectrum_item_records = uint32_t(v[12]);
+                t.use_graph = uint32_t(v[13]);
+                out.emplace_back(key, t);
+            }
+        } else if (c == ']' && depth == 0) {
+            break;
+        }
+    }
+    return Status::ok();
+}
+
+Status load_tuning_file(const char *path, std::vector<std::pair<TuningKey, Tuning>> &out) {
+    FILE *fp = std::fopen(path, "rb");
+    if (!fp) return Status{7, std::string("I/O error for '") + path + "': cannot open the tuning cache"};
+    std::string text;
+    char buf[65536];
+    size_t got;
+    while ((got = std::fread(buf, 1, sizeof(buf), fp)) > 0) text.append(buf, got);
+    std::fclose(fp);
+    return parse_tuning_json(text, out);
+}
+
+// The cache a process starts with: SCB_TUNING_CACHE, else tuning.json beside
+// libscb.so (absent files leave the defaults).
+void load_default_tuning_cache(TuningCache &c) {
+    c.loaded = true;
+    std::string path;
+    if (const char *e = std::getenv("SCB_TUNING_CACHE")) {
+        path = e;
+    } else {
+        Dl_info info;
+        if (dladdr(reinterpret_cast<void *>(&load_tuning_cache), &info) && info.dli_fname) {
+            path = info.dli_fname;
+            const size_t slash = path.rfind('/');
+            path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/tuning.json";
+        }
+    }
+    if (path.empty()) return;
+    std::vector<std::pair<TuningKey, Tuning>> entries;
+    if (load_tuning_file(path.c_str(), entries).good()) c.entries.swap(entries);
+}
+
+// The cache entry of the geometry, if any.
+bool cached_tuning(const Geometry &g, Tuning &t);
+}  // namespace
+
+Status load_tuning_cache(const char *path) {
+    std::vector<std::pair<TuningKey, Tuning>> entries;
+    Status s = load_tuning_file(path, entries);
+    if (!s.good()) return s;
+    TuningCache &c = tuning_cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    c.entries.swap(entries);
+    c.loaded = true;
+    return Status::ok();
+}
+
+namespace {
+bool cached_tuning(const Geometry &g, Tuning &t) {
+    TuningCache &c = tuning_cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    if (!c.loaded) load_default_tuning_cache(c);
+    const TuningKey key{g.k, g.m, g.s, g.nh, g.bits, g.cse, g.bse};
+    bool have = false;
+    for (auto &e : c.entries)
+        if (e.first == key) t = e.second, have = true;
+    return have;
+}
+}  // namespace
+
 Status Engine::create(const Geometry &g, std::unique_ptr<Engine> &out) {
     if (device_count() <= 0) return Status{8, "device error: no HIP device is visible"};
     std::unique_ptr<Engine> e(new Engine);
@@ -2505,8 +2713,9 @@ Status Engine::create(const Geometry &g, std::unique_ptr<Engine> &out) {
     SCB_HIP(table_alloc(e->device_, std::max<uint64_t>(g.table_bytes(), 4), &e->d_table_));
     e->zero_pending_ = true;
     // keep query bins within LDS at every width (u32 counters: 2^15 slots = 128 KiB)
-    e->tune_.insert_bin_log = std::min<uint32_t>(15, g.cse);
-    e->tune_.query_bin_log = std::min<uint32_t>(15, g.cse);
+    e->tune_ = default_tuning(g);
+    Tuning cached;
+    if (cached_tuning(g, cached)) (void)e->set_tuning(cached);  // an entry the kernels cannot run is ignored
     out = std::move(e);
     return Status::ok();
 }
@@ -2763,6 +2972,68 @@ bool batched_ok(const Geometry &g, uint32_t bin_log) {
 
 }  // namespace
 
+namespace {
+uint64_t key_of(std::initializer_list<uint64_t> v) {
+    uint64_t h = 0xCBF29CE484222325ull;
+    for (uint64_t x : v) {
+        h = (h ^ x) * 0x100000001B3ull;
+        h ^= h >> 29;
+    }
+    return h;
+}
+
+// Runs body (the launches of one batched pipeline, ending before the stats
+// copy) eagerly, or from a HIP graph: the first call with a key runs eagerly
+// and leaves its buffers sized; the next call with the same key (and so the
+// same buffers) is captured and launched; later ones replay the graph. Stage
+// events are recorded only around a replay (total time).
+template <class Body>
+Status run_graphable(Workspace &ws, bool use_graph, Workspace::GraphSlot &slot, uint64_t args_key, Body &&body) {
+    ws.stages_timed = true;
+    if (!use_graph || ws.is_view) return body();
+    const uint64_t key = key_of({args_key, ws.buffer_signature()});
+    if (slot.exec && slot.key == key) {
+        ws.stages_timed = false;
+        if (ws.ev[0]) (void)hipEventRecord(ws.ev[0], ws.stream);
+        SCB_HIP(hipGraphLaunch(slot.exec, ws.stream));
+        if (ws.ev[3]) (void)hipEventRecord(ws.ev[3], ws.stream);
+        return Status::ok();
+    }
+    if (slot.eager_key != key) {
+        Status s = body();
+        slot.eager_key = key_of({args_key, ws.buffer_signature()});
+        return s;
+    }
+    slot.drop();
+    ws.capturing = true;
+    hipError_t e = hipStreamBeginCapture(ws.stream, hipStreamCaptureModeRelaxed);
+    Status s = e == hipSuccess ? body() : Status{8, std::string("device error: hipStreamBeginCapture: ") +
+                                                       hipGetErrorString(e)};
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(ws.stream, &g);
+    ws.capturing = false;
+    if (s.good() && e2 != hipSuccess)
+        s = Status{8, std::string("device error: hipStreamEndCapture: ") + hipGetErrorString(e2)};
+    if (s.good()) {
+        hipError_t e3 = hipGraphInstantiate(&slot.exec, g, nullptr, nullptr, 0);
+        if (e3 != hipSuccess) s = Status{8, std::string("device error: hipGraphInstantiate: ") + hipGetErrorString(e3)};
+    }
+    if (!s.good()) {
+        if (g) (void)hipGraphDestroy(g);
+        slot.exec = nullptr;
+        slot.eager_key = 0;
+        return s;
+    }
+    slot.graph = g;
+    slot.key = key;
+    ws.stages_timed = false;
+    if (ws.ev[0]) (void)hipEventRecord(ws.ev[0], ws.stream);
+    SCB_HIP(hipGraphLaunch(slot.exec, ws.stream));
+    if (ws.ev[3]) (void)hipEventRecord(ws.ev[3], ws.stream);
+    return Status::ok();
+}
+}  // namespace
+
 Status Engine::insert(const DeviceBatch &b, Method method, InsertResult &r) {
     r = InsertResult{};
     Status s = check_batch(geo_, b);
@@ -2791,10 +3062,15 @@ Status Engine::insert(const DeviceBatch &b, Method method, InsertResult &r) {
         s = launch_scan<kModePlainInsert>(L, b, 0, nullptr, nullptr);
         mark(*ws, 1);
     } else {
+        const uint64_t args =
+            key_of({1, uint64_t(uintptr_t(b.d_bases)), uint64_t(uintptr_t(b.d_offsets)), b.n_reads, b.n_bases,
+                    uint64_t(uintptr_t(d_table_)), uint64_t(fresh), tune_.insert_bin_log, tune_.insert_item_records});
+        s = run_graphable(*ws, tune_.use_graph, ws->graph_insert, args, [&]() -> Status {
+        Status s;
         uint64_t nbins = 0;
         uint32_t bin_log = tune_.insert_bin_log;
         const uint32_t per_item =
-            std::max<uint32_t>(1, std::min<uint32_t>(tune_.max_records_per_item, kInsertItemRecords));
+            std::max<uint32_t>(1, std::min<uint32_t>(tune_.insert_item_records, kInsertItemRecords));
         // k-mers per bin select the bins counted by distinct record
         SCB_HIP(ws->spec_bin_kmers.reserve(uint64_t(1) << (geo_.cse - bin_log)));
         s = batched_front(L, b, bin_log, per_item, nullptr, nbins, ws->spec_bin_kmers.p);
@@ -2839,6 +3115,8 @@ Status Engine::insert(const DeviceBatch &b, Method method, InsertResult &r) {
                 mark(*ws, 3);
             }
         }
+        return s;
+        });
     }
     if (s.good()) s = finish(*ws);
     if (s.good() && fresh) {
@@ -2890,10 +3168,16 @@ Status Engine::run_query(Workspace &wsr, const DeviceBatch &b, Method method, vo
         s = launch_scan<kModePlainQuery>(L, b, 0, d_valid_bits, d_values);
         mark(*ws, 1);
     } else {
+        const uint64_t args =
+            key_of({2, uint64_t(uintptr_t(b.d_bases)), uint64_t(uintptr_t(b.d_offsets)), b.n_reads, b.n_bases,
+                    uint64_t(uintptr_t(d_table_)), uint64_t(uintptr_t(d_values)), uint64_t(uintptr_t(d_valid_bits)),
+                    tune_.query_bin_log, tune_.query_item_records});
+        s = run_graphable(*ws, tune_.use_graph, ws->graph_query, args, [&]() -> Status {
+        Status s;
         uint64_t nbins = 0;
         const uint32_t bin_log = tune_.query_bin_log;
         const uint32_t per_item =
-            std::max<uint32_t>(1, std::min<uint32_t>(tune_.max_records_per_item, kQueryItemRecords));
+            std::max<uint32_t>(1, std::min<uint32_t>(tune_.query_item_records, kQueryItemRecords));
         s = batched_front(L, b, bin_log, per_item, d_valid_bits, nbins);
         if (s.good()) {
             BinArgs a{};
@@ -2941,6 +3225,8 @@ Status Engine::run_query(Workspace &wsr, const DeviceBatch &b, Method method, vo
                 mark(*ws, 3);
             }
         }
+        return s;
+        });
     }
     if (s.good()) s = finish(*ws);
     if (s.good()) {
@@ -3046,7 +3332,7 @@ Status Engine::query_noalloc(const DeviceBatch &b, Method method, void *d_values
     ws.h_stats = host_stats;
     const bool batched = method == kBatched && batched_ok(geo_, tune_.query_bin_log) && b.n_bases >= geo_.k;
     const uint32_t per_item =
-        std::max<uint32_t>(1, std::min<uint32_t>(tune_.max_records_per_item, kQueryItemRecords));
+        std::max<uint32_t>(1, std::min<uint32_t>(tune_.query_item_records, kQueryItemRecords));
     const uint64_t need = carve_query(base, geo_, b.n_bases, tune_.query_bin_log, per_item, batched, ws);
     if (need > kNoAllocScratchBytes)
         return Status{6, "internal state error: the batch needs " + std::to_string(need) +
@@ -3111,9 +3397,9 @@ Status Engine::spectrum_checks(const DeviceBatch &b) const {
 Status Engine::spectrum_pass(Workspace &ws, const DeviceBatch &b, const void *hist, bool emit) const {
     Launch L{geo_, dev_geo(geo_), ws, d_table_, device_};
     hipStream_t st = ws.stream;
-    const uint32_t bin_log = geo_.bse <= 14 ? std::min<uint32_t>(14, geo_.cse) : geo_.bse;
+    const uint32_t bin_log = geo_.bse <= 14 ? std::min<uint32_t>(tune_.spectrum_bin_log, geo_.cse) : geo_.bse;
     const uint32_t stage = (geo_.bse <= 14 && (uint64_t(1) << bin_log) * geo_.bytes >= 4) ? 1 : 0;
-    const uint32_t per_item = std::max<uint32_t>(1, std::min<uint32_t>(tune_.max_records_per_item, kMaxItemRecords));
+    const uint32_t per_item = std::max<uint32_t>(1, std::min<uint32_t>(tune_.spectrum_item_records, kMaxItemRecords));
     const uint64_t nbins = uint64_t(1) << (geo_.cse - bin_log);
     SCB_HIP(ws.spec_bin_kmers.reserve(nbins));
     if (emit) {
