@@ -12,7 +12,11 @@ Prints one JSON line. `value` is inserted+queried k-mers per second of the
 batched pipeline; the plain one-launch-per-batch method is timed on the same
 inputs and checked bit-exact against it; `cpu_baseline` is the oracle's
 restatement of the reference's multithreaded CPU path (1 producer, bounded
-queue, T workers, 1024 mutex shards) on a bounded sample of the same reads.
+queue, T workers, 1024 mutex shards) on a bounded sample of the same reads
+(median of three runs, at T = host threads and at T = 1). `configs` holds the
+same measurement for the long-read (8-bit) and skewed-mixture (32-bit, 2^32
+slots) workloads, driven in 256 MB device batches; `served` sweeps the serving
+loop's request rate up to the point where p99 passes 10 ms.
 """
 import argparse
 import ctypes as C
@@ -37,11 +41,7 @@ def log(*a):
 
 def gen_workload(seed=1):
     from scbf_amd import synth
-    g = synth.genome(5_000_000, seed)
-    ins_b, ins_o = synth.coverage_reads(g, 50, 150, seed + 1, sub_rate=0.01, n_rate=0.001)
-    new_b, new_o = synth.coverage_reads(g, 10, 150, seed + 2, sub_rate=0.01, n_rate=0.001)
-    q_b = np.concatenate([ins_b, new_b])
-    q_o = np.concatenate([ins_o, new_o[1:] + ins_o[-1]]).astype(np.uint64)
+    _, ins_b, ins_o, q_b, q_o = synth.default_geometry_workload(seed)
     return ins_b, ins_o, q_b, q_o
 
 
@@ -156,16 +156,115 @@ def run_step(scb, cfg, method, dev, out_vals, out_bits):
     return ir, qr, ist, qst, idx
 
 
+class BatchedSet:
+    """A read set resident in HBM as device batches of at most `limit` bases
+    (split at record boundaries), each with its own offsets array, and output
+    slices (abundances, validity bits) per batch."""
+
+    def __init__(self, bases, offsets, limit, W):
+        from scbf_amd.devmem import DeviceBuffer
+        self.bases = DeviceBuffer.from_array(bases)
+        self.batches = []
+        r0, n = 0, offsets.size - 1
+        out_bytes = bit_words = 0
+        while r0 < n:
+            r1 = int(np.searchsorted(offsets, offsets[r0] + limit, side="right")) - 1
+            r1 = max(r0 + 1, min(n, r1))
+            nb = int(offsets[r1] - offsets[r0])
+            offs = DeviceBuffer.from_array((offsets[r0:r1 + 1] - offsets[r0]).astype(np.uint64))
+            self.batches.append((int(offsets[r0]), r1 - r0, nb, offs, out_bytes, bit_words))
+            out_bytes += (nb * W + 255) // 256 * 256
+            bit_words += (nb + 31) // 32 + 64
+            r0 = r1
+        self.values = DeviceBuffer(out_bytes + 4)
+        self.bits = DeviceBuffer(bit_words * 4 + 4)
+
+    def insert(self, builder):
+        ik = ismers = 0
+        st = np.zeros(4)
+        for o, n, nb, offs, _, _ in self.batches:
+            r = builder.add_sequences_device(self.bases.ptr + o, offs.ptr, n, nb)
+            ik += r.inserted_kmers
+            ismers += r.inserted_smers
+            st += np.array(stage_times_module())
+        return ik, ismers, st
+
+    def query(self, idx):
+        q = pos = cs = 0
+        st = np.zeros(4)
+        for o, n, nb, offs, vo, bw in self.batches:
+            r = idx.estimate_sequences_device(self.bases.ptr + o, offs.ptr, n, nb, self.values.ptr + vo,
+                                              self.bits.ptr + 4 * bw)
+            q += r.valid_kmers
+            pos += r.positive_estimates
+            cs = (cs + r.estimate_checksum) & 0xFFFFFFFFFFFFFFFF
+            st += np.array(stage_times_module())
+        return (q, pos, cs), st
+
+
+def stage_times_module():
+    import scbf_amd as scb
+    return stage_times(scb)
+
+
+def config_bench(scb, name, geo, bases, offsets, steps, limit=256_000_000, workload=""):
+    """Throughput of one workload driven in device batches of `limit` bases:
+    builder + insert of every batch + into_index + query of every batch (outputs
+    in HBM), batched and plain methods; batched == plain is checked on the
+    query reports and the counter histograms."""
+    k, m, s, h, bits, cse, bse = geo
+    cfg = scb.SuperCountingBloomConfig(k=k, m=m, s=s, n_hashes=h, counter_bits=bits, counter_slots_exponent=cse,
+                                       block_slots_exponent=bse, threads=max(1, os.cpu_count() or 1), queue=4096)
+    ds = BatchedSet(bases, offsets, limit, bits // 8)
+
+    def step(method):
+        from scbf_amd.devmem import synchronize
+        synchronize()
+        t0 = time.perf_counter()
+        b = scb.SuperCountingBloomBuilder(cfg)
+        b.set_method(method)
+        ik, ismers, ist = ds.insert(b)
+        idx = b.into_index()
+        idx.set_method(method)
+        rep, qst = ds.query(idx)
+        synchronize()
+        return (time.perf_counter() - t0) * 1e3, ik, ismers, rep, ist, qst, idx
+
+    step(scb.METHOD_BATCHED)  # warm-up
+    runs = [step(scb.METHOD_BATCHED) for _ in range(steps)]
+    ms = float(np.median([r[0] for r in runs]))
+    _, ik, ismers, rep, ist, qst, idx = runs[-1]
+    kmers = ik + rep[0]
+    hist_b = idx.counter_histogram()
+    del runs, idx
+    pms, pik, pismers, prep, _, _, pidx = step(scb.METHOD_PLAIN)
+    exact = bool((pik, pismers, prep) == (ik, ismers, rep) and pidx.counter_histogram() == hist_b)
+    del pidx
+    out = {"workload": workload, "value": round(kmers / ms / 1e6, 3), "unit": "G k-mers/s",
+           "ms_per_step": round(ms, 3), "steps": steps, "inserted_kmers": ik, "queried_kmers": rep[0],
+           "records": int(offsets.size - 1), "bases": int(offsets[-1]), "device_batches": len(ds.batches),
+           "batch_bases": limit,
+           "stage_ms": {"insert": {"scan": round(float(ist[0]), 3), "plan_scatter": round(float(ist[1]), 3),
+                                   "bin": round(float(ist[2]), 3), "total": round(float(ist[3]), 3)},
+                        "query": {"scan": round(float(qst[0]), 3), "plan_scatter": round(float(qst[1]), 3),
+                                  "bin": round(float(qst[2]), 3), "total": round(float(qst[3]), 3)}},
+           "plain": {"value": round(kmers / pms / 1e6, 3), "ms_per_step": round(pms, 3)},
+           "batched_over_plain": round(pms / ms, 2), "bit_exact_batched_vs_plain": exact}
+    log(f"{name}: {out}")
+    return out
+
+
 def lines_touched(counters, line_bytes=64):
     per = line_bytes // counters.itemsize
     return int(np.count_nonzero(counters.reshape(-1, per).any(axis=1)))
 
 
-def cpu_baseline(ins_b, ins_o, q_b, q_o, target_s=15.0):
+def cpu_baseline(ins_b, ins_o, q_b, q_o, target_s=6.0, runs=3):
+    """The oracle's restatement of the reference's threaded path, at T = the host's
+    threads (at most 16) and at T = 1: a sample of the workload sized to about
+    target_s of CPU time, then the median of `runs` runs of that sample."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_py
-    threads = max(1, min(16, os.cpu_count() or 1))
-    cfg = oracle_py.make_cfg(31, 21, 25, 4, 16, 30, 9, threads=threads, queue=4096)
 
     def sample(n_ins, n_q):
         ib, io = ins_b[:int(ins_o[n_ins])], ins_o[:n_ins + 1]
@@ -176,15 +275,29 @@ def cpu_baseline(ins_b, ins_o, q_b, q_o, target_s=15.0):
         qo = np.concatenate([io, qi[1:] - qi[0] + io[-1]]).astype(np.uint64)
         return ib, io, qb, qo
 
-    n = 20_000
-    while True:
-        ib, io, qb, qo = sample(n, n // 5)
-        r = oracle_py.cpu_baseline(cfg, ib, io, qb, qo)
-        t = r.build_s + r.query_s
-        if t >= target_s / 4 or n >= ins_o.size - 1:
-            break
-        n = min(ins_o.size - 1, int(n * max(2.0, target_s / max(t, 1e-3) * 0.8)))
-    kps = (r.indexed_kmers + r.queried_kmers) / t
+    def measure(threads):
+        cfg = oracle_py.make_cfg(31, 21, 25, 4, 16, 30, 9, threads=threads, queue=4096)
+        n = 5_000
+        while True:
+            ib, io, qb, qo = sample(n, n // 5)
+            r = oracle_py.cpu_baseline(cfg, ib, io, qb, qo)
+            t = r.build_s + r.query_s
+            if t >= target_s / 2 or n >= ins_o.size - 1:
+                break
+            n = min(ins_o.size - 1, int(n * max(2.0, target_s / max(t, 1e-3))))
+        rates, last = [], r
+        for _ in range(runs):
+            last = oracle_py.cpu_baseline(cfg, ib, io, qb, qo)
+            rates.append((last.indexed_kmers + last.queried_kmers) / (last.build_s + last.query_s))
+        return {"value": float(np.median(rates)) / 1e9, "runs": [round(x / 1e9, 5) for x in rates],
+                "cores": threads,
+                "sample": f"{n} inserted + {n + n // 5} queried 150 bp reads of the same workload "
+                          f"({last.indexed_kmers} + {last.queried_kmers} k-mers; build {last.build_s:.2f} s, "
+                          f"query {last.query_s:.2f} s), default geometry, 2^30 16-bit counters"}
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    multi = measure(threads)
+    single = measure(1)
     model = ""
     try:
         with open("/proc/cpuinfo") as f:
@@ -194,11 +307,9 @@ def cpu_baseline(ins_b, ins_o, q_b, q_o, target_s=15.0):
                     break
     except OSError:
         pass
-    return {"value": kps / 1e9, "unit": "G k-mers/s", "cores": threads, "kind": "port",
-            "sample": f"{n} inserted + {n + n // 5} queried 150 bp reads of the same workload "
-                      f"({r.indexed_kmers} + {r.queried_kmers} k-mers; build {r.build_s:.2f} s, "
-                      f"query {r.query_s:.2f} s), default geometry, 2^30 16-bit counters",
-            "host_cpus": os.cpu_count(), "cpu_model": model}
+    return {"value": multi["value"], "unit": "G k-mers/s", "cores": threads, "kind": "port",
+            "sample": multi["sample"], "runs": multi["runs"], "statistic": f"median of {runs} runs",
+            "single_thread": single, "host_cpus": os.cpu_count(), "cpu_model": model}
 
 
 PMC_PROFILE = os.path.join(ROOT, "profiles", "pmc_default_geometry.md")
@@ -230,14 +341,23 @@ def copy_probe(mib=1024, reps=3):
     print(json.dumps({"copy_probe_bytes": n, "reps": reps}), flush=True)
 
 
-def serve_bench(scb, idx, genome, qps_steps, seconds, seed=9):
-    """Open-loop serving of the C2 index: Poisson arrivals at each QPS step; request
-    sizes log-uniform (1-10,000 records of 31-20,000 bp; 70% genome substrings on a
-    random strand, 30% random); 80% abundance and 20% spectrum requests. Latency is
-    scheduled arrival -> completion (CLOCK_MONOTONIC), so submit lateness counts."""
+def serve_bench(scb, idx, genome, qps_steps, min_requests=2000, pool_size=160, p99_limit_ms=10.0, seed=9):
+    """Open-loop serving of the default-geometry index: Poisson arrivals at each
+    request rate of the sweep, at least min_requests per rate; request sizes
+    log-uniform (1-10,000 records of 31-20,000 bp; 70% genome substrings on a
+    random strand, 30% random); 80% abundance and 20% spectrum requests. Once per
+    rate, half way through, the worst-case request of 10,000 records of 20 kbp is
+    submitted too (its own latency is reported; it counts in the percentiles).
+    The sweep stops after the first rate whose p99 passes p99_limit_ms. Latency
+    is scheduled arrival -> completion (CLOCK_MONOTONIC), so submit lateness
+    counts."""
     import threading
     from scbf_amd import synth
-    pool = synth.serving_requests(genome, 96, seed)
+    pool = synth.serving_requests(genome, pool_size, seed)
+    worst = synth.serving_requests(genome, 1, seed + 1, n_range=(10_000, 10_000), len_range=(20_000, 20_000),
+                                   genome_frac=1.0)[0]
+    pool.append(worst)
+    wi = len(pool) - 1
     pinned = []
     for b, o in pool:  # request bytes live in page-locked memory like a serving front end's
         hb = scb.host_array(b.size, np.uint8)
@@ -256,6 +376,7 @@ def serve_bench(scb, idx, genome, qps_steps, seconds, seed=9):
         for qps in qps_steps:
             arrivals = {}
             lat, kmers_done, refused = [], [0], [0]
+            worst_ms = [None]
             lock = threading.Lock()
             n_sub = [0]
             stop = [False]
@@ -271,15 +392,20 @@ def serve_bench(scb, idx, genome, qps_steps, seconds, seed=9):
                     with lock:
                         t_arr, pi, kind, owns = arrivals.pop(r.ticket)
                         lat.append((r.done_ns - t_arr) / 1e6)
+                        if pi == wi and kind == 0:
+                            worst_ms[0] = lat[-1]
                         kmers_done[0] += r.report.valid_kmers if kind == 0 else r.queried
                         if owns:
                             busy[pi] = False
 
             th = threading.Thread(target=collect)
             th.start()
+            seconds = max(1.0, min_requests / qps)
             t0 = time.monotonic_ns()
             t_next = t0
             end = t0 + int(seconds * 1e9)
+            worst_at = t0 + int(seconds * 0.5e9)
+            worst_sent = False
             while True:
                 t_next += int(rng.exponential(1e9 / qps))
                 if t_next >= end:
@@ -287,8 +413,11 @@ def serve_bench(scb, idx, genome, qps_steps, seconds, seed=9):
                 now = time.monotonic_ns()
                 if t_next > now:
                     time.sleep((t_next - now) / 1e9)
-                pi = int(rng.integers(0, len(pool)))
-                kind = 0 if rng.random() < 0.8 else 1
+                if not worst_sent and t_next >= worst_at:
+                    pi, kind, worst_sent = wi, 0, True
+                else:
+                    pi = int(rng.integers(0, len(pool) - 1))
+                    kind = 0 if rng.random() < 0.8 else 1
                 owns = False
                 try:
                     with lock:
@@ -319,18 +448,55 @@ def serve_bench(scb, idx, genome, qps_steps, seconds, seed=9):
                           "p50_ms": round(float(np.percentile(a, 50)), 3) if a.size else None,
                           "p99_ms": round(float(np.percentile(a, 99)), 3) if a.size else None,
                           "max_ms": round(float(a.max()), 3) if a.size else None,
+                          "worst_case_request_ms": round(worst_ms[0], 3) if worst_ms[0] is not None else None,
                           "served_G_kmers_per_s": round(kmers_done[0] / wall / 1e9, 4)})
             log(f"served qps={qps}: {steps[-1]}")
-        # spot check: served abundances equal the direct call
-        b, o = pool[1]
-        s = srv.wait(srv.submit_abundances(b, o))
-        vals, valid, rep = idx.estimate_sequences(b, o)
-        ok = bool(np.array_equal(s.valid, valid) and np.array_equal(s.values[:b.size].astype(np.uint64), vals)
-                  and s.report == rep)
-    mean_bp = float(np.mean([int(o[-1]) for _, o in pool]))
-    return {"steps": steps, "seconds_per_step": seconds, "request_pool": len(pool),
-            "mean_request_bases": round(mean_bp), "mix": "80% abundances, 20% spectrum",
-            "served_equals_direct": ok}
+            if steps[-1]["p99_ms"] is None or steps[-1]["p99_ms"] > p99_limit_ms:
+                break
+        # spot checks: served abundances equal the direct call, worst case included
+        ok = True
+        for pi in (1, wi):
+            b, o = pool[pi]
+            srv_r = srv.wait(srv.submit_abundances(b, o))
+            vals, valid, rep = idx.estimate_sequences(b, o)
+            ok &= bool(np.array_equal(srv_r.valid, valid)
+                       and np.array_equal(srv_r.values[:b.size].astype(np.uint64), vals) and srv_r.report == rep)
+    knee = None
+    for st in steps:
+        if st["p99_ms"] is not None and st["p99_ms"] <= p99_limit_ms:
+            knee = st["qps"]
+    sizes = [int(o[-1]) for _, o in pool[:-1]]
+    return {"steps": steps, "min_requests_per_step": min_requests, "request_pool": len(pool),
+            "mean_request_bases": round(float(np.mean(sizes))), "max_pool_request_bases": int(max(sizes)),
+            "worst_case_request": {"records": int(worst[1].size - 1), "bases": int(worst[1][-1])},
+            "mix": "80% abundances, 20% spectrum", "p99_limit_ms": p99_limit_ms,
+            "highest_qps_within_limit": knee, "served_equals_direct": ok}
+
+
+def valu_roofline(pmc, stage_ms, kmers):
+    """VALU lane-operations per k-mer (SQ_INSTS_VALU x 64 lanes, from the committed
+    rocprofv3 summary) against the MI355X VALU peak: 256 CUs x 4 SIMDs x 16 lanes
+    x 2.4 GHz = 39.3 T lane-ops/s, for the dominant kernel and the whole step."""
+    peak = 256 * 4 * 16 * 2.4e9
+    ks = pmc.get("kernels", {})
+    by = pmc.get("kernels_by_stage", {})
+    per_launch = {st: ks[kn]["valu_insts"] * 64 for st, kn in by.items() if kn in ks}
+    # steps in the profiled run: one insert-bin launch per step; every kernel's
+    # per-dispatch mean times its dispatches per step
+    steps = ks.get(by.get("insert_bin", ""), {}).get("dispatches", 0)
+    total = sum(v["valu_insts"] * 64 * v["dispatches"] / steps for v in ks.values()) if steps else None
+    dom = max(stage_ms, key=lambda k: stage_ms[k])
+    out = {"peak_T_lane_ops_s": round(peak / 1e12, 1), "source": os.path.relpath(PMC_PROFILE, ROOT)}
+    if dom in per_launch:
+        out["kernel"] = dom
+        out["kernel_lane_ops_per_launch"] = per_launch[dom]
+        out["kernel_achieved_T_lane_ops_s"] = round(per_launch[dom] / (stage_ms[dom] / 1e3) / 1e12, 2)
+        out["kernel_frac"] = round(per_launch[dom] / (stage_ms[dom] / 1e3) / peak, 3)
+    if total:
+        step_ms = sum(stage_ms.values())
+        out["pipeline_lane_ops_per_kmer"] = round(total / kmers, 1)
+        out["pipeline_frac"] = round(total / (step_ms / 1e3) / peak, 3)
+    return out
 
 
 def main():
@@ -345,8 +511,9 @@ def main():
     ap.add_argument("--no-host", action="store_true")
     ap.add_argument("--no-io", action="store_true", help="skip the FASTQ and .scb file measurements")
     ap.add_argument("--no-spectrum", action="store_true", help="skip the distinct-k-mer spectrum timing")
-    ap.add_argument("--serve-qps", default="100,400")
-    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--no-configs", action="store_true", help="skip the long-read and mixture workloads")
+    ap.add_argument("--serve-qps", default="100,200,400,800,1600,3200")
+    ap.add_argument("--serve-min-requests", type=int, default=2000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device copy (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -358,38 +525,35 @@ def main():
         copy_probe()
         return
 
-    import torch
     import scbf_amd as scb
+    from scbf_amd import synth
+    from scbf_amd.devmem import DeviceBuffer, synchronize
 
     t0 = time.time()
     ins_b, ins_o, q_b, q_o = gen_workload()
     log(f"workload: {ins_o.size - 1} insert reads, {q_o.size - 1} query reads "
         f"({time.time() - t0:.1f} s to generate)")
-    dev_t = {
-        "ib": torch.from_numpy(ins_b).cuda(), "io": torch.from_numpy(ins_o.view(np.int64)).cuda(),
-        "qb": torch.from_numpy(q_b).cuda(), "qo": torch.from_numpy(q_o.view(np.int64)).cuda(),
-    }
-    dev = {"ib": dev_t["ib"].data_ptr(), "io": dev_t["io"].data_ptr(), "in": ins_o.size - 1,
-           "iN": int(ins_o[-1]), "qb": dev_t["qb"].data_ptr(), "qo": dev_t["qo"].data_ptr(),
-           "qn": q_o.size - 1, "qN": int(q_o[-1])}
+    buf = {"ib": DeviceBuffer.from_array(ins_b), "io": DeviceBuffer.from_array(ins_o),
+           "qb": DeviceBuffer.from_array(q_b), "qo": DeviceBuffer.from_array(q_o)}
+    dev = {"ib": buf["ib"].ptr, "io": buf["io"].ptr, "in": ins_o.size - 1, "iN": int(ins_o[-1]),
+           "qb": buf["qb"].ptr, "qo": buf["qo"].ptr, "qn": q_o.size - 1, "qN": int(q_o[-1])}
     qN = dev["qN"]
-    out_vals = torch.zeros(qN, dtype=torch.int16, device="cuda")
-    out_bits = torch.zeros((qN + 31) // 32, dtype=torch.int32, device="cuda")
-    torch.cuda.synchronize()
+    out_vals = DeviceBuffer(qN * 2 + 4).zero()
+    out_bits = DeviceBuffer((qN + 31) // 32 * 4 + 4).zero()
+    synchronize()
     cfg = cfg_c2(scb)
 
     for _ in range(args.warmup):
-        run_step(scb, cfg, scb.METHOD_BATCHED, dev, out_vals.data_ptr(), out_bits.data_ptr())
-    torch.cuda.synchronize()
-    ins_ms, q_ms, ist_all, qst_all = [], [], [], []
+        run_step(scb, cfg, scb.METHOD_BATCHED, dev, out_vals.ptr, out_bits.ptr)
+    synchronize()
+    ist_all, qst_all = [], []
     t_start = time.perf_counter()
     for _ in range(args.steps):
-        ir, qr, ist, qst, idx = run_step(scb, cfg, scb.METHOD_BATCHED, dev, out_vals.data_ptr(),
-                                         out_bits.data_ptr())
+        ir, qr, ist, qst, idx = run_step(scb, cfg, scb.METHOD_BATCHED, dev, out_vals.ptr, out_bits.ptr)
         ist_all.append(ist)
         qst_all.append(qst)
         del idx
-    torch.cuda.synchronize()
+    synchronize()
     t_total = time.perf_counter() - t_start
     ms_per_step = t_total * 1e3 / args.steps
     kmers = ir.inserted_kmers + qr.valid_kmers
@@ -400,15 +564,14 @@ def main():
         f"query stages {qst_m}")
 
     # keep one batched result for the checks
-    ir_b, qr_b, _, _, idx_b = run_step(scb, cfg, scb.METHOD_BATCHED, dev, out_vals.data_ptr(),
-                                       out_bits.data_ptr())
+    ir_b, qr_b, _, _, idx_b = run_step(scb, cfg, scb.METHOD_BATCHED, dev, out_vals.ptr, out_bits.ptr)
     counters_b = idx_b.counters()
-    vals_b, bits_b = out_vals.clone(), out_bits.clone()
+    vals_b = out_vals.to_array(np.uint16, qN)
+    bits_b = out_bits.to_array(np.uint32, (qN + 31) // 32)
     served = None
     if not args.no_serve:
-        from scbf_amd import synth
         served = serve_bench(scb, idx_b, synth.genome(5_000_000, 1),
-                             [float(q) for q in args.serve_qps.split(",")], args.serve_seconds)
+                             [float(q) for q in args.serve_qps.split(",")], args.serve_min_requests)
     spectrum = None
     if not args.no_spectrum:
         spectrum = spectrum_bench(scb, idx_b, dev)
@@ -418,12 +581,12 @@ def main():
     plain = None
     bit_exact = None
     if not args.no_plain:
-        out_vals.zero_()
-        torch.cuda.synchronize()
+        out_vals.zero()
+        out_bits.zero()
+        synchronize()
         tp = time.perf_counter()
-        ir_p, qr_p, ist_p, qst_p, idx_p = run_step(scb, cfg, scb.METHOD_PLAIN, dev, out_vals.data_ptr(),
-                                                   out_bits.data_ptr())
-        torch.cuda.synchronize()
+        ir_p, qr_p, ist_p, qst_p, idx_p = run_step(scb, cfg, scb.METHOD_PLAIN, dev, out_vals.ptr, out_bits.ptr)
+        synchronize()
         plain_ms = (time.perf_counter() - tp) * 1e3
         plain = {"value": kmers / (plain_ms / 1e3) / 1e9, "ms_per_step": plain_ms,
                  "insert_ms": ist_p[3], "query_ms": qst_p[3]}
@@ -431,7 +594,8 @@ def main():
         if not args.no_check:
             counters_p = idx_p.counters()
             bit_exact = bool(np.array_equal(counters_b, counters_p)
-                             and torch.equal(vals_b, out_vals) and torch.equal(bits_b, out_bits)
+                             and np.array_equal(vals_b, out_vals.to_array(np.uint16, qN))
+                             and np.array_equal(bits_b, out_bits.to_array(np.uint32, (qN + 31) // 32))
                              and (ir_b.inserted_kmers, ir_b.inserted_smers, qr_b.valid_kmers,
                                   qr_b.positive_estimates, qr_b.estimate_checksum)
                              == (ir_p.inserted_kmers, ir_p.inserted_smers, qr_p.valid_kmers,
@@ -471,12 +635,17 @@ def main():
         roof["traffic"] = int(kp["hbm_bytes"])
         roof["traffic_source"] = os.path.relpath(PMC_PROFILE, ROOT)
         roof["valu_busy"] = kp.get("valu_busy")
+        if "write_kib" in kp:
+            roof["kernel_write_bytes"] = int(kp["write_kib"] * 1024)
+            roof["kernel_output_bytes"] = int(qr.valid_kmers * W) if dom == "query_bin" else None
+    if pmc:
+        roof["valu"] = valu_roofline(pmc, {k: v[0] for k, v in stages.items()}, kmers)
 
     # host-buffer path (PCIe-inclusive): the crate-style calls on numpy arrays in host memory,
     # abundances returned as u64 + validity bytes per position
     host_api = None
     if not args.no_host:
-        torch.cuda.synchronize()
+        synchronize()
         th0 = time.perf_counter()
         hb = scb.SuperCountingBloomBuilder(cfg)
         hrep = hb.add_sequences((ins_b, ins_o))
@@ -495,12 +664,40 @@ def main():
     if not args.no_io:
         io = io_bench(scb, cfg, ins_b, ins_o, q_b, q_o, ir.inserted_kmers, qr.valid_kmers)
         log(f"file boundaries: {io}")
+    del buf, out_vals, out_bits
+
+    configs = None
+    if not args.no_configs:
+        configs = {}
+        t1 = time.time()
+        g3 = synth.repeat_genome()
+        b3, o3 = synth.long_reads(g3, 20.0, 31)
+        log(f"long-read workload: {o3.size - 1} reads, {int(o3[-1])} bases ({time.time() - t1:.1f} s)")
+        configs["long_reads_u8"] = config_bench(
+            scb, "long reads", (31, 21, 25, 4, 8, 30, 9), b3, o3, 3,
+            workload="k31 m21 s25 h4, 8-bit counters, 2^30 slots, 2^9 slots/block; 20 Mbp genome + 300 bp "
+                     "element x10,000; log-normal reads (median 8 kbp, sigma 0.7, 500 bp-60 kbp) at 20x, 3% subst, "
+                     "1% ins, 1% del; insert and query the same reads in 256 MB device batches")
+        del b3, o3, g3
+        t1 = time.time()
+        b4, o4 = synth.mixture_reads(50, 2_000_000, 2_000_000_000, 41)
+        log(f"mixture workload: {o4.size - 1} reads, {int(o4[-1])} bases ({time.time() - t1:.1f} s)")
+        configs["skewed_mixture_u32"] = config_bench(
+            scb, "skewed mixture", (21, 15, 17, 3, 32, 32, 10), b4, o4, 2,
+            workload="k21 m15 s17 h3, 32-bit counters, 2^32 slots (16 GiB), 2^10 slots/block; 50 genomes of 2 Mbp, "
+                     "log-normal (sigma 2) abundances; 150 bp reads, 2e9 bases, 0.5% subst; insert and query the "
+                     "same reads in 256 MB device batches")
+        configs["skewed_mixture_u32_one_batch"] = config_bench(
+            scb, "skewed mixture, one batch", (21, 15, 17, 3, 32, 32, 10), b4, o4, 2, limit=1 << 31,
+            workload="the skewed mixture above (2e9 bases) inserted and queried as one device batch each")
+        del b4, o4
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(ins_b, ins_o, q_b, q_o)
         log(f"cpu baseline: {cpu}")
 
+    tun = scb.SuperCountingBloomBuilder(cfg).tuning()
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "G k-mers/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
@@ -510,13 +707,14 @@ def main():
                                "2^9 slots/block; 5 Mbp genome; insert 50x 150 bp reads (1% subst, "
                                "0.1% N), query those + fresh 10x", "parallelism": "single GPU",
                    "insert_reads": dev["in"], "query_reads": dev["qn"], "inserted_kmers": ir.inserted_kmers,
-                   "queried_kmers": qr.valid_kmers},
+                   "queried_kmers": qr.valid_kmers, "tuning": tun.__dict__},
         "stage_ms": {"insert": {"scan": ist_m[0], "plan_scatter": ist_m[1], "bin": ist_m[2], "total": ist_m[3]},
                      "query": {"scan": qst_m[0], "plan_scatter": qst_m[1], "bin": qst_m[2], "total": qst_m[3]}},
         "plain": plain,
         "batched_over_plain": round(value / plain["value"], 2) if plain else None,
         "bit_exact_batched_vs_plain": bit_exact,
         "roofline": roof,
+        "configs": configs,
         "served": served,
         "host_buffers": host_api,
         "files": io,

@@ -179,9 +179,12 @@ def long_reads(g, coverage, seed):
                           sub=0.03, ins=0.01, dele=0.01)
 
 
-def mixture_reads(n_genomes, genome_len, total_bases, seed, read_len=150, sigma=2.0, sub_rate=0.005):
+def mixture_reads(n_genomes, genome_len, total_bases, seed, read_len=150, sigma=2.0, sub_rate=0.005,
+                  chunk_reads=1_000_000):
     """Reads of a mixture of n_genomes random genomes whose relative abundances are
-    exp(N(0, sigma^2)); 150 bp reads, total_bases in all, 0.5% substitutions."""
+    exp(N(0, sigma^2)); 150 bp reads, total_bases in all, 0.5% substitutions.
+    Each genome's reads are drawn in chunks of at most chunk_reads (each chunk
+    with its own seed), which bounds the generator's memory."""
     rng = np.random.default_rng([seed, 7])
     w = np.exp(rng.normal(0.0, sigma, size=n_genomes))
     w /= w.sum()
@@ -192,5 +195,7 @@ def mixture_reads(n_genomes, genome_len, total_bases, seed, read_len=150, sigma=
         if n == 0:
             continue
         g = genome(genome_len, seed * 1000 + i)
-        parts.append(fixed_reads(g, int(n), read_len, seed * 1000 + 500 + i, sub_rate=sub_rate))
+        for c, a in enumerate(range(0, int(n), chunk_reads)):
+            m = min(chunk_reads, int(n) - a)
+            parts.append(fixed_reads(g, m, read_len, (seed * 1000 + 500 + i) * 4096 + c, sub_rate=sub_rate))
     return concat_sets(parts)
