@@ -330,7 +330,7 @@ typedef int (*scb_record_visitor)(void *user, const char *name, const uint8_t *s
 int scb_for_each_fastx_record(const char *path, scb_record_visitor visit, void *user);
 
 /* ---- reservoir-sampled spectrum (bin/sampled_spectrum.rs:150-308) ---------- */
-/* A row of write_scaled_spectrum (sampled_spectrum.rs:223-253). */
+/* A row of write_scaled_spectrum (sampled_spectrum.rs:225-253). */
 typedef struct scb_sampled_row {
     uint64_t estimated_count, sampled_kmers, linear_scaled_windows, estimated_distinct_kmers;
 } scb_sampled_row;
@@ -348,7 +348,7 @@ typedef struct scb_sampled_stats {
 int scb_index_sampled_spectrum(const scb_index *idx, const uint8_t *bases, const uint64_t *offsets,
                                size_t n, uint64_t sample_size, uint64_t seed, scb_sampled_row *rows,
                                size_t cap, size_t *n_rows, scb_sampled_stats *stats);
-/* run() of bin/sampled_spectrum.rs:109-148: build the index from the FASTA/FASTQ
+/* run() of bin/sampled_spectrum.rs:99-148: build the index from the FASTA/FASTQ
  * file, then sample and query its valid windows. */
 int scb_sampled_spectrum_from_fastx(const char *path, const scb_config *cfg, uint64_t sample_size,
                                     uint64_t seed, scb_sampled_row *rows, size_t cap, size_t *n_rows,

@@ -951,7 +951,7 @@ size_t oracle_spectrum_rows(const oracle_spectrum *sp, uint64_t *values, uint64_
 // windows of the records in input order (SplitMix64 draws, gen_below = next %
 // upper), then each sampled k-mer queried on its own through
 // estimate_sequence_abundances (api.rs:250-262), the first window's estimate
-// counted; rows scaled as write_scaled_spectrum (sampled_spectrum.rs:223-253).
+// counted; rows scaled as write_scaled_spectrum (sampled_spectrum.rs:225-253).
 // rows holds 4 u64 per row: estimate, sampled k-mers, linear scaled windows,
 // estimated distinct k-mers. stats: total windows seen, sampled k-mers,
 // queried k-mers, invalid queries.

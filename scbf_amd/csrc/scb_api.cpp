@@ -1145,7 +1145,7 @@ int scb_stream_estimates_from_fastx_pair(const char *index_path, const char *que
 // ---- reservoir-sampled spectrum (bin/sampled_spectrum.rs) ----------------------
 namespace {
 // query_sampled_kmers' histogram (sampled_spectrum.rs:201-221) of the sampled
-// estimates and write_scaled_spectrum's scaling (223-253), in f64 as there.
+// estimates and write_scaled_spectrum's scaling (225-253), in f64 as there.
 std::vector<scb_sampled_row> sampled_rows(const std::vector<uint32_t> &vals, uint64_t seen) {
     std::vector<std::pair<uint64_t, uint64_t>> h;
     {
@@ -1210,7 +1210,7 @@ int scb_index_sampled_spectrum(const scb_index *idx, const uint8_t *bases, const
     return rc;
 }
 
-// bin/sampled_spectrum.rs:109-148 (run): build the index from the file, then
+// bin/sampled_spectrum.rs:99-148 (run): build the index from the file, then
 // sample its valid windows and query them, in one pass of the file's records.
 int scb_sampled_spectrum_from_fastx(const char *path, const scb_config *cfg, uint64_t sample_size, uint64_t seed,
                                     scb_sampled_row *rows, size_t cap, size_t *n_rows, scb_sampled_stats *stats) {
@@ -1241,7 +1241,7 @@ int scb_sampled_spectrum_from_fastx(const char *path, const scb_config *cfg, uin
     return SCB_OK;
 }
 
-// write_scaled_spectrum (sampled_spectrum.rs:223-253): CSV with its header
+// write_scaled_spectrum (sampled_spectrum.rs:225-253): CSV with its header
 int scb_write_sampled_spectrum(const scb_sampled_row *rows, size_t n, const char *path) {
     FILE *fp = path ? std::fopen(path, "wb") : stdout;
     if (!fp) return fail(SCB_ERR_IO, std::string("I/O error for '") + path + "': " + std::strerror(errno));

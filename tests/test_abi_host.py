@@ -222,7 +222,7 @@ def test_primitives_match_oracle():
 
 def test_sampled_spectrum_host_checks_and_csv(tmp_path):
     """bin/sampled_spectrum.rs:100-105: a zero sample size is an InvalidConfig
-    before any work; write_scaled_spectrum's CSV (223-253)."""
+    before any work; write_scaled_spectrum's CSV (225-253)."""
     L = scb.api.lib()
     n = C.c_size_t()
     rc = L.scb_index_sampled_spectrum(None, None, None, 0, 0, 1, None, 0, C.byref(n), None)
