@@ -444,7 +444,9 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
     // s_SL only at index hj before writing it
     uint16_t *s_mpos = s_SL;
     const int NH = kTile + 2 * w - 1;
-    uint64_t acc_valid = 0, acc_smers = 0, acc_pos = 0, acc_sum = 0;
+    uint64_t acc_smers = 0, acc_pos = 0, acc_sum = 0;
+    uint32_t wave_valid = 0;    // valid windows of the wave (uniform)
+    uint32_t tile_records = 0;  // records the tile emits
 
     // 16 bases per thread are prefetched one tile ahead.
     auto fetch = [&](int64_t tile_lb, uint32_t (&wv)[4]) {
@@ -574,32 +576,49 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
             // the keys overwrite part of the prefixes: held in registers until
             // every thread has read its prefixes
             constexpr int NHW = kTile + 2 * WREG - 1, KR = (NHW + kTileThreads - 1) / kTileThreads;
-            uint16_t key[KR];
+            // thread tid takes hp = tid + q kTileThreads: kTileThreads is a multiple
+            // of 32 and of 16, so every q shares the rotation amounts and the
+            // padded prefix index moves by a constant (folded into the LDS offsets)
+            static_assert(kTileThreads % 32 == 0, "keys of one thread share their rotations");
+            constexpr int kStride = kTileThreads + kTileThreads / 16;
+            const int li0 = tid + kLeft - 1, lm0 = li0 + m;
+            const uint32_t *gfi = s_Gf + li0 + (li0 >> 4), *gfm = s_Gf + lm0 + (lm0 >> 4);
+            const uint32_t *gri = s_Gr + li0 + (li0 >> 4), *grm = s_Gr + lm0 + (lm0 >> 4);
+            const uint32_t rot_f = uint32_t(1 - lm0), rot_r = uint32_t(li0);  // rotl by lm - 1, rotr by li
+            auto key_at = [&](int q) {
+                const int o = q * kStride;
+                const uint32_t gf = gfm[o] ^ gfi[o], gr = grm[o] ^ gri[o];
+                return __builtin_amdgcn_alignbit(gf, gf, rot_f) + __builtin_amdgcn_alignbit(gr, gr, rot_r);
+            };
+            uint32_t key[KR];  // the key is the high half
 #pragma unroll
             for (int q = 0; q < KR; ++q) {
-                const int hp = tid + q * kTileThreads;
-                key[q] = hp < NHW ? mmer_key(hp) : uint16_t(0);
+                if ((q + 1) * kTileThreads <= NHW) key[q] = key_at(q);                       // every thread
+                else key[q] = tid + q * kTileThreads < NHW ? key_at(q) : 0u;                  // the last keys
             }
             __syncthreads();
 #pragma unroll
             for (int q = 0; q < KR; ++q)
-                if (tid + q * kTileThreads < NHW) s_K[tid + q * kTileThreads] = key[q];
+                if (tid + q * kTileThreads < NHW) s_K[tid + q * kTileThreads] = uint16_t(key[q] >> 16);
         } else {
             for (int hp = tid; hp < NH; hp += kTileThreads) s_K[hp] = mmer_key(hp);
         }
         __syncthreads();
 
         if constexpr (WREG > 0) {
-            // (3+4) with w = WREG known at compile time: each thread takes 8
-            // consecutive windows; their 8 + w - 1 keys come from LDS in 16-byte
-            // loads, and the minimum of window i of the group is
+            // (3+4) with w = WREG known at compile time: thread tid takes the 8
+            // windows hj in [8 tid, 8 tid + 8); their 8 + w - 1 keys come from LDS
+            // in 16-byte loads, and the minimum of window i of the group is
             // min(suffix minimum of keys [i, 8), prefix minimum of keys [8, i + w)):
             // w >= 9 makes every window reach key 8. Leftmost and rightmost
-            // minima pack (key, pos) and (key, ~pos) as for the blocks below.
+            // minima pack (key, i) and (key, 31 - i) with the key's index i in
+            // the group (a constant), so a min keeps the smaller key and, among
+            // equal keys, the leftmost or the rightmost index.
             static_assert(WREG >= 9 && WREG <= 25, "register windows need 9 <= w <= 25");
+            static_assert(kTile == 8 * kTileThreads, "one group of 8 windows per thread");
             constexpr int NK = 8 + WREG - 1, NV = (NK + 7) / 8;
-            for (int grp = tid; grp * 8 < kTile + WREG; grp += kTileThreads) {
-                const int h0 = grp * 8;
+            {
+                const int h0 = tid * 8;
                 uint32_t kw[NV * 4];  // two 16-bit keys per word
                 const uint4 *src = reinterpret_cast<const uint4 *>(s_K + h0);
 #pragma unroll
@@ -607,9 +626,10 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
                     const uint4 x = src[v];
                     kw[4 * v] = x.x, kw[4 * v + 1] = x.y, kw[4 * v + 2] = x.z, kw[4 * v + 3] = x.w;
                 }
+                // one byte permute: the key's two bytes above the index's two
                 auto packed = [&](int i, bool right) {
-                    const uint32_t key = ((kw[i >> 1] >> (16 * (i & 1))) & 0xFFFFu) << 16;
-                    return key | uint32_t(right ? 0xFFFF - (h0 + i) : h0 + i);
+                    return __builtin_amdgcn_perm(kw[i >> 1], uint32_t(right ? 31 - i : i),
+                                                 (i & 1) ? 0x07060100u : 0x05040100u);
                 };
                 // the group's windows read bits [base, base + 7 + k) of the masks:
                 // three words per mask, then one funnel shift per window
@@ -638,18 +658,33 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
                 }
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
-                    const int hj = h0 + i;
-                    if (hj < kTile + WREG) {
-                        // bits [li, li + k) with li = base + i (and the breaks after li)
-                        const uint32_t bad = (__builtin_amdgcn_alignbit(bad_hi, bad_lo, i) & mk) |
-                                             (__builtin_amdgcn_alignbit(brk_hi, brk_lo, i + 1) & mk1);
-                        const bool left =  // leftmost on ties
-                            2 * int(__popc(__builtin_amdgcn_alignbit(tg_hi, tg_lo, i) & mk)) > k;
-                        const uint32_t pl = min(sl[i], ql[i + WREG - 9]), pr = min(sr[i], qr[i + WREG - 9]);
-                        const uint32_t pos = left ? (pl & 0xFFFFu) : 0xFFFFu - (pr & 0xFFFFu);
-                        s_mpos[hj] = bad ? kNoWin : uint16_t(pos);
-                    }
+                    // bits [li, li + k) with li = base + i (and the breaks after li)
+                    const uint32_t bad = (__builtin_amdgcn_alignbit(bad_hi, bad_lo, i) & mk) |
+                                         (__builtin_amdgcn_alignbit(brk_hi, brk_lo, i + 1) & mk1);
+                    const bool left =  // leftmost on ties: more than k / 2 of T or G
+                        __popc(__builtin_amdgcn_alignbit(tg_hi, tg_lo, i) & mk) > uint32_t(k >> 1);
+                    const uint32_t pl = min(sl[i], ql[i + WREG - 9]), pr = min(sr[i], qr[i + WREG - 9]);
+                    // 31 - x is ~x in the low five bits
+                    const uint32_t pos = uint32_t(h0) + ((left ? pl : ~pr) & 31u);
+                    s_mpos[h0 + i] = bad ? kNoWin : uint16_t(pos);
                 }
+            }
+            // the w windows after the tile, hj in [kTile, kTile + w): one lane
+            // each of the last wave, over the window's w keys
+            if (tid >= kTileThreads - 64 && tid < kTileThreads - 64 + WREG) {
+                const int hj = kTile + (tid - (kTileThreads - 64));
+                uint32_t cl = ~0u, cr = ~0u;
+#pragma unroll
+                for (int q = 0; q < WREG; ++q) {
+                    const uint32_t key = uint32_t(s_K[hj + q]) << 16;
+                    cl = min(cl, key | uint32_t(q));
+                    cr = min(cr, key | uint32_t(31 - q));
+                }
+                const int li = hj + kLeft - 1;
+                const uint32_t bad = get_bits(s_bad, li, k) | get_bits(s_brk, li + 1, k - 1);
+                const bool left = 2 * int(__popc(get_bits(s_tg, li, k))) > k;  // leftmost minimum on ties
+                const uint32_t pos = uint32_t(hj) + (left ? (cl & 0xFFFFu) : 31u - (cr & 0xFFFFu));
+                s_mpos[hj] = bad ? kNoWin : uint16_t(pos);
             }
             __syncthreads();
         } else {
@@ -694,9 +729,11 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
             __syncthreads();
         }
 
-        // owned windows hj in [1, kTile]: validity bitmap, super-k-mer starts
-        constexpr int kPer = kTile / kTileThreads;
-        uint32_t starts = 0;  // bit u set if window 1 + u*256 + tid starts a super-k-mer
+        // owned windows hj in [1, kTile]: thread tid takes hj = 1 + 8 tid + u,
+        // u in [0, 8), whose validity, super-k-mer starts and boundaries are one
+        // byte each (bit u), written as bytes of the bitmaps
+        static_assert(kTile == 8 * kTileThreads, "eight windows per thread");
+        uint32_t starts = 0;  // bit u set if window 1 + 8 tid + u starts a super-k-mer
         if (MODE == kModeRecords && wid == 0) {
             // boundaries of the windows after the tile (hj in [kTile + 1, kTile + w))
             const int hj = kTile + 1 + lane;
@@ -708,30 +745,29 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
             uint64_t bm = __ballot(b);
             if (lane == 0) s_bnd[kTile / 32] = uint32_t(bm), s_bnd[kTile / 32 + 1] = uint32_t(bm >> 32);
         }
+        {
+            // minimizer positions of windows 8 tid .. 8 tid + 8 (16-byte aligned)
+            const uint4 q = *reinterpret_cast<const uint4 *>(s_mpos + 8 * tid);
+            const uint32_t mpw[5] = {q.x, q.y, q.z, q.w, uint32_t(s_mpos[8 * tid + 8])};
+            uint32_t vbyte = 0, bbyte = 0;
 #pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            int hj = 1 + u * kTileThreads + tid;
-            uint16_t mp = s_mpos[hj], pm = s_mpos[hj - 1];
-            bool valid = mp != kNoWin;
-            bool st = valid && (pm == kNoWin || pm != mp);
-            starts |= uint32_t(st) << u;
-            acc_valid += valid;
-            if (MODE == kModeRecords) {
-                uint64_t bm = __ballot(st || !valid);
-                if (lane == 0) {
-                    const int b0 = (u * kTileThreads + wid * 64) >> 5;
-                    s_bnd[b0] = uint32_t(bm), s_bnd[b0 + 1] = uint32_t(bm >> 32);
-                }
+            for (int u = 0; u < 8; ++u) {
+                const uint32_t pm = (mpw[u >> 1] >> (16 * (u & 1))) & 0xFFFFu;
+                const uint32_t mp = (mpw[(u + 1) >> 1] >> (16 * ((u + 1) & 1))) & 0xFFFFu;
+                const bool valid = mp != kNoWin;
+                // a boundary: invalid, or a new minimizer position (a start)
+                const bool bnd = !valid || pm != mp;
+                wave_valid += uint32_t(__popcll(__ballot(valid)));
+                vbyte |= uint32_t(valid) << u;
+                bbyte |= uint32_t(bnd) << u;
             }
+            starts = vbyte & bbyte;
+            if (MODE == kModeRecords) reinterpret_cast<uint8_t *>(s_bnd)[tid] = uint8_t(bbyte);
             if (a.valid_bits) {
-                uint64_t vm = __ballot(valid);
-                if (lane == 0) {
-                    int64_t j0 = t0 + u * kTileThreads + wid * 64;
-                    if (j0 < N) {
-                        a.valid_bits[j0 >> 5] = uint32_t(vm);
-                        if (j0 + 32 < N) a.valid_bits[(j0 >> 5) + 1] = uint32_t(vm >> 32);
-                    }
-                }
+                // windows j0 .. j0 + 7 (t0 is a multiple of 32): every byte of each
+                // bitmap word that starts before the batch end is written
+                const int64_t j0 = t0 + 8 * tid;
+                if ((j0 & ~int64_t(31)) < N) reinterpret_cast<uint8_t *>(a.valid_bits)[j0 >> 3] = uint8_t(vbyte);
             }
         }
 
@@ -750,6 +786,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
                 if (i < wid) before += s_wc[i];
                 total += s_wc[i];
             }
+            tile_records = total;
             if (tid == 0) {
                 uint32_t seg = tile & (kSeg - 1);
                 s_base = seg * a.seg_cap + atomicAdd(&a.seg_count[seg], total);
@@ -760,7 +797,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
             {
                 uint32_t at = before + incl - my;
                 for (uint32_t bits = starts; bits; bits &= bits - 1)
-                    s_list[at++] = uint16_t(1 + (__ffs(bits) - 1) * kTileThreads + tid);
+                    s_list[at++] = uint16_t(1 + 8 * tid + (__ffs(bits) - 1));
             }
             __syncthreads();
             const uint64_t base = s_base;
@@ -776,13 +813,12 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
                 uint64_t pblk = ((block & g.shard_mask) << g.sub_log) | (block >> g.shard_shift);
                 uint64_t j = uint64_t(t0 - 1 + hj);
                 a.rec[base + i] = j | (uint64_t(nk - 1) << 32) | (pblk << 37);
-                acc_smers += nk + g.k - g.s;
             }
         } else {
             // Plain path: this thread applies its own super-k-mers straight to HBM.
             for (uint32_t bits = starts; bits; bits &= bits - 1) {
                 int u = __ffs(bits) - 1;
-                int hj = 1 + u * kTileThreads + tid;
+                int hj = 1 + 8 * tid + u;
                 uint16_t mp = s_mpos[hj];
                 int e = hj + 1;
                 while (e < hj + w && s_mpos[e] == mp) ++e;
@@ -833,21 +869,27 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
 
     // Block reductions of the statistics, added into one of kSeg spread slots
     // (per-tile counts fit 32 bits; only the plain query's estimate sum needs 64).
+    // The valid windows come per wave from the ballots. With records, the
+    // s-mers need no sum: a super-k-mer's nk windows are the valid windows from
+    // its start to the next boundary, so over the batch the records' k-mers are
+    // its valid windows, and each record adds k - s s-mers to its k-mers.
     {
-        uint32_t v0 = uint32_t(acc_valid), v1 = uint32_t(MODE == kModePlainQuery ? acc_pos : acc_smers);
+        uint32_t v1 = uint32_t(MODE == kModePlainQuery ? acc_pos : acc_smers);
         uint64_t v2 = acc_sum;
-        for (int off = 32; off > 0; off >>= 1) {
-            v0 += __shfl_down(v0, off, 64);
-            v1 += __shfl_down(v1, off, 64);
-            if (MODE == kModePlainQuery) v2 += __shfl_down(v2, off, 64);
+        if (MODE != kModeRecords) {
+            for (int off = 32; off > 0; off >>= 1) {
+                v1 += __shfl_down(v1, off, 64);
+                if (MODE == kModePlainQuery) v2 += __shfl_down(v2, off, 64);
+            }
         }
-        if (lane == 0) s_red[0][wid] = v0, s_red[1][wid] = v1, s_red[2][wid] = MODE == kModePlainQuery ? v2 : 0;
+        if (lane == 0) s_red[0][wid] = wave_valid, s_red[1][wid] = v1, s_red[2][wid] = MODE == kModePlainQuery ? v2 : 0;
     }
     __syncthreads();
     if (tid == 0) {
         uint64_t t[3] = {0, 0, 0};
         for (int q = 0; q < 3; ++q)
             for (int i = 0; i < nwave; ++i) t[q] += s_red[q][i];
+        if (MODE == kModeRecords) t[1] = t[0] + uint64_t(g.k - g.s) * tile_records;
         unsigned long long *st = (unsigned long long *)a.stats + kStSpread + (blockIdx.x & (kSeg - 1));
         if (t[0]) atomicAdd(&st[kSeg * (MODE == kModePlainQuery ? kSpQueried : kSpValid)], t[0]);
         if (MODE == kModePlainQuery) {
