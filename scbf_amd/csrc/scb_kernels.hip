@@ -14,7 +14,7 @@
 //             counters fit in LDS)
 //
 // Batched pipeline (one stream, no host sync inside):
-//   k_scan      per 2048-window tile: ASCII -> codes, ACGT-run and read-boundary
+//   k_scan      per 1952-window tile: ASCII -> codes, ACGT-run and read-boundary
 //               masks, canonical ntHash of every m-mer, minimizer of every k-mer
 //               window, super-k-mer cut, block hash -> rec (appended to one of
 //               64 segments)
@@ -48,10 +48,15 @@ namespace {
 // ----------------------------------------------------------------------------
 // Constants and small device helpers
 // ----------------------------------------------------------------------------
-constexpr int kTile = 2048;            // windows owned by one scan tile
+// A tile's 256 threads load 16 bases each for 2048 bases of span, which hold
+// its windows, the 32 bases before them and 64 after (a window of k <= 32 and
+// the hashes of the w - 1 windows after the tile reach 62 bases past its end).
+constexpr int kTile = 1952;            // windows owned by one scan tile
 constexpr int kTileThreads = 256;
-constexpr int kLeft = 32;              // LDS holds bases [t0 - 32, t0 + kTile + 96)
-constexpr int kSpan = kTile + 128;
+constexpr int kLeft = 32;              // LDS holds bases [t0 - 32, t0 + kTile + 64)
+constexpr int kSpan = kTile + 96;
+static_assert(kSpan == 16 * 128, "two waves load the span, 16 bases per thread");
+static_assert(kTile % 32 == 0 && kTile <= 8 * kTileThreads, "at most eight windows per thread, whole bitmap words");
 constexpr int kSpanWords = kSpan / 32 + 2;
 constexpr int kPkWords = kSpan / 16 + 8;
 constexpr int kHashN = kTile + 64;
@@ -615,9 +620,8 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
             // the group (a constant), so a min keeps the smaller key and, among
             // equal keys, the leftmost or the rightmost index.
             static_assert(WREG >= 9 && WREG <= 25, "register windows need 9 <= w <= 25");
-            static_assert(kTile == 8 * kTileThreads, "one group of 8 windows per thread");
             constexpr int NK = 8 + WREG - 1, NV = (NK + 7) / 8;
-            {
+            if (tid < kTile / 8) {
                 const int h0 = tid * 8;
                 uint32_t kw[NV * 4];  // two 16-bit keys per word
                 const uint4 *src = reinterpret_cast<const uint4 *>(s_K + h0);
@@ -732,7 +736,6 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
         // owned windows hj in [1, kTile]: thread tid takes hj = 1 + 8 tid + u,
         // u in [0, 8), whose validity, super-k-mer starts and boundaries are one
         // byte each (bit u), written as bytes of the bitmaps
-        static_assert(kTile == 8 * kTileThreads, "eight windows per thread");
         uint32_t starts = 0;  // bit u set if window 1 + 8 tid + u starts a super-k-mer
         if (MODE == kModeRecords && wid == 0) {
             // boundaries of the windows after the tile (hj in [kTile + 1, kTile + w))
@@ -745,7 +748,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
             uint64_t bm = __ballot(b);
             if (lane == 0) s_bnd[kTile / 32] = uint32_t(bm), s_bnd[kTile / 32 + 1] = uint32_t(bm >> 32);
         }
-        {
+        if (tid < kTile / 8) {
             // minimizer positions of windows 8 tid .. 8 tid + 8 (16-byte aligned)
             const uint4 q = *reinterpret_cast<const uint4 *>(s_mpos + 8 * tid);
             const uint32_t mpw[5] = {q.x, q.y, q.z, q.w, uint32_t(s_mpos[8 * tid + 8])};
