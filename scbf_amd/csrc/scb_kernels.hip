@@ -221,6 +221,23 @@ __device__ __forceinline__ uint32_t rotr32(uint32_t x, uint32_t r) {
     return __builtin_amdgcn_alignbit(x, x, r & 31);
 }
 
+// Inclusive scan over the 64 lanes of a wave (XOR or +) with DPP moves: rows
+// of 16 lanes by shifts of 1, 2, 4 and 8, then the last lane of rows 0 and 2
+// into rows 1 and 3, then lane 31 into rows 2 and 3. A lane without a source
+// takes 0, the identity of both operations. Every lane of the wave must be
+// active.
+template <bool XOR>
+__device__ __forceinline__ uint32_t wave_scan_incl(uint32_t x) {
+    auto step = [&](uint32_t y) { return XOR ? (x ^ y) : (x + y); };
+    x = step(__builtin_amdgcn_update_dpp(0u, x, 0x111, 0xF, 0xF, false));  // row_shr:1
+    x = step(__builtin_amdgcn_update_dpp(0u, x, 0x112, 0xF, 0xF, false));  // row_shr:2
+    x = step(__builtin_amdgcn_update_dpp(0u, x, 0x114, 0xF, 0xF, false));  // row_shr:4
+    x = step(__builtin_amdgcn_update_dpp(0u, x, 0x118, 0xF, 0xF, false));  // row_shr:8
+    x = step(__builtin_amdgcn_update_dpp(0u, x, 0x142, 0xA, 0xF, false));  // row_bcast:15
+    x = step(__builtin_amdgcn_update_dpp(0u, x, 0x143, 0xC, 0xF, false));  // row_bcast:31
+    return x;
+}
+
 // ntHash seeds (low 32 bits of the original ntHash seeds), by 2-bit code
 // A=0 C=1 T=2 G=3. Third-party arithmetic (seq-hash 0.1.2), restated.
 __device__ __forceinline__ uint32_t nt_seed(uint32_t c) {
@@ -538,12 +555,15 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
                 xr ^= s_seed[4 * kSeedRow + ix];
             }
         }
-        uint32_t sf = xf, sr = xr;
-        for (int off = 1; off < 64; off <<= 1) {
-            uint32_t yf = __shfl_up(sf, off, 64), yr = __shfl_up(sr, off, 64);
-            if (lane >= off) sf ^= yf, sr ^= yr;
+        // the chunks' XOR totals, scanned over the wave (the waves that load
+        // bases are whole: kChunks is a multiple of 64)
+        static_assert(kChunks % 64 == 0, "whole waves load the span");
+        uint32_t sf = 0, sr = 0;
+        if (tid < kChunks) {
+            sf = wave_scan_incl<true>(xf);
+            sr = wave_scan_incl<true>(xr);
+            if (lane == 63) s_wx[0][wid] = sf, s_wx[1][wid] = sr;
         }
-        if (lane == 63) s_wx[0][wid] = sf, s_wx[1][wid] = sr;
         __syncthreads();
         {   // record starts inside the span mark read boundaries
             uint64_t r = rfirst;
@@ -777,11 +797,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
         const uint32_t my = __popc(starts);
         if (MODE == kModeRecords) {
             // exclusive scan of the per-thread start counts (wave scan + wave totals)
-            uint32_t incl = my;
-            for (int off = 1; off < 64; off <<= 1) {
-                uint32_t y = __shfl_up(incl, off, 64);
-                if (lane >= off) incl += y;
-            }
+            const uint32_t incl = wave_scan_incl<false>(my);
             if (lane == 63) s_wc[wid] = incl;
             __syncthreads();
             uint32_t before = 0, total = 0;
