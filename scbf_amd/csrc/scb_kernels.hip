@@ -1653,8 +1653,22 @@ __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
                         for (int u = 0; u < kU; ++u) old[u] = make_uint4(0, 0, 0, 0);
                     } else {
+                        // all in flight; a vector none of whose slots received a
+                        // count is neither read nor written (a sparse batch into a
+                        // large table touches a fraction of it)
 #pragma unroll
-                        for (int u = 0; u < kU; ++u) old[u] = tv[min(v0 + u * nthr, nvec - 1)];  // all in flight
+                        for (int u = 0; u < kU; ++u) {
+                            const uint32_t vi = min(v0 + u * nthr, nvec - 1);
+                            const uint32_t s0 = vi * 4 * per, sh = 16 * (s0 >> half_log);
+                            const uint4 *cw = reinterpret_cast<const uint4 *>(s_cnt + (s0 & (half_words - 1)));
+                            uint32_t orw = 0;
+#pragma unroll
+                            for (uint32_t q = 0; q < per; ++q) {
+                                const uint4 x = cw[q];
+                                orw |= x.x | x.y | x.z | x.w;
+                            }
+                            old[u] = ((orw >> sh) & 0xFFFFu) ? tv[vi] : make_uint4(0, 0, 0, 0);
+                        }
                     }
 #pragma unroll
                     for (int u = 0; u < kU; ++u) {
