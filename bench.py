@@ -341,7 +341,7 @@ def copy_probe(mib=1024, reps=3):
     print(json.dumps({"copy_probe_bytes": n, "reps": reps}), flush=True)
 
 
-def serve_bench(scb, idx, genome, qps_steps, min_requests=2000, pool_size=160, p99_limit_ms=10.0, seed=9):
+def serve_bench(scb, idx, genome, qps_steps, min_requests=2000, pool_size=160, p99_limit_ms=10.0, seed=9, cfg=None):
     """Open-loop serving of the default-geometry index: Poisson arrivals at each
     request rate of the sweep, at least min_requests per rate; request sizes
     log-uniform (1-10,000 records of 31-20,000 bp; 70% genome substrings on a
@@ -369,7 +369,7 @@ def serve_bench(scb, idx, genome, qps_steps, min_requests=2000, pool_size=160, p
     busy = [False] * len(pool)
     rng = np.random.default_rng([seed, 1])
     steps = []
-    with scb.SuperCountingBloomServer(idx) as srv:
+    with scb.SuperCountingBloomServer(idx, cfg) as srv:
         # warm-up: one request of each kind
         srv.wait(srv.submit_abundances(*pinned[0], *outs[0]))
         srv.wait(srv.submit_spectrum(*pinned[0]))

@@ -53,7 +53,7 @@ enum {
  * counter block and applied block-by-block in LDS. PLAIN: one fused launch per
  * batch, one thread per super-k-mer, global atomics / loads. Both are
  * bit-identical. */
-enum { SCB_METHOD_BATCHED = 0, SCB_METHOD_PLAIN = 1 };
+enum { SCB_METHOD_BATCHED = 0, SCB_METHOD_PLAIN = 1, SCB_METHOD_AUTO = 2 };
 
 /* SuperCountingBloomConfig (config.rs:20-31); counter_bits is 8, 16 or 32. */
 typedef struct scb_config {
@@ -174,7 +174,13 @@ int scb_builder_set_threads(scb_builder *b, size_t threads);
 void scb_builder_clear_threads(scb_builder *b);
 size_t scb_builder_threads(const scb_builder *b);
 void scb_builder_config(const scb_builder *b, scb_config *out);
-/* extension: choose SCB_METHOD_BATCHED (default) or SCB_METHOD_PLAIN */
+/* extension: the device method. SCB_METHOD_AUTO (default) runs the plain
+ * method (one thread per super-k-mer, global counter accesses) for batches
+ * below 2^17 bases + one base per 512 table bytes, where its lower fixed cost
+ * wins, and the batched method (sort by counter bin, bins staged in LDS)
+ * above; SCB_METHOD_BATCHED and SCB_METHOD_PLAIN force one. Results are the
+ * same with every method. The index takes the builder's method; see
+ * scb_index_set_method. */
 int scb_builder_set_method(scb_builder *b, int method);
 /* extension: copy the counter table (physical = .scb payload order) to host */
 int scb_builder_counters(const scb_builder *b, void *out, size_t bytes);

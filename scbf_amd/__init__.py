@@ -1,7 +1,7 @@
 """MI355X-native SuperCountingBloom engine: hand-written gfx950 HIP kernels
 behind a C ABI (include/scb.h, scbf_amd/libscb.so), with this Python mirror of
 the super-counting-bloom crate API on top."""
-from .api import (METHOD_BATCHED, METHOD_PLAIN, REQ_ABUNDANCES, REQ_SPECTRUM, AddReport,
+from .api import (METHOD_AUTO, METHOD_BATCHED, METHOD_PLAIN, REQ_ABUNDANCES, REQ_SPECTRUM, AddReport,
                   ApproxSpectrum, BuildIndexReport, CounterBits, FilterOnlySpectrumEstimate,
                   QueryFastaReport, SampledSpectrum, SequenceAbundanceSummary, ServeConfig, Served, SpectrumAccumulator,
                   StreamEstimateReport, StreamEstimateStats, SuperCountingBloomBuilder,

@@ -23,6 +23,7 @@ ERROR_KINDS = {1: "InvalidConfig", 2: "FastxOpen", 3: "InvalidIndexFormat", 4: "
 
 METHOD_BATCHED = 0
 METHOD_PLAIN = 1
+METHOD_AUTO = 2  # the default: plain below a size that depends on the table, batched above
 
 
 class SuperCountingBloomError(Exception):

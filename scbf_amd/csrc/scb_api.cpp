@@ -556,8 +556,8 @@ size_t scb_builder_threads(const scb_builder *b) { return size_t(b->cfg.threads)
 void scb_builder_config(const scb_builder *b, scb_config *out) { *out = b->cfg; }
 
 int scb_builder_set_method(scb_builder *b, int method) {
-    if (method != SCB_METHOD_BATCHED && method != SCB_METHOD_PLAIN)
-        return invalid_config("method must be SCB_METHOD_BATCHED or SCB_METHOD_PLAIN");
+    if (method != SCB_METHOD_BATCHED && method != SCB_METHOD_PLAIN && method != SCB_METHOD_AUTO)
+        return invalid_config("method must be SCB_METHOD_BATCHED, SCB_METHOD_PLAIN or SCB_METHOD_AUTO");
     b->method = scb::Method(method);
     return SCB_OK;
 }
@@ -645,8 +645,8 @@ void scb_index_config(const scb_index *idx, scb_config *out) { *out = idx->cfg; 
 uint64_t scb_index_inserted_kmers(const scb_index *idx) { return idx->inserted_kmers; }
 
 int scb_index_set_method(scb_index *idx, int method) {
-    if (method != SCB_METHOD_BATCHED && method != SCB_METHOD_PLAIN)
-        return invalid_config("method must be SCB_METHOD_BATCHED or SCB_METHOD_PLAIN");
+    if (method != SCB_METHOD_BATCHED && method != SCB_METHOD_PLAIN && method != SCB_METHOD_AUTO)
+        return invalid_config("method must be SCB_METHOD_BATCHED, SCB_METHOD_PLAIN or SCB_METHOD_AUTO");
     idx->method = scb::Method(method);
     return SCB_OK;
 }

@@ -12,14 +12,14 @@ struct scb_builder {
     scb_config cfg;
     std::unique_ptr<scb::Engine> engine;
     uint64_t inserted_kmers = 0, inserted_smers = 0;
-    scb::Method method = scb::kBatched;
+    scb::Method method = scb::kAuto;
 };
 
 struct scb_index {
     scb_config cfg;
     std::unique_ptr<scb::Engine> engine;
     uint64_t inserted_kmers = 0;
-    scb::Method method = scb::kBatched;
+    scb::Method method = scb::kAuto;
 };
 
 struct scb_spectrum_acc {

@@ -3111,6 +3111,7 @@ Status run_graphable(Workspace &ws, bool use_graph, Workspace::GraphSlot &slot, 
 
 Status Engine::insert(const DeviceBatch &b, Method method, InsertResult &r) {
     r = InsertResult{};
+    method = resolve(method, b.n_bases);
     Status s = check_batch(geo_, b);
     if (!s.good()) return s;
     if (b.n_bases < geo_.k || b.n_reads == 0) return Status::ok();
@@ -3211,6 +3212,7 @@ Status Engine::insert(const DeviceBatch &b, Method method, InsertResult &r) {
 Status Engine::query(const DeviceBatch &b, Method method, void *d_values, uint32_t *d_valid_bits,
                      QueryResult &r) const {
     r = QueryResult{};
+    method = resolve(method, b.n_bases);
     Status s = check_batch(geo_, b);
     if (!s.good()) return s;
     s = ensure_zeroed();
@@ -3389,6 +3391,7 @@ uint64_t carve_query(uint8_t *base, const Geometry &g, uint64_t n_bases, uint32_
 Status Engine::query_noalloc(const DeviceBatch &b, Method method, void *d_values, uint32_t *d_valid_bits,
                              QueryResult &r) const {
     r = QueryResult{};
+    method = resolve(method, b.n_bases);
     if (b.n_bases > kNoAllocMaxBases)
         return Status{6, "internal state error: the allocation-free query takes at most " +
                              std::to_string(kNoAllocMaxBases) + " bases per call (" + std::to_string(b.n_bases) +
