@@ -912,7 +912,7 @@ int scb_index_spectrum(const scb_index *idx, const uint8_t *bases, const uint64_
     int rc = check_offsets(offsets, n);
     if (rc) return rc;
     scb::SpectrumResult r;
-    Status s = idx->engine->spectrum_host(bases, offsets, n, r);
+    Status s = idx->engine->spectrum_host(bases, offsets, n, r, idx->method);
     if (!s.good()) return fail(s);
     for (size_t i = 0; i < r.rows.size() && i < cap; ++i) values[i] = r.rows[i].first, counts[i] = r.rows[i].second;
     if (n_rows) *n_rows = r.rows.size();
@@ -926,7 +926,7 @@ int scb_index_spectrum_device(const scb_index *idx, const uint8_t *d_bases, cons
                               size_t cap, size_t *n_rows, uint64_t *queried, uint64_t *distinct) {
     scb::DeviceBatch db{d_bases, d_offsets, n, total_bases};
     scb::SpectrumResult r;
-    Status s = idx->engine->spectrum(db, r);
+    Status s = idx->engine->spectrum(db, r, idx->method);
     if (!s.good()) return fail(s);
     for (size_t i = 0; i < r.rows.size() && i < cap; ++i) values[i] = r.rows[i].first, counts[i] = r.rows[i].second;
     if (n_rows) *n_rows = r.rows.size();

@@ -141,7 +141,8 @@ class Engine {
     // stream; calls on one device are serialized.
     Status query_noalloc(const DeviceBatch &b, Method method, void *d_values, uint32_t *d_valid_bits,
                          QueryResult &r) const;
-    Status spectrum(const DeviceBatch &b, SpectrumResult &r) const;
+    // kAuto as for query; spectra are the same with every method
+    Status spectrum(const DeviceBatch &b, SpectrumResult &r, Method method = kBatched) const;
     // Spectrum over several batches: each call adds the batch's distinct keys
     // to ks (keys already in it are not counted again); keyset_rows gives the
     // rows, the distinct keys and the k-mers queried so far.
@@ -175,7 +176,7 @@ class Engine {
                       uint64_t *values, uint8_t *valid, QueryResult &r) const;
     // any size: inputs beyond one device batch are counted chunk by chunk in a key set
     Status spectrum_host(const uint8_t *bases, const uint64_t *offsets, uint64_t n,
-                         SpectrumResult &r) const;
+                         SpectrumResult &r, Method method = kBatched) const;
     Status spectrum_accumulate_host(const uint8_t *bases, const uint64_t *offsets, uint64_t n, KeySet &ks) const;
     // pairs sorted by key
     Status kmer_map_host(const uint8_t *bases, const uint64_t *offsets, uint64_t n,
@@ -200,6 +201,7 @@ class Engine {
     Status spectrum_checks(const DeviceBatch &b) const;
     // hist: the kernel-side histogram targets (HistOut) or null with emit
     Status spectrum_pass(Workspace &ws, const DeviceBatch &b, const void *hist, bool emit) const;
+    Status spectrum_plain(Workspace &ws, const DeviceBatch &b, const void *hist) const;
 };
 
 // Device time of the stages of the calling thread's last insert/query call,

@@ -362,6 +362,74 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
         p[i] = v;
 }
 
+constexpr uint32_t kHistSmall = 4096;
+constexpr uint32_t kHistBigCap = 1u << 20;
+
+// ---- distinct-k-mer spectrum (filter.rs:407-460; fastx.rs:255-308, 392-410) ----
+// A canonical k-mer always lands in the same counter block (its minimizer is
+// strand-symmetric), so every occurrence of a key is processed by the work
+// items of one bin. Single-item bins whose k-mers fit dedup in an LDS hash set;
+// split or oversized bins dedup in a global open-addressing set. The estimate
+// of a key is the same at every occurrence, so "first wins" needs no order.
+constexpr uint32_t kSetCap = 8192;            // LDS key set (64 KiB)
+constexpr uint32_t kSetFill = kSetCap * 3 / 4;
+constexpr uint64_t kEmptyKey = ~0ull;          // keys use at most 62 bits (k <= 31)
+
+__device__ __forceinline__ uint32_t key_slot(uint64_t key, uint32_t log2cap) {
+    return uint32_t((key * 0x9E3779B97F4A7C15ull) >> (64 - log2cap));
+}
+
+// Returns true if key was not in the set (this call inserted it).
+__device__ __forceinline__ bool set_insert(unsigned long long *set, uint32_t log2cap, uint64_t key) {
+    const uint32_t mask = (1u << log2cap) - 1;
+    uint32_t h = key_slot(key, log2cap);
+    for (;;) {
+        unsigned long long cur = set[h];
+        if (cur == key) return false;
+        if (cur == kEmptyKey) {
+            unsigned long long prev = atomicCAS(&set[h], kEmptyKey, (unsigned long long)key);
+            if (prev == kEmptyKey) return true;
+            if (prev == key) return false;
+        }
+        h = (h + 1) & mask;
+    }
+}
+
+struct HistOut {
+    unsigned long long *dense;
+    uint32_t dense_n;
+    uint32_t *big_keys;
+    unsigned long long *big_counts;
+    uint64_t *stats;  // stats[kStOverflow] is raised when big_keys is full
+};
+
+__device__ __forceinline__ void spec_hist_add(const HistOut &a, uint32_t *s_hist, uint32_t v) {
+    if (v < kHistSmall) {
+        atomicAdd(&s_hist[v], 1u);
+    } else if (v < a.dense_n) {
+        atomicAdd(&a.dense[v], 1ull);
+    } else {
+        uint32_t hsh = uint32_t(xs64(v)) & (kHistBigCap - 1);
+        for (uint32_t probe = 0; probe < kHistBigCap; ++probe) {
+            uint32_t key = a.big_keys[hsh];
+            if (key == 0) {
+                uint32_t prev = atomicCAS(&a.big_keys[hsh], 0u, v);
+                if (prev == 0 || prev == v) {
+                    atomicAdd(&a.big_counts[hsh], 1ull);
+                    return;
+                }
+                key = prev;
+            }
+            if (key == v) {
+                atomicAdd(&a.big_counts[hsh], 1ull);
+                return;
+            }
+            hsh = (hsh + 1) & (kHistBigCap - 1);
+        }
+        atomicAdd((unsigned long long *)&a.stats[kStOverflow], 1ull);
+    }
+}
+
 // ----------------------------------------------------------------------------
 // Scan tile (front end shared by both pipelines)
 // ----------------------------------------------------------------------------
@@ -385,6 +453,10 @@ struct ScanArgs {
     uint32_t aligned16;    // bases pointer is 16-byte aligned
     const uint32_t *tile_r0;  // first record starting after each tile's left edge
     uint32_t ntiles;
+    // plain spectrum: the key set (a power of two of slots, at most half full) and the histogram
+    unsigned long long *gset;
+    uint32_t glog2;
+    HistOut hist;
 };
 
 // For every scan tile, the first record whose start lies after the tile's left
@@ -408,7 +480,7 @@ __global__ void k_tile_r0(const uint64_t *offsets, uint64_t n_reads, uint64_t n_
     r0[t] = uint32_t(lo);
 }
 
-enum ScanMode { kModeRecords = 0, kModePlainInsert = 1, kModePlainQuery = 2 };
+enum ScanMode { kModeRecords = 0, kModePlainInsert = 1, kModePlainQuery = 2, kModePlainSpectrum = 3 };
 
 // One tile owns windows [t0, t0 + kTile). Window j is valid iff its k bases are
 // ACGT and belong to one record (the maximal-ACGT-run split of api.rs:676-726,
@@ -441,6 +513,8 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
     __shared__ uint32_t s_bnd[kTile / 32 + 2];
     __shared__ uint64_t s_red[3][kTileThreads / 64];
     __shared__ uint64_t s_base;
+    // plain spectrum: counts of small estimates of the tile's distinct keys
+    __shared__ uint32_t s_hsm[MODE == kModePlainSpectrum ? kHistSmall : 1];
     // ntHash XOR-prefix terms by (strand, code, rotation): forward rotr(seed(c), r),
     // reverse rotl(seed(c ^ 2), r); rows of 33 words put the four codes of one
     // rotation in different banks. Read before the prefixes are written to s_G.
@@ -502,6 +576,8 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
         const int64_t t0 = int64_t(tile) * kTile;
         const int64_t LB = t0 - kLeft;
         for (int i = tid; i < kSpanWords; i += kTileThreads) s_brk[i] = 0;
+        if (MODE == kModePlainSpectrum)
+            for (int i = tid; i < int(kHistSmall); i += kTileThreads) s_hsm[i] = 0;
         // the first read start after the span's left edge (uniform), and this
         // thread's first candidate start, loaded now so that the load overlaps
         // the work before the read boundaries are marked (the index is clamped,
@@ -850,7 +926,25 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
                 uint32_t A0, A1, A2, A3;
                 load_aligned(s_pk, uint64_t(hj + kLeft - 1), A0, A1, A2, A3);
                 uint32_t nbases = nk + g.k - 1;
-                if (MODE == kModePlainInsert) {
+                if (MODE == kModePlainSpectrum) {
+                    // filter.rs:407-460 (as filter.rs:302-335): the estimate of every
+                    // k-mer, counted when its canonical key enters the key set
+                    WinMin<WINC> wm;
+                    for_each_smer_kmer(A0, A1, A2, A3, nbases, g.s, g.k, [&](uint32_t j, uint64_t smer, uint64_t key) {
+                        uint64_t h = xs64(smer);
+                        uint32_t est = Ctr<BITS>::kMax;
+                        for (uint32_t t = 0; t < g.nh; ++t) {
+                            uint32_t v = load_ctr<BITS>(a.table, cbase + (h & amask));
+                            est = v < est ? v : est;
+                            if (t + 1 < g.nh) h = xs64(h);
+                        }
+                        const uint32_t ab = wm.push(est, g.win);
+                        if (j + 1 >= g.win && set_insert(a.gset, a.glog2, key)) {
+                            ++acc_pos;  // distinct keys
+                            spec_hist_add(a.hist, s_hsm, ab);
+                        }
+                    });
+                } else if (MODE == kModePlainInsert) {
                     // filter.rs:224-233
                     for_each_smer(A0, A1, A2, A3, nbases, g.s, [&](uint32_t, uint64_t smer) {
                         uint64_t h = xs64(smer);
@@ -893,7 +987,7 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
     // its start to the next boundary, so over the batch the records' k-mers are
     // its valid windows, and each record adds k - s s-mers to its k-mers.
     {
-        uint32_t v1 = uint32_t(MODE == kModePlainQuery ? acc_pos : acc_smers);
+        uint32_t v1 = uint32_t(MODE == kModePlainQuery || MODE == kModePlainSpectrum ? acc_pos : acc_smers);
         uint64_t v2 = acc_sum;
         if (MODE != kModeRecords) {
             for (int off = 32; off > 0; off >>= 1) {
@@ -910,14 +1004,20 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
             for (int i = 0; i < nwave; ++i) t[q] += s_red[q][i];
         if (MODE == kModeRecords) t[1] = t[0] + uint64_t(g.k - g.s) * tile_records;
         unsigned long long *st = (unsigned long long *)a.stats + kStSpread + (blockIdx.x & (kSeg - 1));
-        if (t[0]) atomicAdd(&st[kSeg * (MODE == kModePlainQuery ? kSpQueried : kSpValid)], t[0]);
+        constexpr bool kQueried = MODE == kModePlainQuery || MODE == kModePlainSpectrum;
+        if (t[0]) atomicAdd(&st[kSeg * (kQueried ? kSpQueried : kSpValid)], t[0]);
         if (MODE == kModePlainQuery) {
             if (t[1]) atomicAdd(&st[kSeg * kSpPositive], t[1]);
             if (t[2]) atomicAdd(&st[kSeg * kSpChecksum], t[2]);
+        } else if (MODE == kModePlainSpectrum) {
+            if (t[1]) atomicAdd((unsigned long long *)a.stats + kStDistinct, t[1]);
         } else if (t[1]) {
             atomicAdd(&st[kSeg * kSpSmers], t[1]);
         }
     }
+    if (MODE == kModePlainSpectrum)  // the tile's small estimates into the histogram
+        for (int i = tid; i < int(kHistSmall); i += kTileThreads)
+            if (s_hsm[i]) atomicAdd(&a.hist.dense[i], (unsigned long long)s_hsm[i]);
 }
 
 // ----------------------------------------------------------------------------
@@ -1837,39 +1937,6 @@ __global__ __launch_bounds__(kQueryThreads) void k_bin_query(BinArgs a, DevGeo g
     }
 }
 
-constexpr uint32_t kHistSmall = 4096;
-constexpr uint32_t kHistBigCap = 1u << 20;
-
-// ---- distinct-k-mer spectrum (filter.rs:407-460; fastx.rs:255-308, 392-410) ----
-// A canonical k-mer always lands in the same counter block (its minimizer is
-// strand-symmetric), so every occurrence of a key is processed by the work
-// items of one bin. Single-item bins whose k-mers fit dedup in an LDS hash set;
-// split or oversized bins dedup in a global open-addressing set. The estimate
-// of a key is the same at every occurrence, so "first wins" needs no order.
-constexpr uint32_t kSetCap = 8192;            // LDS key set (64 KiB)
-constexpr uint32_t kSetFill = kSetCap * 3 / 4;
-constexpr uint64_t kEmptyKey = ~0ull;          // keys use at most 62 bits (k <= 31)
-
-__device__ __forceinline__ uint32_t key_slot(uint64_t key, uint32_t log2cap) {
-    return uint32_t((key * 0x9E3779B97F4A7C15ull) >> (64 - log2cap));
-}
-
-// Returns true if key was not in the set (this call inserted it).
-__device__ __forceinline__ bool set_insert(unsigned long long *set, uint32_t log2cap, uint64_t key) {
-    const uint32_t mask = (1u << log2cap) - 1;
-    uint32_t h = key_slot(key, log2cap);
-    for (;;) {
-        unsigned long long cur = set[h];
-        if (cur == key) return false;
-        if (cur == kEmptyKey) {
-            unsigned long long prev = atomicCAS(&set[h], kEmptyKey, (unsigned long long)key);
-            if (prev == kEmptyKey) return true;
-            if (prev == key) return false;
-        }
-        h = (h + 1) & mask;
-    }
-}
-
 // One workgroup: total k-mers of the bins that must use the global set.
 __global__ __launch_bounds__(1024) void k_spectrum_plan(const uint32_t *bin_count, const uint32_t *bin_kmers,
                                                         uint64_t nbins, uint32_t per_item, uint64_t *stats) {
@@ -1887,14 +1954,6 @@ __global__ __launch_bounds__(1024) void k_spectrum_plan(const uint32_t *bin_coun
 // Histogram of estimates: values below kHistSmall are staged in LDS by the
 // caller, values below dense_n go to a dense global array, larger (32-bit)
 // values to an open-addressing table of kHistBigCap entries.
-struct HistOut {
-    unsigned long long *dense;
-    uint32_t dense_n;
-    uint32_t *big_keys;
-    unsigned long long *big_counts;
-    uint64_t *stats;  // stats[kStOverflow] is raised when big_keys is full
-};
-
 struct SpecArgs {
     BinArgs b;
     const uint32_t *bin_kmers;
@@ -1909,33 +1968,6 @@ struct SpecArgs {
     uint32_t *emit_vals;
     uint64_t emit_cap;
 };
-
-__device__ __forceinline__ void spec_hist_add(const HistOut &a, uint32_t *s_hist, uint32_t v) {
-    if (v < kHistSmall) {
-        atomicAdd(&s_hist[v], 1u);
-    } else if (v < a.dense_n) {
-        atomicAdd(&a.dense[v], 1ull);
-    } else {
-        uint32_t hsh = uint32_t(xs64(v)) & (kHistBigCap - 1);
-        for (uint32_t probe = 0; probe < kHistBigCap; ++probe) {
-            uint32_t key = a.big_keys[hsh];
-            if (key == 0) {
-                uint32_t prev = atomicCAS(&a.big_keys[hsh], 0u, v);
-                if (prev == 0 || prev == v) {
-                    atomicAdd(&a.big_counts[hsh], 1ull);
-                    return;
-                }
-                key = prev;
-            }
-            if (key == v) {
-                atomicAdd(&a.big_counts[hsh], 1ull);
-                return;
-            }
-            hsh = (hsh + 1) & (kHistBigCap - 1);
-        }
-        atomicAdd((unsigned long long *)&a.stats[kStOverflow], 1ull);
-    }
-}
 
 // Appends (key, v) to the emit arrays: the emitting lanes of a wave take
 // consecutive slots with one atomic add by the lowest of them.
@@ -2876,8 +2908,12 @@ struct Launch {
 };
 
 template <int MODE>
-Status launch_scan(Launch &L, const DeviceBatch &b, uint64_t seg_cap, uint32_t *valid_bits, void *out) {
+Status launch_scan(Launch &L, const DeviceBatch &b, uint64_t seg_cap, uint32_t *valid_bits, void *out,
+                   unsigned long long *gset = nullptr, uint32_t glog2 = 0, const HistOut *hist = nullptr) {
     ScanArgs a{};
+    a.gset = gset;
+    a.glog2 = glog2;
+    if (hist) a.hist = *hist;
     a.bases = b.d_bases;
     a.offsets = b.d_offsets;
     a.n_reads = b.n_reads;
@@ -3544,11 +3580,36 @@ Status Engine::spectrum_pass(Workspace &ws, const DeviceBatch &b, const void *hi
     return Status::ok();
 }
 
-Status Engine::spectrum(const DeviceBatch &b, SpectrumResult &r) const {
+// The plain spectrum: one thread per super-k-mer estimates its k-mers from the
+// table in HBM and inserts their canonical keys into one key set for the batch
+// (a slot per base, so it stays at most half full); a key counts when it
+// enters the set.
+Status Engine::spectrum_plain(Workspace &ws, const DeviceBatch &b, const void *hist) const {
+    Launch L{geo_, dev_geo(geo_), ws, d_table_, device_};
+    hipStream_t st = ws.stream;
+    fill(reinterpret_cast<uint32_t *>(ws.stats.p), 2 * kStatsLen, 0, st);
+    uint32_t glog2 = 10;
+    while ((uint64_t(1) << glog2) < 2 * b.n_bases + 2) ++glog2;
+    SCB_HIP(ws.spec_gset.reserve(uint64_t(1) << glog2));
+    fill(reinterpret_cast<uint32_t *>(ws.spec_gset.p), (uint64_t(1) << glog2) * 2, 0xFFFFFFFFu, st);
+    HistOut h = *static_cast<const HistOut *>(hist);
+    h.stats = ws.stats.p;
+    mark(ws, 0);
+    Status s = launch_scan<kModePlainSpectrum>(L, b, 0, nullptr, nullptr, ws.spec_gset.p, glog2, &h);
+    if (!s.good()) return s;
+    mark(ws, 1);
+    if ((s = finish(ws)).code) return s;
+    collect_times(ws, false);
+    if (ws.h_stats[kStOverflow]) return Status{8, "device error: spectrum has more than 2^20 distinct large estimates"};
+    return Status::ok();
+}
+
+Status Engine::spectrum(const DeviceBatch &b, SpectrumResult &r, Method method) const {
     r = SpectrumResult{};
     Status s = spectrum_checks(b);
     if (!s.good()) return s;
     if (b.n_bases < geo_.k || b.n_reads == 0) return Status::ok();
+    const bool plain = resolve(method, b.n_bases) == kPlain;
     SCB_HIP(hipSetDevice(device_));
     auto wsp = acquire();
     if (!wsp) return Status{8, "device error: cannot create a workspace"};
@@ -3561,7 +3622,7 @@ Status Engine::spectrum(const DeviceBatch &b, SpectrumResult &r) const {
         }
         ws.hist.clear(geo_, ws.stream);
         HistOut h = ws.hist.out(ws.stats.p);
-        if ((s = spectrum_pass(ws, b, &h, false)).code) break;
+        if ((s = plain ? spectrum_plain(ws, b, &h) : spectrum_pass(ws, b, &h, false)).code) break;
         if ((s = ws.hist.rows(geo_, ws.stream, r.rows)).code) break;
         r.queried = ws.h_stats[kStQueried];
         r.distinct = ws.h_stats[kStDistinct];
@@ -3883,7 +3944,7 @@ constexpr uint64_t kSpectrumChunkBases = uint64_t(1) << 28;
 // One call when the input fits one device batch; larger inputs go through a
 // key set chunk by chunk, so a key seen in an earlier chunk is not counted again.
 Status Engine::spectrum_host(const uint8_t *bases, const uint64_t *offsets, uint64_t n,
-                             SpectrumResult &r) const {
+                             SpectrumResult &r, Method method) const {
     r = SpectrumResult{};
     SCB_HIP(hipSetDevice(device_));
     const uint64_t nb = offsets[n] - offsets[0];
@@ -3892,7 +3953,7 @@ Status Engine::spectrum_host(const uint8_t *bases, const uint64_t *offsets, uint
     Status s;
     if (nb < kMaxBatchBases) {
         s = for_each_staged_chunk(*st, bases, offsets, n, kMaxBatchBases,
-                                  [&](const DeviceBatch &b) { return spectrum(b, r); });
+                                  [&](const DeviceBatch &b) { return spectrum(b, r, method); });
     } else {
         KeySetPtr ks;
         s = keyset_create(ks);

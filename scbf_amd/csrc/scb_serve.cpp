@@ -268,9 +268,9 @@ void scb_server::run_spectrum(Staging &s, Request *r) {
             hipMemcpyAsync(s.d_offsets, s.h_offsets, (r->n + 1) * 8, hipMemcpyHostToDevice, s.st) != hipSuccess ||
             hipStreamSynchronize(s.st) != hipSuccess)
             st = Status{8, "device error: staging a spectrum request failed"};
-        if (st.good()) st = eng.spectrum(scb::DeviceBatch{s.d_bases, s.d_offsets, r->n, r->nb}, sr);
+        if (st.good()) st = eng.spectrum(scb::DeviceBatch{s.d_bases, s.d_offsets, r->n, r->nb}, sr, idx->method);
     } else {
-        st = eng.spectrum_host(r->bases, r->offsets, r->n, sr);  // larger than the staging area
+        st = eng.spectrum_host(r->bases, r->offsets, r->n, sr, idx->method);  // larger than the staging area
     }
     if (!st.good()) {
         set_error(r, st);

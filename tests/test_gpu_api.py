@@ -135,11 +135,12 @@ SPEC_GEOS = [(9, 5, 7, 3, 8, 16, 6), (31, 21, 25, 4, 16, 20, 9), (21, 15, 17, 3,
              (31, 11, 15, 2, 16, 18, 8), (5, 3, 4, 2, 8, 12, 5), (31, 21, 25, 4, 16, 22, 16)]
 
 
+@pytest.mark.parametrize("method", [0, 1], ids=["batched", "plain"])
 @pytest.mark.parametrize("geo", SPEC_GEOS, ids=[f"k{g[0]}s{g[2]}u{g[4]}c{g[5]}b{g[6]}" for g in SPEC_GEOS])
-def test_spectrum_parity(gpu, oracle, geo):
+def test_spectrum_parity(gpu, oracle, geo, method):
     scb = gpu
     k, m, s, h, bits, cse, bse = geo
-    print(f"spectrum case {geo}", flush=True)
+    print(f"spectrum case {geo} method {method}", flush=True)
     g = synth.genome(30_000, 21)
     ib, io = synth.fixed_reads(g, 1500, 150, 22, sub_rate=0.01, n_rate=0.002)
     qb, qo = synth.fixed_reads(g, 700, 150, 23, sub_rate=0.02, n_rate=0.002)
@@ -148,6 +149,7 @@ def test_spectrum_parity(gpu, oracle, geo):
     b = scb.SuperCountingBloomBuilder(cfg)
     b.add_sequences((ib, io))
     idx = b.into_index()
+    idx.set_method(method)
     # query as DnaChunks (split_non_actg): maximal ACGT runs >= k of every read
     chunks = []
     for i in range(qo.size - 1):
@@ -168,15 +170,16 @@ def test_spectrum_parity(gpu, oracle, geo):
 SPEC_DUP_GEOS = SPEC_GEOS + [(9, 5, 7, 2, 32, 1, 0), (9, 5, 7, 3, 8, 4, 2)]  # bins of 8 and 4 bytes
 
 
+@pytest.mark.parametrize("method", [0, 1], ids=["batched", "plain"])
 @pytest.mark.parametrize("geo", SPEC_DUP_GEOS, ids=[f"k{g[0]}s{g[2]}u{g[4]}c{g[5]}b{g[6]}" for g in SPEC_DUP_GEOS])
-def test_spectrum_repeated_chunks_parity(gpu, oracle, geo):
+def test_spectrum_repeated_chunks_parity(gpu, oracle, geo, method):
     """Every chunk five times over plus homopolymers and tandem repeats: most
     records of a bin repeat an earlier one and most keys are found present;
     queried, distinct and the rows equal the oracle's, also for bins of 8 and 4
     bytes."""
     scb = gpu
     k, m, s, h, bits, cse, bse = geo
-    print(f"spectrum repeated case {geo}", flush=True)
+    print(f"spectrum repeated case {geo} method {method}", flush=True)
     g = synth.genome(20_000, 31)
     ib, io = synth.fixed_reads(g, 1000, 150, 32, sub_rate=0.01)
     qb, qo = synth.fixed_reads(g, 300, 150, 33, sub_rate=0.01)
@@ -190,6 +193,7 @@ def test_spectrum_repeated_chunks_parity(gpu, oracle, geo):
     b.add_sequences((ib, io))
     b.add_sequence(b"".join(reads[-5:]))
     idx = b.into_index()
+    idx.set_method(method)
     rows, queried, distinct = idx.spectrum(chunks)
     of = oracle.Filter(oracle.make_cfg(k, m, s, h, bits, cse, bse))
     oracle.insert_records(of, ib, io)
