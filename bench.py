@@ -353,6 +353,10 @@ def serve_bench(scb, idx, genome, qps_steps, min_requests=2000, pool_size=160, p
     counts."""
     import threading
     from scbf_amd import synth
+    # the server runs the index's method: the default one, which takes the plain
+    # path for coalesced batches below the table-dependent size and the batched
+    # path above
+    idx.set_method(scb.METHOD_AUTO)
     pool = synth.serving_requests(genome, pool_size, seed)
     worst = synth.serving_requests(genome, 1, seed + 1, n_range=(10_000, 10_000), len_range=(20_000, 20_000),
                                    genome_frac=1.0)[0]
@@ -469,7 +473,7 @@ def serve_bench(scb, idx, genome, qps_steps, min_requests=2000, pool_size=160, p
     return {"steps": steps, "min_requests_per_step": min_requests, "request_pool": len(pool),
             "mean_request_bases": round(float(np.mean(sizes))), "max_pool_request_bases": int(max(sizes)),
             "worst_case_request": {"records": int(worst[1].size - 1), "bases": int(worst[1][-1])},
-            "mix": "80% abundances, 20% spectrum", "p99_limit_ms": p99_limit_ms,
+            "mix": "80% abundances, 20% spectrum", "method": "auto", "p99_limit_ms": p99_limit_ms,
             "highest_qps_within_limit": knee, "served_equals_direct": ok}
 
 
