@@ -380,7 +380,7 @@ typedef struct scb_serve_config {
     uint64_t max_batch_requests; /* requests coalesced into one launch (default 1024) */
     uint64_t hbm_budget_bytes;   /* device memory for staging and pipeline scratch of all
                                     workers (default 16 GiB); max_batch_bases is lowered to fit */
-    uint32_t workers;            /* launches in flight at once (default 2) */
+    uint32_t workers;            /* launches in flight at once (default 4) */
     uint32_t max_pending;        /* queued requests before submit answers SCB_ERR_BUSY (4096) */
 } scb_serve_config;
 

@@ -705,7 +705,7 @@ class ServeConfig:
     max_batch_bases: int = 1 << 27
     max_batch_requests: int = 1024
     hbm_budget_bytes: int = 16 << 30
-    workers: int = 2
+    workers: int = 4
     max_pending: int = 4096
 
     def _c(self):

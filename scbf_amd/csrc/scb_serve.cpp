@@ -334,7 +334,7 @@ void scb_serve_config_default(scb_serve_config *cfg) {
     cfg->max_batch_bases = uint64_t(1) << 27;
     cfg->max_batch_requests = 1024;
     cfg->hbm_budget_bytes = uint64_t(16) << 30;
-    cfg->workers = 2;
+    cfg->workers = 4;
     cfg->max_pending = 4096;
 }
 
