@@ -176,7 +176,7 @@ size_t scb_builder_threads(const scb_builder *b);
 void scb_builder_config(const scb_builder *b, scb_config *out);
 /* extension: the device method. SCB_METHOD_AUTO (default) runs the plain
  * method (one thread per super-k-mer, global counter accesses) for batches
- * below 2^17 bases + one base per 512 table bytes, where its lower fixed cost
+ * below 2^17 bases + one base per 256 table bytes, where its lower fixed cost
  * wins, and the batched method (sort by counter bin, bins staged in LDS)
  * above; SCB_METHOD_BATCHED and SCB_METHOD_PLAIN force one. Results are the
  * same with every method. The index takes the builder's method; see

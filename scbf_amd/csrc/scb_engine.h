@@ -125,10 +125,10 @@ class Engine {
     // the method kAuto resolves to for a batch of n_bases bases
     Method resolve(Method m, uint64_t n_bases) const { return m != kAuto ? m : n_bases < plain_bases() ? kPlain : kBatched; }
     // Below this many bases the plain method is faster: 2^17 bases plus one per
-    // 512 table bytes (the batched query stages the bins it touches; measured on
+    // 256 table bytes (the batched query stages the bins it touches; measured on
     // the MI355X at 2^30 16-bit slots: plain 0.11 ms against batched 0.42 ms at
     // 150 kbp, 0.60 against 0.91 ms at 4.5 Mbp, 1.56 against 0.98 ms at 15 Mbp).
-    uint64_t plain_bases() const { return (uint64_t(1) << 17) + geo_.table_bytes() / 512; }
+    uint64_t plain_bases() const { return (uint64_t(1) << 17) + geo_.table_bytes() / 256; }
 
     Status insert(const DeviceBatch &b, Method method, InsertResult &r);
     // d_values: n_bases counters of geo.bytes each (or null); d_valid_bits:

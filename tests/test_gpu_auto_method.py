@@ -1,5 +1,5 @@
 """GPU tests of the default method choice (SCB_METHOD_AUTO): a batch below
-2^17 bases + one per 512 table bytes runs the plain method, a larger one the
+2^17 bases + one per 256 table bytes runs the plain method, a larger one the
 batched method, and both give the oracle's counters and estimates."""
 import ctypes as C
 
@@ -11,7 +11,7 @@ from scbf_amd import synth
 
 pytestmark = pytest.mark.gpu
 
-GEO = (31, 21, 25, 4, 16, 22, 9)  # 8 MiB table: plain below 2^17 + 2^14 bases
+GEO = (31, 21, 25, 4, 16, 22, 9)  # 8 MiB table: plain below 2^17 + 2^15 bases
 
 
 def _stages(scb):
