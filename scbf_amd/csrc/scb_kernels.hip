@@ -240,11 +240,26 @@ __device__ __forceinline__ uint32_t wave_scan_incl(uint32_t x) {
 
 // ntHash seeds (low 32 bits of the original ntHash seeds), by 2-bit code
 // A=0 C=1 T=2 G=3. Third-party arithmetic (seq-hash 0.1.2), restated.
-__device__ __forceinline__ uint32_t nt_seed(uint32_t c) {
-    uint32_t a = (c & 1) ? 0x62a02b4cu : 0x95c60474u;  // C : A
-    uint32_t b = (c & 1) ? 0x82572324u : 0x4be24456u;  // G : T
-    return (c & 2) ? b : a;
+constexpr uint32_t seed_of(uint32_t c) {
+    return c == 0 ? 0x95c60474u : c == 1 ? 0x62a02b4cu : c == 2 ? 0x4be24456u : 0x82572324u;
 }
+
+// The ntHash XOR-prefix terms of the scan by (strand, code, rotation), entry
+// strand * 128 + code * 32 + r: forward rotr(seed(code), r), reverse
+// rotl(seed(code ^ 2), r). Built at compile time.
+struct SeedTerms {
+    uint32_t v[256];
+};
+constexpr uint32_t rotl_c(uint32_t x, uint32_t r) { return r == 0 ? x : (x << r) | (x >> (32 - r)); }
+constexpr SeedTerms make_seed_terms() {
+    SeedTerms s{};
+    for (uint32_t e = 0; e < 256; ++e) {
+        const uint32_t strand = e >> 7, c = (e >> 5) & 3, r = e & 31;
+        s.v[e] = strand ? rotl_c(seed_of(c ^ 2), r) : rotl_c(seed_of(c), (32 - r) & 31);
+    }
+    return s;
+}
+__device__ constexpr SeedTerms kSeedTerms = make_seed_terms();
 
 __device__ __forceinline__ uint32_t get_bits(const uint32_t *bm, int li, int n) {
     if (n <= 0) return 0;
@@ -585,11 +600,8 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
         const uint32_t r0 = a.tile_r0[tile];
         const uint64_t rfirst = uint64_t(r0) + tid;
         const int64_t ofirst = int64_t(a.offsets[min<uint64_t>(rfirst, a.n_reads - 1)]);
-        for (int e = tid; e < 2 * 4 * 32; e += kTileThreads) {
-            const uint32_t strand = uint32_t(e) >> 7, c = (uint32_t(e) >> 5) & 3, r = uint32_t(e) & 31;
-            s_seed[(strand * 4 + c) * kSeedRow + r] =
-                strand ? rotl32(nt_seed(c ^ 2), r) : rotl32(nt_seed(c), (32 - r) & 31);
-        }
+        static_assert(kTileThreads == 256, "one seed term per thread");
+        s_seed[(tid >> 5) * kSeedRow + (tid & 31)] = kSeedTerms.v[tid];
 
         // (1) codes and masks of this thread's 16 bases
         uint32_t cw = 0;
