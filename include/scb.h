@@ -147,7 +147,12 @@ int scb_tuning_cache_load(const char *path);
  * 2^counter_slots_exponent-counter table in HBM. */
 int scb_builder_new(const scb_config *cfg, scb_builder **out);
 void scb_builder_free(scb_builder *b);
-/* SuperCountingBloomBuilder::add_sequence (api.rs:123-138) */
+/* SuperCountingBloomBuilder::add_sequence (api.rs:123-138). *inserted_kmers
+ * (the sequence's valid k-mer windows) is counted on the host at once; the
+ * sequence joins a pending host batch that goes to the device when it holds
+ * 64 MiB or 2^20 sequences, or before any call that reads the builder
+ * (inserted_smers, counters, into_index, the batch and FASTA inserts). A
+ * device error of a pending batch is returned by that later call. */
 int scb_builder_add_sequence(scb_builder *b, const uint8_t *seq, size_t len,
                              uint64_t *inserted_kmers);
 /* Batched add_sequence over n host records (api.rs:123-138 per record;

@@ -284,8 +284,35 @@ uint64_t count_chunks(const uint8_t *bases, const uint64_t *offsets, size_t n, s
     return c;
 }
 
+// add_sequence calls are gathered into one host batch of up to this many bases
+// (or records) before it goes to the device: a call then costs a host copy and
+// a host count of its k-mers, not a device round trip.
+constexpr size_t kDeferBases = size_t(64) << 20, kDeferRecords = size_t(1) << 20;
+
+// Sends the gathered add_sequence records to the device. The pending batch is
+// dropped either way: after a device error the builder's table is undefined.
+int builder_flush(scb_builder *b) {
+    if (b->pend_offsets.size() <= 1) return SCB_OK;
+    scb::InsertResult r;
+    Status s = b->engine->insert_host(b->pend_bases.data(), b->pend_offsets.data(), b->pend_offsets.size() - 1,
+                                      b->method, r);
+    const uint64_t want = b->pend_kmers;
+    b->pend_bases.clear();
+    b->pend_offsets.assign(1, 0);
+    b->pend_kmers = 0;
+    if (!s.good()) return fail(s);
+    if (r.inserted_kmers != want)
+        return fail(SCB_ERR_INTERNAL_STATE, "internal state error: the device inserted " +
+                                                std::to_string(r.inserted_kmers) + " k-mers of " +
+                                                std::to_string(want) + " counted on the host");
+    b->inserted_smers = sat_add(b->inserted_smers, r.inserted_smers);
+    return SCB_OK;
+}
+
+
 int builder_insert_host(scb_builder *b, const uint8_t *bases, const uint64_t *offsets, size_t n,
                         scb_add_report *rep) {
+    if (int rc = builder_flush(b)) return rc;
     scb::InsertResult r;
     // the host count overlaps the device insert
     uint64_t indexed = 0;
@@ -471,14 +498,18 @@ int scb_builder_new(const scb_config *cfg, scb_builder **out) {
 
 void scb_builder_free(scb_builder *b) { delete b; }
 
+// api.rs:123-138: returns the k-mers of the sequence (valid windows), counted on
+// the host; the insert itself joins the builder's pending batch.
 int scb_builder_add_sequence(scb_builder *b, const uint8_t *seq, size_t len, uint64_t *inserted_kmers) {
-    uint64_t offs[2] = {0, len};
-    scb_add_report rep{};
-    uint64_t before = b->inserted_kmers;
-    int rc = builder_insert_host(b, seq, offs, 1, &rep);
-    if (rc) return rc;
-    (void)before;
-    if (inserted_kmers) *inserted_kmers = rep.inserted_kmers;
+    const uint64_t kmers = valid_runs(seq, len, size_t(b->cfg.k)).second;
+    if (len) {
+        b->pend_bases.insert(b->pend_bases.end(), seq, seq + len);
+        b->pend_offsets.push_back(b->pend_bases.size());
+        b->pend_kmers += kmers;
+    }
+    b->inserted_kmers = sat_add(b->inserted_kmers, kmers);  // api.rs:135-136
+    if (inserted_kmers) *inserted_kmers = kmers;
+    if (b->pend_bases.size() >= kDeferBases || b->pend_offsets.size() > kDeferRecords) return builder_flush(b);
     return SCB_OK;
 }
 
@@ -498,6 +529,7 @@ int scb_builder_add_sequences_device(scb_builder *b, const uint8_t *d_bases, con
                                      size_t n, uint64_t total_bases, scb_add_report *report) {
     if (total_bases >= scb::kMaxBatchBases)
         return fail(SCB_ERR_INTERNAL_STATE, "internal state error: device batch exceeds 2^32 bases");
+    if (int rc = builder_flush(b)) return rc;
     double t0 = now_s();
     scb::DeviceBatch db{d_bases, d_offsets, n, total_bases};
     scb::InsertResult r;
@@ -532,6 +564,7 @@ int scb_builder_add_fasta(scb_builder *b, const char *path, scb_add_report *repo
 int scb_builder_into_index(scb_builder *b, scb_index **out) {
     *out = nullptr;
     std::unique_ptr<scb_builder> owned(b);
+    if (int rc = builder_flush(b)) return rc;
     std::unique_ptr<scb_index> idx(new scb_index);
     idx->cfg = b->cfg;
     idx->engine = std::move(b->engine);
@@ -544,7 +577,10 @@ int scb_builder_into_index(scb_builder *b, scb_index **out) {
 }
 
 uint64_t scb_builder_inserted_kmers(const scb_builder *b) { return b->inserted_kmers; }
-uint64_t scb_builder_inserted_smers(const scb_builder *b) { return b->inserted_smers; }
+uint64_t scb_builder_inserted_smers(const scb_builder *b) {
+    (void)builder_flush(const_cast<scb_builder *>(b));  // an error stays in scb_last_error
+    return b->inserted_smers;
+}
 
 int scb_builder_set_threads(scb_builder *b, size_t threads) {  // api.rs:187-195
     if (threads == 0) return invalid_config("threads must be positive");
@@ -601,6 +637,7 @@ int scb_index_set_tuning(scb_index *idx, const scb_tuning *t) {
 void scb_index_tuning(const scb_index *idx, scb_tuning *out) { to_c(idx->engine->tuning(), out); }
 
 int scb_builder_counters(const scb_builder *b, void *out, size_t bytes) {
+    if (int rc = builder_flush(const_cast<scb_builder *>(b))) return rc;
     Status s = b->engine->download(out, bytes);
     return s.good() ? SCB_OK : fail(s);
 }

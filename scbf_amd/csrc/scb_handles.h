@@ -4,6 +4,7 @@
 
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "../../include/scb.h"
 #include "scb_engine.h"
@@ -13,6 +14,12 @@ struct scb_builder {
     std::unique_ptr<scb::Engine> engine;
     uint64_t inserted_kmers = 0, inserted_smers = 0;
     scb::Method method = scb::kAuto;
+    // Sequences of add_sequence not sent to the device yet, as one host batch
+    // (inserted_kmers already counts their k-mers); flushed before anything
+    // reads the builder's table or s-mer count.
+    std::vector<uint8_t> pend_bases;
+    std::vector<uint64_t> pend_offsets{0};
+    uint64_t pend_kmers = 0;
 };
 
 struct scb_index {
