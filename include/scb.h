@@ -49,10 +49,11 @@ enum {
     SCB_ERR_BUSY = 9 /* serving loop: the request queue is full (admission control) */
 };
 
-/* Insert/query execution method (extension). BATCHED: super-k-mers binned by
- * counter block and applied block-by-block in LDS. PLAIN: one fused launch per
- * batch, one thread per super-k-mer, global atomics / loads. Both are
- * bit-identical. */
+/* Insert/query/spectrum execution method (extension). BATCHED: super-k-mers
+ * binned by counter block and applied block-by-block in LDS. PLAIN: one fused
+ * launch per batch, one thread per super-k-mer, global atomics / loads. AUTO
+ * (the default): PLAIN below a batch size that grows with the table
+ * (scb_builder_set_method), BATCHED above. All are bit-identical. */
 enum { SCB_METHOD_BATCHED = 0, SCB_METHOD_PLAIN = 1, SCB_METHOD_AUTO = 2 };
 
 /* SuperCountingBloomConfig (config.rs:20-31); counter_bits is 8, 16 or 32. */
