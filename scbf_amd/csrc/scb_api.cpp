@@ -171,8 +171,6 @@ int for_each_batch(const char *path, Fn &&fn) {
     for (int i = 0; i < 2; ++i) free_list.emplace_back(new HostBatch);
     const size_t batch_bytes = file_batch_bytes();
     std::thread parser([&] {
-        std::string hdr;
-        std::vector<uint8_t> seq;
         std::unique_ptr<HostBatch> hb;
         auto take_free = [&]() -> bool {
             std::unique_lock<std::mutex> lk(mu);
@@ -181,6 +179,7 @@ int for_each_batch(const char *path, Fn &&fn) {
             hb = std::move(free_list.back());
             free_list.pop_back();
             hb->clear();
+            hb->bases.reserve(batch_bytes + (batch_bytes >> 4));
             return true;
         };
         auto publish = [&]() {
@@ -188,10 +187,10 @@ int for_each_batch(const char *path, Fn &&fn) {
             full.push_back(std::move(hb));
         };
         if (!take_free()) return;
-        while (rd.next(hdr, seq)) {
+        while (rd.next_seq(hb->bases)) {
             // whole records: the device scan drops every window with a non-ACGT
             // base, so a record gives the k-mers of its ACGT runs (fastx.rs:14-19)
-            hb->add(seq.data(), seq.size());
+            hb->offsets.push_back(hb->bases.size());
             if (hb->bases.size() >= batch_bytes) {
                 publish();
                 cv.notify_all();

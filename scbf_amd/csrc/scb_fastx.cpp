@@ -294,50 +294,78 @@ int Reader::open(const char *path) {
 
 int Reader::error() { return ra_ ? ra_->error() : SCB_OK; }
 
-bool Reader::getline(std::string &out) {
-    out.clear();
-    for (;;) {
-        if (pos_ >= block_.size()) {
-            if (!ra_ || !ra_->take(block_)) return !out.empty();
-            pos_ = 0;
-        }
-        const char *start = block_.data() + pos_;
-        const char *end = block_.data() + block_.size();
-        const char *nl = static_cast<const char *>(memchr(start, '\n', size_t(end - start)));
-        if (nl) {
-            out.append(start, nl);
-            pos_ = size_t(nl - block_.data()) + 1;
-            if (!out.empty() && out.back() == '\r') out.pop_back();
-            return true;
-        }
-        out.append(start, end);
+bool Reader::next_line(const char *&p, size_t &n) {
+    if (pos_ >= block_.size()) {
+        if (!ra_ || !ra_->take(block_)) return false;
+        pos_ = 0;
+    }
+    const char *start = block_.data() + pos_;
+    const char *end = block_.data() + block_.size();
+    const char *nl = static_cast<const char *>(memchr(start, '\n', size_t(end - start)));
+    if (nl) {  // the whole line is in this block
+        p = start, n = size_t(nl - start);
+        pos_ = size_t(nl - block_.data()) + 1;
+    } else {  // it continues in the next blocks (or ends the input without a '\n')
+        span_.assign(start, end);
         pos_ = block_.size();
+        while (ra_->take(block_)) {
+            pos_ = 0;
+            start = block_.data(), end = start + block_.size();
+            nl = static_cast<const char *>(memchr(start, '\n', size_t(end - start)));
+            if (nl) {
+                span_.append(start, nl);
+                pos_ = size_t(nl - start) + 1;
+                break;
+            }
+            span_.append(start, end);
+            pos_ = block_.size();
+        }
+        if (span_.empty() && pos_ >= block_.size() && !nl) return false;
+        p = span_.data(), n = span_.size();
+    }
+    if (n && p[n - 1] == '\r') --n;
+    return true;
+}
+
+bool Reader::find_header() {
+    for (;;) {
+        if (!have_line_ && !next_line(lp_, ln_)) return false;
+        have_line_ = false;
+        if (ln_ && (lp_[0] == '>' || lp_[0] == '@')) return true;
     }
 }
 
 bool Reader::next(std::string &header, std::vector<uint8_t> &seq) {
     header.clear();
     seq.clear();
-    for (;;) {  // find a header line
-        if (!have_line_ && !getline(line_)) return false;
-        have_line_ = false;
-        if (!line_.empty() && (line_[0] == '>' || line_[0] == '@')) break;
-    }
-    const bool fastq = line_[0] == '@';
-    header.assign(line_.begin() + 1, line_.end());
+    if (!find_header()) return false;
+    header.assign(lp_ + 1, lp_ + ln_);
+    return next_seq_body(seq, lp_[0] == '@');
+}
+
+bool Reader::next_seq(std::vector<uint8_t> &dst) {
+    if (!find_header()) return false;
+    return next_seq_body(dst, lp_[0] == '@');
+}
+
+// The sequence of the record whose header was just read, appended to dst.
+bool Reader::next_seq_body(std::vector<uint8_t> &dst, bool fastq) {
+    const char *p;
+    size_t n;
     if (fastq) {
-        if (!getline(line_)) return true;
-        seq.assign(line_.begin(), line_.end());
-        if (!getline(line_)) return true;  // '+'
-        if (!getline(line_)) return true;  // quality
+        if (!next_line(p, n)) return true;
+        dst.insert(dst.end(), reinterpret_cast<const uint8_t *>(p), reinterpret_cast<const uint8_t *>(p) + n);
+        if (!next_line(p, n)) return true;  // '+'
+        if (!next_line(p, n)) return true;  // quality
         return true;
     }
-    while (getline(line_)) {
-        if (!line_.empty() && line_[0] == '>') {
+    while (next_line(p, n)) {
+        if (n && p[0] == '>') {  // the next record's header: pushed back
+            lp_ = p, ln_ = n;
             have_line_ = true;
             break;
         }
-        seq.insert(seq.end(), line_.begin(), line_.end());
+        dst.insert(dst.end(), reinterpret_cast<const uint8_t *>(p), reinterpret_cast<const uint8_t *>(p) + n);
     }
     return true;
 }

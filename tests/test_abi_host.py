@@ -141,6 +141,47 @@ def test_fastx_multiline_fastq_gzip_and_empty_header(tmp_path):
     assert seqs == [b"ACGT", b"TT"]
 
 
+def _py_fasta(data: bytes):
+    """A plain FASTA parse: '>' lines start records, '\r\n' and '\n' end lines,
+    the remaining lines of a record are joined."""
+    names, seqs = [], []
+    for line in data.split(b"\n"):
+        if line.endswith(b"\r"):
+            line = line[:-1]
+        if line.startswith(b">"):
+            names.append(line[1:].split(b" ")[0].decode())
+            seqs.append(b"")
+        elif names:
+            seqs[-1] += line
+    return names, seqs
+
+
+def test_fastx_lines_across_read_blocks(tmp_path):
+    """Lines longer than the 4 MiB read-ahead block, a '\r\n' split across two
+    blocks and a last line with no '\n' read like a plain parse."""
+    rng = np.random.default_rng(5)
+    alpha = np.frombuffer(b"ACGTN", np.uint8)
+    parts = []
+    block = 4 << 20
+    for i, n in enumerate((10, 9_000_000, 70, 3_000_000, 1, 5_000_000)):
+        hdr = b">r%d desc\n" % i
+        body = alpha[rng.integers(0, 5, n)].tobytes()
+        parts += [hdr, body, b"\r\n"]
+    data = b"".join(parts)
+    # move a '\r\n' onto the boundary: the '\r' ends one block, the '\n' starts the next
+    k = data.rfind(b"\r\n", 0, block - 1)
+    assert k > 0
+    data = data[:k] + b"A" * (block - 1 - k) + data[k:]
+    assert data[block - 1:block + 1] == b"\r\n"
+    data = data + b">last\nACGTACGT"  # no final newline
+    p = tmp_path / "long.fa"
+    p.write_bytes(data)
+    names, seqs = _records(p)
+    want_n, want_s = _py_fasta(data)
+    assert names == want_n and [len(s) for s in seqs] == [len(s) for s in want_s]
+    assert seqs == want_s
+
+
 def _zstd_compress(data: bytes) -> bytes:
     import ctypes as C
     z = C.CDLL("libzstd.so.1")
