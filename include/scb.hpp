@@ -5,11 +5,15 @@
 // Display text.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <exception>
+#include <functional>
 #include <optional>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 
@@ -77,6 +81,57 @@ struct SuperCountingBloomConfig {
 
 using AddReport = scb_add_report;          // api.rs:66-73
 using QueryFastaReport = scb_query_report;  // api.rs:75-82
+using BuildReport = scb_build_report;       // fastx.rs:21-34 (build half)
+using Tuning = scb_tuning;                  // extension: tunables of the batched method
+using SampledRow = scb_sampled_row;         // sampled_spectrum.rs:225-253
+using SampledStats = scb_sampled_stats;     // sampled_spectrum.rs:113-146
+
+// extension: which device method runs a batch (results are the same with every method)
+enum class Method : int { Batched = SCB_METHOD_BATCHED, Plain = SCB_METHOD_PLAIN, Auto = SCB_METHOD_AUTO };
+
+// A batch of records in the layout every batched entry point takes: record i
+// is bases[offsets[i], offsets[i+1]).
+struct Records {
+    std::vector<uint8_t> bases;
+    std::vector<uint64_t> offsets{0};
+    void add(const std::string &seq) {
+        bases.insert(bases.end(), seq.begin(), seq.end());
+        offsets.push_back(bases.size());
+    }
+    size_t size() const { return offsets.size() - 1; }
+    uint64_t total_bases() const { return offsets.back(); }
+    std::string record(size_t i) const {
+        return std::string(bases.begin() + offsets[i], bases.begin() + offsets[i + 1]);
+    }
+};
+
+// estimate_sequence_abundances over a batch (api.rs:250-262 per record): the
+// estimate of the window starting at each position of the concatenated records.
+struct SequenceEstimates {
+    std::vector<uint64_t> values;
+    std::vector<uint8_t> valid;
+    QueryFastaReport report;
+    std::optional<uint64_t> at(uint64_t pos) const {
+        return valid[pos] ? std::optional<uint64_t>(values[pos]) : std::nullopt;
+    }
+};
+
+// Distinct-k-mer spectrum (fastx.rs:255-308, 392-410): rows (estimate, #keys).
+struct DistinctSpectrum {
+    ApproxSpectrum rows;
+    uint64_t queried = 0, distinct = 0;
+};
+
+// The HashMap<KmerKey, C> of estimate_packed_sequence (filter.rs:302-335).
+struct KmerEstimates {
+    std::unordered_map<uint64_t, uint64_t> map;
+    uint64_t queried = 0;
+};
+
+struct SampledSpectrum {  // sampled_spectrum.rs:150-253
+    std::vector<SampledRow> rows;
+    SampledStats stats;
+};
 
 struct SequenceAbundanceSummary {  // api.rs:84-92
     std::string name;
@@ -117,12 +172,36 @@ class SuperCountingBloomBuilder {
         check(scb_builder_add_sequences(h_, bases, offsets, n, &r));
         return r;
     }
+    AddReport add_sequences(const Records &r) { return add_sequences(r.bases.data(), r.offsets.data(), r.size()); }
+    // HBM-resident bases and offsets (scb_builder_add_sequences_device)
+    AddReport add_sequences_device(const void *d_bases, const void *d_offsets, size_t n, uint64_t total_bases) {
+        AddReport r{};
+        check(scb_builder_add_sequences_device(h_, static_cast<const uint8_t *>(d_bases),
+                                               static_cast<const uint64_t *>(d_offsets), n, total_bases, &r));
+        return r;
+    }
     AddReport add_fasta(const std::string &path) {
         AddReport r{};
         check(scb_builder_add_fasta(h_, path.c_str(), &r));
         return r;
     }
     inline SuperCountingBloomIndex into_index() &&;
+    void set_method(Method m) { check(scb_builder_set_method(h_, int(m))); }
+    void set_tuning(const Tuning &t) { check(scb_builder_set_tuning(h_, &t)); }
+    Tuning tuning() const {
+        Tuning t{};
+        scb_builder_tuning(h_, &t);
+        return t;
+    }
+    // The counter table in .scb payload order, counter_bits/8 bytes per counter.
+    std::vector<uint8_t> counter_bytes() const {
+        std::vector<uint8_t> v(table_bytes(config()));
+        check(scb_builder_counters(h_, v.data(), v.size()));
+        return v;
+    }
+    static size_t table_bytes(const SuperCountingBloomConfig &c) {
+        return (size_t(1) << c.counter_slots_exponent) * (size_t(c.counter_bits) / 8);
+    }
     uint64_t inserted_kmers() const { return scb_builder_inserted_kmers(h_); }
     uint64_t inserted_smers() const { return scb_builder_inserted_smers(h_); }
     void set_threads(size_t threads) { check(scb_builder_set_threads(h_, threads)); }
@@ -191,6 +270,70 @@ class SuperCountingBloomIndex {
         if (!std::isnan(s.median_abundance)) r.median_abundance = s.median_abundance;
         return r;
     }
+    // estimate_sequence_abundances of every record of a batch in one device pass
+    SequenceEstimates estimate_sequences(const Records &r) const {
+        SequenceEstimates e;
+        e.values.resize(r.total_bases() ? r.total_bases() : 1);
+        e.valid.resize(e.values.size());
+        e.report = QueryFastaReport{};
+        check(scb_index_estimate_sequences(h_, r.bases.data(), r.offsets.data(), r.size(), e.values.data(),
+                                           e.valid.data(), &e.report));
+        e.values.resize(r.total_bases()), e.valid.resize(r.total_bases());
+        return e;
+    }
+    // HBM-resident form: d_values holds counter_bits/8 bytes per position, d_valid_bits one bit
+    QueryFastaReport estimate_sequences_device(const void *d_bases, const void *d_offsets, size_t n,
+                                               uint64_t total_bases, void *d_values, uint32_t *d_valid_bits) const {
+        QueryFastaReport q{};
+        check(scb_index_estimate_sequences_device(h_, static_cast<const uint8_t *>(d_bases),
+                                                  static_cast<const uint64_t *>(d_offsets), n, total_bases,
+                                                  d_values, d_valid_bits, &q));
+        return q;
+    }
+    // estimate_packed_sequence (filter.rs:302-335) over the records' ACGT runs
+    KmerEstimates estimate_kmer_map(const Records &r) const {
+        size_t cap = r.total_bases() ? r.total_bases() : 1, n = 0;
+        std::vector<uint64_t> keys(cap), vals(cap);
+        KmerEstimates out;
+        check(scb_index_estimate_kmer_map(h_, r.bases.data(), r.offsets.data(), r.size(), keys.data(), vals.data(),
+                                          cap, &n, &out.queried));
+        out.map.reserve(n);
+        for (size_t i = 0; i < n; ++i) out.map.emplace(keys[i], vals[i]);
+        return out;
+    }
+    DistinctSpectrum spectrum(const Records &r) const {
+        return rows_of([&](uint64_t *v, uint64_t *c, size_t cap, size_t *n, uint64_t *q, uint64_t *d) {
+            return scb_index_spectrum(h_, r.bases.data(), r.offsets.data(), r.size(), v, c, cap, n, q, d);
+        });
+    }
+    // sample_inserted_kmers + query_sampled_kmers (sampled_spectrum.rs:150-221)
+    SampledSpectrum sampled_spectrum(const Records &r, uint64_t sample_size, uint64_t seed) const {
+        SampledSpectrum s{};
+        std::vector<SampledRow> rows(1 << 16);
+        size_t n = 0;
+        check(scb_index_sampled_spectrum(h_, r.bases.data(), r.offsets.data(), r.size(), sample_size, seed,
+                                         rows.data(), rows.size(), &n, &s.stats));
+        if (n > rows.size()) {
+            rows.resize(n);
+            check(scb_index_sampled_spectrum(h_, r.bases.data(), r.offsets.data(), r.size(), sample_size, seed,
+                                             rows.data(), rows.size(), &n, &s.stats));
+        }
+        rows.resize(n);
+        s.rows = std::move(rows);
+        return s;
+    }
+    void set_method(Method m) { check(scb_index_set_method(h_, int(m))); }
+    void set_tuning(const Tuning &t) { check(scb_index_set_tuning(h_, &t)); }
+    Tuning tuning() const {
+        Tuning t{};
+        scb_index_tuning(h_, &t);
+        return t;
+    }
+    std::vector<uint8_t> counter_bytes() const {
+        std::vector<uint8_t> v(SuperCountingBloomBuilder::table_bytes(config()));
+        check(scb_index_counters(h_, v.data(), v.size()));
+        return v;
+    }
     QueryFastaReport query_fasta(const std::string &path) const {
         QueryFastaReport r{};
         check(scb_index_query_fasta(h_, path.c_str(), &r));
@@ -220,10 +363,114 @@ class SuperCountingBloomIndex {
         check(scb_index_filter_only_count_hint(h_, nullptr, nullptr, 0, 255, &h));
         return h;
     }
+    // api.rs:345-368, 375-385 on a given counter histogram
+    FilterOnlySpectrumEstimate estimate_filter_only_spectrum_from_counter_histogram(const ApproxSpectrum &hist,
+                                                                                    uint64_t max_abundance,
+                                                                                    uint64_t hint) const {
+        std::vector<uint64_t> hv, hc, counts(max_abundance ? max_abundance : 1);
+        for (auto &r : hist) hv.push_back(r.first), hc.push_back(r.second);
+        check(scb_index_filter_only_spectrum(h_, hv.data(), hc.data(), hv.size(), max_abundance, hint,
+                                             counts.data()));
+        FilterOnlySpectrumEstimate e{"compound_poisson_counter_deconvolution", hint, {}};
+        for (uint64_t a = 1; a <= max_abundance; ++a) e.rows.emplace_back(a, counts[a - 1]);
+        return e;
+    }
+    uint64_t filter_only_count_hint_from_counter_histogram(const ApproxSpectrum &hist, uint64_t max_abundance) const {
+        std::vector<uint64_t> hv, hc;
+        for (auto &r : hist) hv.push_back(r.first), hc.push_back(r.second);
+        uint64_t h = 0;
+        check(scb_index_filter_only_count_hint(h_, hv.data(), hc.data(), hv.size(), max_abundance, &h));
+        return h;
+    }
     scb_index *handle() const { return h_; }
+
+    // Calls f(values, counts, cap, &n_rows, &queried, &distinct), growing the
+    // row buffers once when the first call reports more rows than fit.
+    template <class F>
+    static DistinctSpectrum rows_of(F &&f) {
+        std::vector<uint64_t> v(1 << 16), c(1 << 16);
+        size_t n = 0;
+        DistinctSpectrum s;
+        check(f(v.data(), c.data(), v.size(), &n, &s.queried, &s.distinct));
+        if (n > v.size()) {
+            v.resize(n), c.resize(n);
+            check(f(v.data(), c.data(), v.size(), &n, &s.queried, &s.distinct));
+        }
+        for (size_t i = 0; i < n; ++i) s.rows.emplace_back(v[i], c[i]);
+        return s;
+    }
 
    private:
     scb_index *h_ = nullptr;
+};
+
+// estimate_distinct_kmers over many batches (fastx.rs:255-308, 392-400): one
+// device key set across add() calls; the index must outlive the accumulator.
+class SpectrumAccumulator {
+   public:
+    explicit SpectrumAccumulator(const SuperCountingBloomIndex &idx) { check(scb_spectrum_acc_new(idx.handle(), &h_)); }
+    ~SpectrumAccumulator() {
+        if (h_) scb_spectrum_acc_free(h_);
+    }
+    SpectrumAccumulator(const SpectrumAccumulator &) = delete;
+    SpectrumAccumulator &operator=(const SpectrumAccumulator &) = delete;
+    void add(const Records &r) { check(scb_spectrum_acc_add(h_, r.bases.data(), r.offsets.data(), r.size())); }
+    void add_device(const void *d_bases, const void *d_offsets, size_t n, uint64_t total_bases) {
+        check(scb_spectrum_acc_add_device(h_, static_cast<const uint8_t *>(d_bases),
+                                          static_cast<const uint64_t *>(d_offsets), n, total_bases));
+    }
+    DistinctSpectrum rows() const {
+        return SuperCountingBloomIndex::rows_of(
+            [&](uint64_t *v, uint64_t *c, size_t cap, size_t *n, uint64_t *q, uint64_t *d) {
+                return scb_spectrum_acc_rows(h_, v, c, cap, n, q, d);
+            });
+    }
+
+   private:
+    scb_spectrum_acc *h_ = nullptr;
+};
+
+// extension: the serving loop over a frozen index (requests coalesced into
+// device launches by worker threads). The index must outlive the server; the
+// buffers of a request must stay valid until its ticket is waited for.
+using ServeConfig = scb_serve_config;
+using Served = scb_served;
+class Server {
+   public:
+    explicit Server(const SuperCountingBloomIndex &idx, const ServeConfig *cfg = nullptr) {
+        check(scb_server_new(idx.handle(), cfg, &h_));
+    }
+    ~Server() {
+        if (h_) scb_server_free(h_);
+    }
+    Server(const Server &) = delete;
+    Server &operator=(const Server &) = delete;
+    static ServeConfig default_config() {
+        ServeConfig c;
+        scb_serve_config_default(&c);
+        return c;
+    }
+    // values: counter_bits/8 bytes per position; valid_bits: one bit per position
+    uint64_t submit_abundances(const Records &r, void *values, uint32_t *valid_bits) {
+        uint64_t t = 0;
+        check(scb_server_submit_abundances(h_, r.bases.data(), r.offsets.data(), r.size(), values, valid_bits, &t));
+        return t;
+    }
+    uint64_t submit_spectrum(const Records &r, uint64_t *values, uint64_t *counts, size_t cap) {
+        uint64_t t = 0;
+        check(scb_server_submit_spectrum(h_, r.bases.data(), r.offsets.data(), r.size(), values, counts, cap, &t));
+        return t;
+    }
+    // The request's outcome; a failed request throws its own error.
+    Served wait(uint64_t ticket) {
+        Served s{};
+        check(scb_server_wait(h_, ticket, &s));
+        if (s.status != SCB_OK) throw SuperCountingBloomError(s.status, s.message);
+        return s;
+    }
+
+   private:
+    scb_server *h_ = nullptr;
 };
 
 inline SuperCountingBloomIndex SuperCountingBloomBuilder::into_index() && {
@@ -273,5 +520,88 @@ inline void write_spectrum(const ApproxSpectrum &rows, const char *path) {  // s
     check(scb_write_spectrum(v.data(), n.data(), rows.size(), path));
 }
 inline uint64_t xorshift_u64(uint64_t x) { return scb_xorshift_u64(x); }
+// canonical_kmer of the ACGT window seq[pos, pos+k)
+inline uint64_t canonical_kmer(const std::string &seq, size_t k, size_t pos) {
+    return scb_canonical_kmer(reinterpret_cast<const uint8_t *>(seq.data()), seq.size(), k, pos);
+}
+inline int device_count() { return scb_device_count(); }
+
+// summarize_abundances (api.rs:855-891) of per-window estimates
+inline SequenceAbundanceSummary summarize_abundances(const std::string &name, size_t sequence_len,
+                                                     const std::vector<std::optional<uint64_t>> &windows) {
+    std::vector<uint64_t> v(windows.size() ? windows.size() : 1);
+    std::vector<uint8_t> ok(v.size());
+    for (size_t i = 0; i < windows.size(); ++i) v[i] = windows[i].value_or(0), ok[i] = windows[i].has_value();
+    scb_summary s{};
+    check(scb_summarize_abundances(v.data(), ok.data(), windows.size(), &s));
+    SequenceAbundanceSummary r{name, sequence_len, size_t(s.total_windows), size_t(s.valid_kmers), std::nullopt,
+                               std::nullopt};
+    if (!std::isnan(s.mean_abundance)) r.mean_abundance = s.mean_abundance;
+    if (!std::isnan(s.median_abundance)) r.median_abundance = s.median_abundance;
+    return r;
+}
+
+// for_each_fastx_record (api.rs:893-927); an exception thrown by visit stops
+// the walk and is rethrown here.
+inline void for_each_fastx_record(const std::string &path,
+                                  const std::function<void(const std::string &, const std::string &)> &visit) {
+    struct Ctx {
+        const std::function<void(const std::string &, const std::string &)> *fn;
+        std::exception_ptr err;
+    } ctx{&visit, nullptr};
+    auto tramp = [](void *user, const char *name, const uint8_t *seq, size_t len) -> int {
+        auto *c = static_cast<Ctx *>(user);
+        try {
+            (*c->fn)(name, std::string(reinterpret_cast<const char *>(seq), len));
+            return 0;
+        } catch (...) {
+            c->err = std::current_exception();
+            return 1;
+        }
+    };
+    int rc = scb_for_each_fastx_record(path.c_str(), tramp, &ctx);
+    if (ctx.err) std::rethrow_exception(ctx.err);
+    check(rc);
+}
+
+// run() of bin/sampled_spectrum.rs:99-148: build from a FASTA/FASTQ file, then
+// sample and query its k-mers
+inline SampledSpectrum sampled_spectrum_from_fastx(const std::string &path, const SuperCountingBloomConfig &cfg,
+                                                   uint64_t sample_size, uint64_t seed) {
+    scb_config c = cfg.to_c();
+    SampledSpectrum s{};
+    // one row per distinct estimate among the sampled k-mers: at most sample_size
+    std::vector<SampledRow> rows(std::max<uint64_t>(1, std::min<uint64_t>(sample_size, uint64_t(1) << 22)));
+    size_t n = 0;
+    check(scb_sampled_spectrum_from_fastx(path.c_str(), &c, sample_size, seed, rows.data(), rows.size(), &n,
+                                          &s.stats));
+    if (n > rows.size())
+        throw SuperCountingBloomError(SCB_ERR_INTERNAL_STATE, "more than 2^22 distinct sampled estimates");
+    rows.resize(n);
+    s.rows = std::move(rows);
+    return s;
+}
+inline void write_sampled_spectrum(const std::vector<SampledRow> &rows, const char *path) {
+    check(scb_write_sampled_spectrum(rows.data(), rows.size(), path));
+}
+
+// The filter-only estimators on a histogram with no index (api.rs:933-1033, 375-385)
+inline std::vector<uint64_t> filter_only_spectrum_from_histogram(uint64_t n_hashes, const ApproxSpectrum &hist,
+                                                                 uint64_t max_abundance, uint64_t hint) {
+    std::vector<uint64_t> hv, hc, counts(max_abundance ? max_abundance : 1);
+    for (auto &r : hist) hv.push_back(r.first), hc.push_back(r.second);
+    check(scb_filter_only_spectrum_from_histogram(n_hashes, hv.data(), hc.data(), hv.size(), max_abundance, hint,
+                                                  counts.data()));
+    counts.resize(max_abundance);
+    return counts;
+}
+inline uint64_t filter_only_count_hint_from_histogram(uint64_t n_hashes, const ApproxSpectrum &hist,
+                                                      uint64_t max_abundance) {
+    std::vector<uint64_t> hv, hc;
+    for (auto &r : hist) hv.push_back(r.first), hc.push_back(r.second);
+    uint64_t h = 0;
+    check(scb_filter_only_count_hint_from_histogram(n_hashes, hv.data(), hc.data(), hv.size(), max_abundance, &h));
+    return h;
+}
 
 }  // namespace scb
