@@ -347,7 +347,7 @@ def serve_bench(scb, idx, genome, qps_steps, min_requests=2000, pool_size=160, p
     log-uniform (1-10,000 records of 31-20,000 bp; 70% genome substrings on a
     random strand, 30% random); 80% abundance and 20% spectrum requests. Once per
     rate, half way through, the worst-case request of 10,000 records of 20 kbp is
-    submitted too (its own latency is reported; it counts in the percentiles).
+    submitted in place of a pool request (its own latency is reported; it counts in the percentiles).
     The sweep stops after the first rate whose p99 passes p99_limit_ms. Latency
     is scheduled arrival -> completion (CLOCK_MONOTONIC), so submit lateness
     counts."""
@@ -385,39 +385,43 @@ def serve_bench(scb, idx, genome, qps_steps, min_requests=2000, pool_size=160, p
             n_sub = [0]
             stop = [False]
 
+            failed = []
+
             def collect():
-                while True:
-                    r = srv.wait_any(0.05)
-                    if r is None:
+                try:
+                    while True:
+                        r = srv.wait_any(0.05)
+                        if r is None:
+                            with lock:
+                                if stop[0] and len(lat) + refused[0] >= n_sub[0]:
+                                    return
+                            continue
                         with lock:
-                            if stop[0] and len(lat) + refused[0] >= n_sub[0]:
-                                return
-                        continue
-                    with lock:
-                        t_arr, pi, kind, owns = arrivals.pop(r.ticket)
-                        lat.append((r.done_ns - t_arr) / 1e6)
-                        if pi == wi and kind == 0:
-                            worst_ms[0] = lat[-1]
-                        kmers_done[0] += r.report.valid_kmers if kind == 0 else r.queried
-                        if owns:
-                            busy[pi] = False
+                            t_arr, pi, kind, owns = arrivals.pop(r.ticket)
+                            lat.append((r.done_ns - t_arr) / 1e6)
+                            if pi == wi and kind == 0:
+                                worst_ms[0] = lat[-1]
+                            kmers_done[0] += r.report.valid_kmers if kind == 0 else r.queried
+                            if owns:
+                                busy[pi] = False
+                except BaseException as ex:  # re-raised by the submitting thread
+                    failed.append(ex)
 
             th = threading.Thread(target=collect)
             th.start()
-            seconds = max(1.0, min_requests / qps)
+            # at least one second of arrivals and at least min_requests requests
             t0 = time.monotonic_ns()
             t_next = t0
-            end = t0 + int(seconds * 1e9)
-            worst_at = t0 + int(seconds * 0.5e9)
+            end = t0 + int(1e9)
             worst_sent = False
-            while True:
+            for i in range(1 << 40):
                 t_next += int(rng.exponential(1e9 / qps))
-                if t_next >= end:
+                if i >= min_requests and t_next >= end:
                     break
                 now = time.monotonic_ns()
                 if t_next > now:
                     time.sleep((t_next - now) / 1e9)
-                if not worst_sent and t_next >= worst_at:
+                if not worst_sent and i >= min_requests // 2:
                     pi, kind, worst_sent = wi, 0, True
                 else:
                     pi = int(rng.integers(0, len(pool) - 1))
@@ -446,6 +450,8 @@ def serve_bench(scb, idx, genome, qps_steps, min_requests=2000, pool_size=160, p
             with lock:
                 stop[0] = True
             th.join()
+            if failed:
+                raise RuntimeError(f"serving collector failed at {qps} requests/s") from failed[0]
             wall = (time.monotonic_ns() - t0) / 1e9
             a = np.array(lat)
             steps.append({"qps": qps, "requests": int(a.size), "refused": refused[0],
@@ -574,8 +580,11 @@ def main():
     bits_b = out_bits.to_array(np.uint32, (qN + 31) // 32)
     served = None
     if not args.no_serve:
-        served = serve_bench(scb, idx_b, synth.genome(5_000_000, 1),
-                             [float(q) for q in args.serve_qps.split(",")], args.serve_min_requests)
+        try:
+            served = serve_bench(scb, idx_b, synth.genome(5_000_000, 1),
+                                 [float(q) for q in args.serve_qps.split(",")], args.serve_min_requests)
+        except RuntimeError as e:  # reported in the line instead of numbers from a broken sweep
+            served = {"error": f"{e}: {e.__cause__!r}", "steps": []}
     spectrum = None
     if not args.no_spectrum:
         spectrum = spectrum_bench(scb, idx_b, dev)
@@ -723,8 +732,8 @@ def main():
         "host_buffers": host_api,
         "files": io,
         "spectrum": spectrum,
-        "served_p50_ms": served["steps"][0]["p50_ms"] if served else None,
-        "served_p99_ms": served["steps"][0]["p99_ms"] if served else None,
+        "served_p50_ms": served["steps"][0]["p50_ms"] if served and served["steps"] else None,
+        "served_p99_ms": served["steps"][0]["p99_ms"] if served and served["steps"] else None,
         "cpu_baseline": cpu,
         "host": platform.node(),
     }

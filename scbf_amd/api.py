@@ -7,6 +7,7 @@ import math
 import os
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence, Tuple
+import threading
 
 import numpy as np
 
@@ -765,6 +766,10 @@ class SuperCountingBloomServer:
         _check(lib().scb_server_new(index._h, C.byref(c), C.byref(self._h)))
         self._dtype = {8: np.uint8, 16: np.uint16, 32: np.uint32}[index.config().counter_bits]
         self._live = {}
+        # Held from a submit call until its ticket is registered in _live, and taken
+        # before a finished ticket is looked up: a request can finish (and be returned
+        # by wait_any in another thread) before its submit call has returned.
+        self._reg = threading.Lock()
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
@@ -797,10 +802,11 @@ class SuperCountingBloomServer:
                 or valid_bits.size < (nb + 31) // 32:
             raise ValueError("output buffers are too small or of the wrong dtype")
         t = C.c_uint64()
-        _check(lib().scb_server_submit_abundances(self._h, bases.ctypes.data, offsets.ctypes.data,
-                                                  offsets.size - 1, values.ctypes.data, valid_bits.ctypes.data,
-                                                  C.byref(t)))
-        self._live[t.value] = (REQ_ABUNDANCES, bases, offsets, values, valid_bits, nb)
+        with self._reg:
+            _check(lib().scb_server_submit_abundances(self._h, bases.ctypes.data, offsets.ctypes.data,
+                                                      offsets.size - 1, values.ctypes.data, valid_bits.ctypes.data,
+                                                      C.byref(t)))
+            self._live[t.value] = (REQ_ABUNDANCES, bases, offsets, values, valid_bits, nb)
         return t.value
 
     def submit_spectrum(self, bases: np.ndarray, offsets: np.ndarray, cap: int = 1 << 17) -> int:
@@ -809,14 +815,19 @@ class SuperCountingBloomServer:
         rv = np.zeros(cap, dtype=np.uint64)
         rc = np.zeros(cap, dtype=np.uint64)
         t = C.c_uint64()
-        _check(lib().scb_server_submit_spectrum(self._h, bases.ctypes.data, offsets.ctypes.data,
-                                                offsets.size - 1, rv.ctypes.data, rc.ctypes.data, cap,
-                                                C.byref(t)))
-        self._live[t.value] = (REQ_SPECTRUM, bases, offsets, rv, rc, cap)
+        with self._reg:
+            _check(lib().scb_server_submit_spectrum(self._h, bases.ctypes.data, offsets.ctypes.data,
+                                                    offsets.size - 1, rv.ctypes.data, rc.ctypes.data, cap,
+                                                    C.byref(t)))
+            self._live[t.value] = (REQ_SPECTRUM, bases, offsets, rv, rc, cap)
         return t.value
 
+    def _take(self, ticket):
+        with self._reg:
+            return self._live.pop(ticket, None)
+
     def _result(self, ticket, r: ScbServed) -> Served:
-        kind, _b, _o, x, y, z = self._live.pop(ticket)
+        kind, _b, _o, x, y, z = self._take(ticket)
         out = Served(ticket, kind, submit_ns=r.submit_ns, start_ns=r.start_ns, done_ns=r.done_ns,
                      batch_requests=r.batch_requests)
         if kind == REQ_ABUNDANCES:
@@ -830,8 +841,8 @@ class SuperCountingBloomServer:
     def wait(self, ticket: int) -> Served:
         r = ScbServed()
         rc = lib().scb_server_wait(self._h, ticket, C.byref(r))
-        if rc != 0 and r.done_ns and ticket in self._live:  # a failed request was taken
-            self._live.pop(ticket)
+        if rc != 0 and r.done_ns:  # a failed request was taken
+            self._take(ticket)
         _check(rc)
         return self._result(ticket, r)
 
@@ -839,8 +850,8 @@ class SuperCountingBloomServer:
         r = ScbServed()
         t = C.c_uint64()
         rc = lib().scb_server_wait_any(self._h, int(timeout_s * 1e6), C.byref(t), C.byref(r))
-        if rc != 0 and t.value and t.value in self._live:
-            self._live.pop(t.value)
+        if rc != 0 and t.value:
+            self._take(t.value)
         _check(rc)
         return self._result(t.value, r) if t.value else None
 

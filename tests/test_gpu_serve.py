@@ -218,3 +218,40 @@ def test_wait_and_wait_any_from_several_threads(gpu):
         assert all(t in delivered for t in refused)
         for t, r in zip(tickets, reqs):
             _check_abundances(idx, r, delivered[t])
+
+
+def test_wait_any_while_another_thread_submits(gpu):
+    """One thread submits small requests while another takes finished tickets with
+    wait_any: a request can finish before its submit call has returned, and its
+    result must still be delivered once, equal to the direct call."""
+    import threading
+    scb = gpu
+    g, idx, _ = _index(scb)
+    reqs = _requests(g, 300, 777, max_records=3, max_len=300)
+    submitted, delivered, errors = {}, {}, []
+    done_submitting = threading.Event()
+
+    def collector():
+        try:
+            while True:
+                r = srv.wait_any(0.05)
+                if r is not None:
+                    assert r.ticket not in delivered, f"ticket {r.ticket} delivered twice"
+                    delivered[r.ticket] = r
+                elif done_submitting.is_set() and len(delivered) == len(submitted):
+                    return
+        except Exception as ex:  # reported by the main thread
+            errors.append(ex)
+
+    with scb.SuperCountingBloomServer(idx, scb.ServeConfig(workers=4)) as srv:
+        th = threading.Thread(target=collector)
+        th.start()
+        for i, (b, o) in enumerate(reqs):
+            submitted[srv.submit_abundances(b, o)] = i
+        done_submitting.set()
+        th.join(timeout=120)
+        assert not th.is_alive(), "the collecting thread did not finish"
+    assert not errors, errors
+    assert sorted(delivered) == sorted(submitted)
+    for t, i in submitted.items():
+        _check_abundances(idx, reqs[i], delivered[t])
