@@ -142,63 +142,65 @@ __device__ __forceinline__ void for_each_address(uint64_t smer, uint32_t nh, uin
     f(xs64_low15(h) & amask);
 }
 
-// for_each_address with the chain unrolled for n_hashes up to four (a
-// uniform branch on nh; longer chains take the loop).
+// for_each_address with the chain unrolled for NH = n_hashes (NH = 0: any
+// n_hashes, read at run time).
 template <int NH, class F>
-__device__ __forceinline__ void for_each_address_n(uint64_t h, uint32_t amask, F &&f) {
+__device__ __forceinline__ void for_each_address_n(uint64_t h, uint32_t nh, uint32_t amask, F &&f) {
+    if constexpr (NH == 0) {
+        for_each_address(h, nh, amask, f);
+    } else {
 #pragma unroll
-    for (int t = 0; t + 1 < NH; ++t) {
-        h = xs64(h);
-        f(uint32_t(h) & amask);
+        for (int t = 0; t + 1 < NH; ++t) {
+            h = xs64(h);
+            f(uint32_t(h) & amask);
+        }
+        f(xs64_low15(h) & amask);
     }
-    f(xs64_low15(h) & amask);
 }
+
+// n_hashes as a compile-time constant for the s-mer loops: with_nh(nh, f)
+// calls f(NhTag<NH>{}) with NH = nh for 1..4 and NH = 0 (run-time count)
+// otherwise. The uniform switch runs once per record, so the per-s-mer code
+// holds no scalar compares and branches on n_hashes.
+template <int N>
+struct NhTag {
+    static constexpr int value = N;
+};
 template <class F>
-__device__ __forceinline__ void for_each_address_any(uint64_t smer, uint32_t nh, uint32_t amask, F &&f) {
+__device__ __forceinline__ void with_nh(uint32_t nh, F &&f) {
     switch (nh) {
-        case 1: for_each_address_n<1>(smer, amask, f); break;
-        case 2: for_each_address_n<2>(smer, amask, f); break;
-        case 3: for_each_address_n<3>(smer, amask, f); break;
-        case 4: for_each_address_n<4>(smer, amask, f); break;
-        default: for_each_address(smer, nh, amask, f); break;
+        case 4: f(NhTag<4>{}); break;
+        case 3: f(NhTag<3>{}); break;
+        case 2: f(NhTag<2>{}); break;
+        case 1: f(NhTag<1>{}); break;
+        default: f(NhTag<0>{}); break;
     }
 }
 
 // Minimum of the counters base[a] over the addresses of one canonical s-mer
-// (filter.rs:478-489), with all NH loads issued before the first is used.
+// (filter.rs:478-489), with all NH loads issued before the first is used
+// (NH = 0: any n_hashes, one load at a time); amask < 2^15.
 template <int NH, class T>
-__device__ __forceinline__ uint32_t min_at_nh(uint64_t h, uint32_t amask, const T *base) {
-    uint32_t v[NH];
+__device__ __forceinline__ uint32_t min_at_nh(uint64_t h, uint32_t nh, uint32_t amask, const T *base, uint32_t kmax) {
+    if constexpr (NH == 0) {
+        uint32_t est = kmax;
+        for_each_address(h, nh, amask, [&](uint32_t ad) {
+            const uint32_t v = base[ad];
+            est = v < est ? v : est;
+        });
+        return est;
+    } else {
+        uint32_t v[NH];
 #pragma unroll
-    for (int t = 0; t + 1 < NH; ++t) {
-        h = xs64(h);
-        v[t] = base[uint32_t(h) & amask];
-    }
-    v[NH - 1] = base[xs64_low15(h) & amask];
-    uint32_t m = v[0];
-#pragma unroll
-    for (int t = 1; t < NH; ++t) m = v[t] < m ? v[t] : m;
-    return m;
-}
-
-// The same for any n_hashes (amask < 2^15); chains of up to four hashes are
-// unrolled so that one load latency is paid per s-mer instead of one per hash.
-template <class T>
-__device__ __forceinline__ uint32_t min_at_addresses(uint64_t smer, uint32_t nh, uint32_t amask, const T *base,
-                                                     uint32_t kmax) {
-    switch (nh) {
-        case 1: return min_at_nh<1>(smer, amask, base);
-        case 2: return min_at_nh<2>(smer, amask, base);
-        case 3: return min_at_nh<3>(smer, amask, base);
-        case 4: return min_at_nh<4>(smer, amask, base);
-        default: {
-            uint32_t est = kmax;
-            for_each_address(smer, nh, amask, [&](uint32_t ad) {
-                const uint32_t v = base[ad];
-                est = v < est ? v : est;
-            });
-            return est;
+        for (int t = 0; t + 1 < NH; ++t) {
+            h = xs64(h);
+            v[t] = base[uint32_t(h) & amask];
         }
+        v[NH - 1] = base[xs64_low15(h) & amask];
+        uint32_t m = v[0];
+#pragma unroll
+        for (int t = 1; t < NH; ++t) m = v[t] < m ? v[t] : m;
+        return m;
     }
 }
 
@@ -1730,25 +1732,28 @@ __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(
                 const uint32_t mult = dedup ? (s_mult[ri >> 1] >> (16 * (ri & 1))) & 0xFFFFu : 1u;
                 uint32_t A0, A1, A2, A3;
                 align_words(W, j, A0, A1, A2, A3);
-                if (lshift > 0) {  // the record's block lies in one half of the bin
-                    const uint32_t wbase = lbase & (half_words - 1), inc = mult << (16 * (lbase >> half_log));
-                    for_each_smer(
-                        A0, A1, A2, A3, nk + g.k - 1, g.s,
-                        [&](uint32_t, uint64_t smer) {
-                            for_each_address_any(smer, g.nh, amask,
-                                                 [&](uint32_t ad) { atomicAdd(&s_cnt[wbase + ad], inc); });
-                        },
-                        one_word);
-                } else {  // one block per bin: its slots span both halves
-                    for_each_smer(
-                        A0, A1, A2, A3, nk + g.k - 1, g.s,
-                        [&](uint32_t, uint64_t smer) {
-                            for_each_address_any(smer, g.nh, amask, [&](uint32_t ad) {
-                                atomicAdd(&s_cnt[ad & (half_words - 1)], mult << (16 * (ad >> half_log)));
-                            });
-                        },
-                        one_word);
-                }
+                with_nh(g.nh, [&](auto nht) {
+                    constexpr int NH = decltype(nht)::value;
+                    if (lshift > 0) {  // the record's block lies in one half of the bin
+                        const uint32_t wbase = lbase & (half_words - 1), inc = mult << (16 * (lbase >> half_log));
+                        for_each_smer(
+                            A0, A1, A2, A3, nk + g.k - 1, g.s,
+                            [&](uint32_t, uint64_t smer) {
+                                for_each_address_n<NH>(smer, g.nh, amask,
+                                                       [&](uint32_t ad) { atomicAdd(&s_cnt[wbase + ad], inc); });
+                            },
+                            one_word);
+                    } else {  // one block per bin: its slots span both halves
+                        for_each_smer(
+                            A0, A1, A2, A3, nk + g.k - 1, g.s,
+                            [&](uint32_t, uint64_t smer) {
+                                for_each_address_n<NH>(smer, g.nh, amask, [&](uint32_t ad) {
+                                    atomicAdd(&s_cnt[ad & (half_words - 1)], mult << (16 * (ad >> half_log)));
+                                });
+                            },
+                            one_word);
+                    }
+                });
             });
             __syncthreads();
             // merge this batch's counts: 32-bit table words of the bin
@@ -1904,22 +1909,25 @@ __global__ __launch_bounds__(kQueryThreads) void k_bin_query(BinArgs a, DevGeo g
             // per-record sums: at most 32 k-mers, so 8/16-bit estimates sum in 32 bits
             uint32_t rec_pos = 0, rec_sum = 0;
             uint64_t rec_sum64 = 0;
-            for_each_smer(
-                A0, A1, A2, A3, nk + g.k - 1, g.s,
-                [&](uint32_t j, uint64_t smer) {
-                    const uint32_t est = min_at_addresses(smer, g.nh, amask, lt + lbase, Ctr<BITS>::kMax);
-                    const uint32_t ab = EXACT ? wm.push_exact(est) : wm.push(est, g.win);
-                    if (j + 1 >= g.win) {
-                        const uint32_t q = j + 1 - g.win;
-                        slot[q & (kV - 1)] = T(ab);
-                        if ((q & (kV - 1)) == kV - 1 && out)  // a full slot: kV estimates in one store
-                            store_run(reinterpret_cast<uint8_t *>(out + j0 + q - (kV - 1)), s_slot[tid], 16);
-                        rec_pos += ab > 0;
-                        if (BITS < 32) rec_sum += ab;
-                        else rec_sum64 += ab;
-                    }
-                },
-                one_word);
+            with_nh(g.nh, [&](auto nht) {
+                constexpr int NH = decltype(nht)::value;
+                for_each_smer(
+                    A0, A1, A2, A3, nk + g.k - 1, g.s,
+                    [&](uint32_t j, uint64_t smer) {
+                        const uint32_t est = min_at_nh<NH>(smer, g.nh, amask, lt + lbase, Ctr<BITS>::kMax);
+                        const uint32_t ab = EXACT ? wm.push_exact(est) : wm.push(est, g.win);
+                        if (j + 1 >= g.win) {
+                            const uint32_t q = j + 1 - g.win;
+                            slot[q & (kV - 1)] = T(ab);
+                            if ((q & (kV - 1)) == kV - 1 && out)  // a full slot: kV estimates in one store
+                                store_run(reinterpret_cast<uint8_t *>(out + j0 + q - (kV - 1)), s_slot[tid], 16);
+                            rec_pos += ab > 0;
+                            if (BITS < 32) rec_sum += ab;
+                            else rec_sum64 += ab;
+                        }
+                    },
+                    one_word);
+            });
             // the estimates left in the slot, in two or three stores
             const uint32_t rest = nk & (kV - 1);
             if (rest && out)
@@ -2047,33 +2055,38 @@ __global__ __launch_bounds__(kSpectrumThreads) void k_bin_spectrum(SpecArgs sa, 
             uint32_t A0, A1, A2, A3;
             align_words(W, rec & 0xFFFFFFFFull, A0, A1, A2, A3);
             WinMin<WINC> wm;
-            // the canonical k-mer key rolls along with the s-mers
-            for_each_smer_kmer(
-                A0, A1, A2, A3, nk + g.k - 1, g.s, g.k,
-                [&](uint32_t j, uint64_t smer, uint64_t key) {
-                    uint32_t est = Ctr<BITS>::kMax;
-                    if (g.bse <= 15) {
-                        est = min_at_addresses(smer, g.nh, amask, tbase + lbase, Ctr<BITS>::kMax);
-                    } else {
-                        uint64_t h = xs64(smer);
-                        for (uint32_t t = 0; t < g.nh; ++t) {
-                            uint32_t v = tbase[lbase + (uint32_t(h) & amask)];
-                            est = v < est ? v : est;
-                            if (t + 1 < g.nh) h = xs64(h);
+            // blocks of at most 2^15 slots use the unrolled chains; larger blocks
+            // (addresses above 15 bits) the run-time loop below
+            with_nh(g.bse <= 15 ? g.nh : 0u, [&](auto nht) {
+                constexpr int NH = decltype(nht)::value;
+                // the canonical k-mer key rolls along with the s-mers
+                for_each_smer_kmer(
+                    A0, A1, A2, A3, nk + g.k - 1, g.s, g.k,
+                    [&](uint32_t j, uint64_t smer, uint64_t key) {
+                        uint32_t est = Ctr<BITS>::kMax;
+                        if (NH > 0 || g.bse <= 15) {
+                            est = min_at_nh<NH>(smer, g.nh, amask, tbase + lbase, Ctr<BITS>::kMax);
+                        } else {
+                            uint64_t h = xs64(smer);
+                            for (uint32_t t = 0; t < g.nh; ++t) {
+                                uint32_t v = tbase[lbase + (uint32_t(h) & amask)];
+                                est = v < est ? v : est;
+                                if (t + 1 < g.nh) h = xs64(h);
+                            }
                         }
-                    }
-                    uint32_t ab = wm.push(est, g.win);
-                    if (j + 1 >= g.win) {
-                        bool fresh = global ? set_insert(sa.gset, sa.glog2, key) : set_insert(s_set, 13, key);
-                        ++queried;
-                        if (fresh) {
-                            ++distinct;
-                            if (sa.emit_keys) emit_pair(sa, key, ab);
-                            else spec_hist_add(sa.h, s_hist, ab);
+                        uint32_t ab = wm.push(est, g.win);
+                        if (j + 1 >= g.win) {
+                            bool fresh = global ? set_insert(sa.gset, sa.glog2, key) : set_insert(s_set, 13, key);
+                            ++queried;
+                            if (fresh) {
+                                ++distinct;
+                                if (sa.emit_keys) emit_pair(sa, key, ab);
+                                else spec_hist_add(sa.h, s_hist, ab);
+                            }
                         }
-                    }
-                },
-                smer_tail_one_word(g));
+                    },
+                    smer_tail_one_word(g));
+            });
         });
         __syncthreads();
     }
