@@ -1078,12 +1078,16 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_sums(const uint32_t *__re
     if (tid == 0) blk[2 * blockIdx.x] = ta, blk[2 * blockIdx.x + 1] = tb;
 }
 
+// A work item: x = bin, y/z = its first and one-past-last record in the
+// sorted array, w = the bin's k-mers (bit 31 set when the bin is split over
+// several items), so that a bin kernel starts an item with one 16-byte load.
 __global__ __launch_bounds__(kPlanThreads) void k_plan_write(const uint32_t *__restrict__ bin_count, uint64_t nbins,
                                                              uint32_t per_item, uint32_t per_shift,
                                                              const uint64_t *__restrict__ blk,
                                                              uint32_t *__restrict__ bin_off,
                                                              uint32_t *__restrict__ cursor,
-                                                             uint64_t *__restrict__ work,
+                                                             uint4 *__restrict__ work,
+                                                             const uint32_t *__restrict__ bin_kmers,
                                                              uint64_t *__restrict__ stats) {
     __shared__ uint64_t s_a[kPlanThreads / 64], s_b[kPlanThreads / 64];
     __shared__ uint64_t s_base[2];
@@ -1103,7 +1107,11 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan_write(const uint32_t *__r
     if (bin < nbins) {
         bin_off[bin] = uint32_t(ro);
         cursor[bin] = uint32_t(ro);
-        for (uint32_t p = 0; p < parts; ++p) work[io + p] = (bin << 32) | p;
+        const uint32_t info = (bin_kmers ? bin_kmers[bin] & 0x7FFFFFFFu : 0u) | (parts > 1 ? 0x80000000u : 0u);
+        for (uint32_t p = 0; p < parts; ++p) {
+            const uint32_t r0 = uint32_t(ro) + p * per_item;
+            work[io + p] = make_uint4(uint32_t(bin), r0, min(uint32_t(ro) + c, r0 + per_item), info);
+        }
     }
     if (blockIdx.x == gridDim.x - 1 && tid == kPlanThreads - 1) {
         stats[kStNWork] = s_base[1] + tb;
@@ -1475,14 +1483,20 @@ __device__ __forceinline__ void order_by_hi(const uint32_t (&hi)[R], uint32_t n,
     }
 }
 
+// The high words of records [rb, rb + n) for order_by_hi (record r * nthr + tid
+// in hi[r]; the clamped index keeps the load in bounds).
+template <int R>
+__device__ __forceinline__ void load_hi(const uint64_t *sorted, uint32_t rb, uint32_t n, uint32_t (&hi)[R],
+                                        uint32_t tid, uint32_t nthr) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) hi[r] = n ? uint32_t(sorted[rb + min(r * nthr + tid, n - 1)] >> 32) : 0u;
+}
 template <int R>
 __device__ __forceinline__ void order_by_length(const uint64_t *sorted, uint32_t rb, uint32_t n, uint16_t *s_perm,
                                                 uint32_t *s_cls, uint32_t tid, uint32_t nthr,
                                                 uint32_t *s_smers = nullptr, uint32_t k_minus_s = 0) {
     uint32_t hi[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r)  // clamped index keeps the load in bounds
-        hi[r] = n ? uint32_t(sorted[rb + min(r * nthr + tid, n - 1)] >> 32) : 0u;
+    load_hi<R>(sorted, rb, n, hi, tid, nthr);
     order_by_hi<R>(hi, n, s_perm, s_cls, tid, nthr, s_smers, k_minus_s);
 }
 
@@ -1545,7 +1559,7 @@ __device__ __forceinline__ void walk_records(const uint64_t *sorted, const uint3
 struct BinArgs {
     const uint64_t *sorted;
     const uint32_t *bin_off, *bin_count;
-    const uint64_t *work;
+    const uint4 *work;          // work items (k_plan_write)
     const uint32_t *bin_kmers;  // k-mers per bin (insert: selects the items counted by distinct record)
     uint64_t *stats;
     const uint32_t *pk;
@@ -1691,20 +1705,17 @@ __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(
         __syncthreads();
         uint64_t item = s_item;
         if (item >= nwork) break;
-        uint64_t wk = a.work[item];
-        uint64_t bin = wk >> 32;
-        uint32_t part = uint32_t(wk);
-        uint32_t cnt = a.bin_count[bin];
-        uint32_t rb = a.bin_off[bin] + part * a.per_item;
-        uint32_t re = min(a.bin_off[bin] + cnt, rb + a.per_item);
+        const uint4 wk = a.work[item];
+        const uint64_t bin = wk.x;
+        const uint32_t rb = wk.y, re = wk.z;
         const uint32_t n = re - rb;
-        const bool multi = cnt > a.per_item;
+        const bool multi = wk.w >> 31;
         // A bin that is one work item and makes at most 65535 increments (its
         // k-mers plus k - s per record, times n_hashes) is counted by distinct
         // record: each leader adds its multiplicity c at every increment, which
         // equals c single increments (and stays within one 16-bit pass).
         const bool dedup = !multi && n > 1 && a.bin_kmers &&
-                           (uint64_t(a.bin_kmers[bin]) + uint64_t(g.k - g.s) * n) * g.nh <= 65535u;
+                           (uint64_t(wk.w & 0x7FFFFFFFu) + uint64_t(g.k - g.s) * n) * g.nh <= 65535u;
         uint32_t nwalk = n;
         bool one = true;
         if (dedup) {
@@ -1888,15 +1899,16 @@ __global__ __launch_bounds__(kQueryThreads) void k_bin_query(BinArgs a, DevGeo g
         __syncthreads();
         uint64_t item = s_item;
         if (item >= nwork) break;
-        uint64_t wk = a.work[item];
-        uint64_t bin = wk >> 32;
-        uint32_t part = uint32_t(wk);
-        uint32_t cnt = a.bin_count[bin];
-        uint32_t rb = a.bin_off[bin] + part * a.per_item;
-        uint32_t re = min(a.bin_off[bin] + cnt, rb + a.per_item);
+        const uint4 wk = a.work[item];
+        const uint64_t bin = wk.x;
+        const uint32_t rb = wk.y, re = wk.z;
         const uint32_t *tw = reinterpret_cast<const uint32_t *>(a.table) + bin * uint64_t(nwords);
+        // the records' high words load with the bin's table words (one round trip)
+        constexpr int R = kQueryItemRecords / kQueryThreads;
+        uint32_t hi[R];
+        load_hi<R>(a.sorted, rb, re - rb, hi, tid, nthr);
         stage_to_lds<8>(s_tab, tw, nwords, tid, nthr);
-        order_by_length<kQueryItemRecords / kQueryThreads>(a.sorted, rb, re - rb, s_perm, s_cls, tid, nthr);  // (synchronizes)
+        order_by_hi<R>(hi, re - rb, s_perm, s_cls, tid, nthr);  // (synchronizes)
         walk_records(a.sorted, a.pk, g.k, rb, re - rb, s_perm, tid, nthr, [&](uint64_t rec, const uint32_t (&W)[5]) {
             const uint64_t j0 = rec & 0xFFFFFFFFull;
             const uint32_t nk = uint32_t((rec >> 32) & 31) + 1;
@@ -2035,20 +2047,20 @@ __global__ __launch_bounds__(kSpectrumThreads) void k_bin_spectrum(SpecArgs sa, 
         __syncthreads();
         uint64_t item = s_item;
         if (item >= nwork) break;
-        uint64_t wk = a.work[item];
-        uint64_t bin = wk >> 32;
-        uint32_t part = uint32_t(wk);
-        uint32_t cnt = a.bin_count[bin];
-        uint32_t rb = a.bin_off[bin] + part * a.per_item;
-        uint32_t re = min(a.bin_off[bin] + cnt, rb + a.per_item);
-        bool global = cnt > a.per_item || sa.bin_kmers[bin] > kSetFill;
+        const uint4 wk = a.work[item];
+        const uint64_t bin = wk.x;
+        const uint32_t rb = wk.y, re = wk.z;
+        const bool global = (wk.w >> 31) || (wk.w & 0x7FFFFFFFu) > kSetFill;
         const uint32_t *tw = reinterpret_cast<const uint32_t *>(a.table) + bin * uint64_t(bin_slots * (BITS / 8) / 4);
+        constexpr int R = kMaxItemRecords / kSpectrumThreads;
+        uint32_t hi[R];
+        load_hi<R>(a.sorted, rb, re - rb, hi, tid, nthr);
         stage_to_lds<8>(s_tab, tw, nwords, tid, nthr);
         if (!global)
             for (uint32_t i = tid; i < kSetCap; i += nthr) s_set[i] = kEmptyKey;
         __syncthreads();
         const T *tbase = sa.stage ? lt : reinterpret_cast<const T *>(a.table) + (bin << a.bin_log);
-        order_by_length<kMaxItemRecords / kSpectrumThreads>(a.sorted, rb, re - rb, s_perm, s_cls, tid, nthr);  // (synchronizes)
+        order_by_hi<R>(hi, re - rb, s_perm, s_cls, tid, nthr);  // (synchronizes)
         walk_records(a.sorted, a.pk, g.k, rb, re - rb, s_perm, tid, nthr, [&](uint64_t rec, const uint32_t (&W)[5]) {
             const uint32_t nk = uint32_t((rec >> 32) & 31) + 1;
             const uint32_t lbase = uint32_t((rec >> 37) & ((1u << lshift) - 1)) << g.bse;
@@ -2417,7 +2429,8 @@ class Workspace {
    public:
     hipStream_t stream = nullptr;
     DevBuf<uint32_t> pk, valid;
-    DevBuf<uint64_t> rec, sorted, work, stats, plan_blk;
+    DevBuf<uint64_t> rec, sorted, stats, plan_blk;
+    DevBuf<uint4> work;
     DevBuf<uint32_t> bin_count, bin_off, cursor, ccursor, coarse_hist, seg_count, tile_r0;
     uint64_t *final_sorted = nullptr;  // records grouped by bin after the partition passes
     DevBuf<uint8_t> h_bases_dev;   // staging for host batches
@@ -2458,8 +2471,8 @@ class Workspace {
         for (auto *b : {&pk, &valid, &bin_count, &bin_off, &cursor, &ccursor, &coarse_hist, &seg_count, &tile_r0,
                         &out_bits, &hist_big_keys, &spec_bin_kmers})
             b->owned = false;
-        for (auto *b : {&rec, &sorted, &work, &stats, &plan_blk, &h_offsets_dev}) b->owned = false;
-        h_bases_dev.owned = out_dev.owned = false;
+        for (auto *b : {&rec, &sorted, &stats, &plan_blk, &h_offsets_dev}) b->owned = false;
+        h_bases_dev.owned = out_dev.owned = work.owned = false;
         hist_dense.owned = hist_big_counts.owned = spec_gset.owned = false;
         emit_keys.owned = emit_vals.owned = false;
         hist.dense.owned = hist.big_counts.owned = hist.big_keys.owned = false;
@@ -2988,13 +3001,13 @@ Status launch_scan(Launch &L, const DeviceBatch &b, uint64_t seg_cap, uint32_t *
     return Status::ok();
 }
 
-void launch_plan(Workspace &ws, uint64_t nbins, uint32_t per_item, hipStream_t st) {
+void launch_plan(Workspace &ws, uint64_t nbins, uint32_t per_item, hipStream_t st, const uint32_t *bin_kmers) {
     const unsigned nblk = unsigned((nbins + kPlanThreads - 1) / kPlanThreads);
     const uint32_t per_shift = (per_item & (per_item - 1)) == 0 ? uint32_t(__builtin_ctz(per_item)) : 32u;
     hipLaunchKernelGGL(k_plan_sums, dim3(nblk), dim3(kPlanThreads), 0, st, ws.bin_count.p, nbins, per_item, per_shift,
                        ws.plan_blk.p);
     hipLaunchKernelGGL(k_plan_write, dim3(nblk), dim3(kPlanThreads), 0, st, ws.bin_count.p, nbins, per_item,
-                       per_shift, ws.plan_blk.p, ws.bin_off.p, ws.cursor.p, ws.work.p, ws.stats.p);
+                       per_shift, ws.plan_blk.p, ws.bin_off.p, ws.cursor.p, ws.work.p, bin_kmers, ws.stats.p);
 }
 
 // Batched front half shared by insert, query and spectrum: scan to records,
@@ -3057,13 +3070,13 @@ Status batched_front(Launch &L, const DeviceBatch &b, uint32_t bin_log, uint32_t
         SegIn comp{ws.sorted.p, ws.seg_count.p + kSeg, 1, cap};
         KeySpec fine{bin_shift, 0, fine_bits, 1, 1};
         hipLaunchKernelGGL(k_seg_hist, hg, pb, 0, st, comp, fine, ws.bin_count.p, bin_kmers);
-        launch_plan(ws, nbins, per_item, st);
+        launch_plan(ws, nbins, per_item, st, bin_kmers);
         hipLaunchKernelGGL(k_partition, pg, pb, 0, st, comp, fine, ws.cursor.p, ws.rec.p);
         ws.final_sorted = ws.rec.p;
     } else {
         KeySpec one{bin_shift, 0, 0, 0, 1};
         hipLaunchKernelGGL(k_seg_hist, hg, pb, 0, st, segs, one, ws.bin_count.p, bin_kmers);
-        launch_plan(ws, nbins, per_item, st);
+        launch_plan(ws, nbins, per_item, st, bin_kmers);
         hipLaunchKernelGGL(k_partition, pg, pb, 0, st, segs, one, ws.cursor.p, ws.sorted.p);
         ws.final_sorted = ws.sorted.p;
     }
