@@ -468,6 +468,15 @@ class Server {
         if (s.status != SCB_OK) throw SuperCountingBloomError(s.status, s.message);
         return s;
     }
+    // Any finished request not yet waited for: its ticket (0 when none finished
+    // within timeout_us) and outcome; a failed request throws its own error.
+    uint64_t wait_any(uint64_t timeout_us, Served &s) {
+        uint64_t t = 0;
+        s = Served{};
+        check(scb_server_wait_any(h_, timeout_us, &t, &s));
+        if (t && s.status != SCB_OK) throw SuperCountingBloomError(s.status, s.message);
+        return t;
+    }
 
    private:
     scb_server *h_ = nullptr;

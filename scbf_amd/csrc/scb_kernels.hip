@@ -3119,6 +3119,26 @@ bool batched_ok(const Geometry &g, uint32_t bin_log) {
     return g.bse <= bin_log && g.nb <= (uint64_t(1) << 27) && (uint64_t(1) << bin_log) * g.bytes >= 4;
 }
 
+// Insert bin size for one batch: the tuned size, halved while the batch's
+// expected records per bin exceed 90% of a work item. A bin with more records
+// than one item is split over several items, which merge into the table with
+// global atomics instead of one read-modify-write of the bin; for a large batch
+// smaller bins that stay whole cost less than that merge. Records are expected
+// at the random-minimizer density 2 / (w + 1) per valid window, plus one per
+// read; the choice only affects speed, never the counters.
+uint32_t insert_bin_log_for(const Geometry &g, uint32_t tuned, uint32_t per_item, uint64_t n_bases,
+                            uint64_t n_reads) {
+    const uint64_t lost = n_reads * uint64_t(g.k - 1);
+    const double windows = n_bases > lost ? double(n_bases - lost) : 0.0;
+    const double records = windows * 2.0 / double(g.w + 1) + double(n_reads);
+    uint32_t bl = tuned;
+    const uint32_t floor_log = std::max<uint32_t>(g.bse, 12);
+    while (bl > floor_log && batched_ok(g, bl - 1) &&
+           records / double(uint64_t(1) << (g.cse - bl)) > 0.9 * double(per_item))
+        --bl;
+    return bl;
+}
+
 }  // namespace
 
 namespace {
@@ -3212,15 +3232,15 @@ Status Engine::insert(const DeviceBatch &b, Method method, InsertResult &r) {
         s = launch_scan<kModePlainInsert>(L, b, 0, nullptr, nullptr);
         mark(*ws, 1);
     } else {
+        const uint32_t per_item =
+            std::max<uint32_t>(1, std::min<uint32_t>(tune_.insert_item_records, kInsertItemRecords));
+        const uint32_t bin_log = insert_bin_log_for(geo_, tune_.insert_bin_log, per_item, b.n_bases, b.n_reads);
         const uint64_t args =
             key_of({1, uint64_t(uintptr_t(b.d_bases)), uint64_t(uintptr_t(b.d_offsets)), b.n_reads, b.n_bases,
-                    uint64_t(uintptr_t(d_table_)), uint64_t(fresh), tune_.insert_bin_log, tune_.insert_item_records});
+                    uint64_t(uintptr_t(d_table_)), uint64_t(fresh), bin_log, tune_.insert_item_records});
         s = run_graphable(*ws, tune_.use_graph, ws->graph_insert, args, [&]() -> Status {
         Status s;
         uint64_t nbins = 0;
-        uint32_t bin_log = tune_.insert_bin_log;
-        const uint32_t per_item =
-            std::max<uint32_t>(1, std::min<uint32_t>(tune_.insert_item_records, kInsertItemRecords));
         // k-mers per bin select the bins counted by distinct record
         SCB_HIP(ws->spec_bin_kmers.reserve(uint64_t(1) << (geo_.cse - bin_log)));
         s = batched_front(L, b, bin_log, per_item, nullptr, nbins, ws->spec_bin_kmers.p);

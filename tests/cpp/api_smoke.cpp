@@ -132,6 +132,13 @@ static int hot_path(const std::string &dir) {
         }
         REQUIRE(s2.n_rows == sp.rows.size() && s2.distinct == sp.distinct);
         for (size_t i = 0; i < sp.rows.size(); ++i) REQUIRE(sv[i] == sp.rows[i].first && sc[i] == sp.rows[i].second);
+        // wait_any hands out each finished ticket once
+        const uint64_t t3 = srv.submit_abundances(qry, vals.data(), bits.data());
+        scb::Served s3{};
+        uint64_t got = 0;
+        for (int i = 0; i < 200 && !got; ++i) got = srv.wait_any(50000, s3);
+        REQUIRE(got == t3 && s3.report.estimate_checksum == est.report.estimate_checksum);
+        REQUIRE(srv.wait_any(1000, s3) == 0);
     }
     // sampled spectrum: a reservoir of 5000 valid windows, estimates drawn from the map's
     auto ss = idx[0].sampled_spectrum(qry, 5000, 42);

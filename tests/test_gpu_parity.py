@@ -26,8 +26,9 @@ GEOMETRIES = [
     ("k31m21s25_u16_bse15", 31, 21, 25, 4, 16, 20, 15),  # 15-bit slot addresses (largest staged block)
     ("k15m9s11_u32_h1", 15, 9, 11, 1, 32, 18, 12),  # one hash per s-mer
     ("k31m21s30_u16_win2", 31, 21, 30, 3, 16, 20, 9),  # s-mers of 60 bits, fimpera window 2
-    # one 2^15-slot bin holds every record: work items split at 2048 records
-    # (insert) merge into the table with atomics, sub-batches of ~960 records each
+    # a 2^15-slot table: even the smallest insert bins (2^12 slots) hold more
+    # records than one work item, so items split at 2048 records merge into the
+    # table with atomics, in sub-batches of ~960 records each
     ("k31m21s25_u16_onebin", 31, 21, 25, 4, 16, 15, 9),
     # tiny tables (config.rs:76 admits any bse < cse): one bin smaller than the
     # merge vectors, 16-bit counts saturating across insert sub-batches
@@ -272,17 +273,19 @@ def test_weighted_mixture_u32_parity(gpu, oracle, method):
     _parity_roundtrip(scb, oracle, (21, 15, 17, 3, 32, 22, 10), ib, io, ib, io, method)
 
 
-@pytest.mark.parametrize("reads", [4000, 2500], ids=["split_items", "one_item"])
+@pytest.mark.parametrize("reads,cse", [(4000, 16), (2500, 20)], ids=["split_items", "one_item"])
 @pytest.mark.parametrize("bits", [8, 16, 32])
-def test_work_items_counted_in_sub_batches(gpu, oracle, bits, reads):
-    """About 2,600 (4,000 reads) or 1,600 (2,500 reads) records per 2^15-slot
-    bin: each work item makes more than 65,535 counter increments, so it is
-    counted and merged in sub-batches (the 16-bit LDS counts cannot overflow),
-    including the first insert into a new table and a second insert on top of
-    it. With 2,500 reads every bin is one work item (at most 2048 records) that
-    is too large for distinct-record counting in one pass."""
+def test_work_items_counted_in_sub_batches(gpu, oracle, bits, reads, cse):
+    """Work items that make more than 65,535 counter increments are counted and
+    merged in sub-batches (the 16-bit LDS counts cannot overflow), including the
+    first insert into a new table and a second insert on top of it. 2,500 reads
+    into 2^20 slots put ~1,600 records in each 2^15-slot bin: every bin is one
+    work item (at most 2048 records) too large for distinct-record counting in
+    one pass. 4,000 reads into 2^16 slots put ~5,200 records in each bin even at
+    the smallest insert bin size (2^12 slots): bins split into items that merge
+    with atomics."""
     scb = gpu
-    k, m, s, h, cse, bse = 31, 21, 25, 4, 20, 9
+    k, m, s, h, bse = 31, 21, 25, 4, 9
     g = synth.genome(400_000, 81)
     b1, o1 = synth.fixed_reads(g, reads, 150, 82, sub_rate=0.01)
     b2, o2 = synth.fixed_reads(g, reads, 150, 83, sub_rate=0.01)
