@@ -81,6 +81,7 @@ enum StatSlot {
     kStGlobalKmers = 10,
     kStBadOffsets = 11,
     kStEmitted = 12,
+    kStSetFull = 13,  // a key found no free slot in the global spectrum set
     kStCount = 16
 };
 
@@ -385,15 +386,44 @@ constexpr uint32_t kHistBigCap = 1u << 20;
 // ---- distinct-k-mer spectrum (filter.rs:407-460; fastx.rs:255-308, 392-410) ----
 // A canonical k-mer always lands in the same counter block (its minimizer is
 // strand-symmetric), so every occurrence of a key is processed by the work
-// items of one bin. Single-item bins whose k-mers fit dedup in an LDS hash set;
-// split or oversized bins dedup in a global open-addressing set. The estimate
-// of a key is the same at every occurrence, so "first wins" needs no order.
-constexpr uint32_t kSetCap = 8192;            // LDS key set (64 KiB)
-constexpr uint32_t kSetFill = kSetCap * 3 / 4;
+// items of one bin. A single-item bin dedups in an LDS hash set of kSetCap keys
+// (32 KiB: two spectrum workgroups share a CU); a key whose first kSetProbes
+// slots there hold other keys goes to a global open-addressing set, as do all
+// keys of split bins and of bins with more than kSetDirect k-mers. The
+// estimate of a key is the same at every occurrence, so "first wins" needs no
+// order.
+constexpr uint32_t kSetLog = 12, kSetCap = 1u << kSetLog;
+constexpr uint32_t kSetProbes = 32;
+constexpr uint32_t kSetDirect = 4 * kSetCap;
+constexpr uint32_t kGlobalProbes = 1024;      // a global insert that finds no slot raises kStSetFull
+constexpr uint32_t kSpecHist = 1024;          // LDS histogram entries of the spectrum bin kernel
 constexpr uint64_t kEmptyKey = ~0ull;          // keys use at most 62 bits (k <= 31)
 
 __device__ __forceinline__ uint32_t key_slot(uint64_t key, uint32_t log2cap) {
     return uint32_t((key * 0x9E3779B97F4A7C15ull) >> (64 - log2cap));
+}
+
+// Inserts key into an open-addressing set of 2^log2cap slots, visiting at most
+// max_probes slots of its probe sequence: 1 if this call inserted it, 0 if it
+// was there, 2 if every visited slot holds another key. Slots only change from
+// empty to a key, so a key never occupies two slots, and a key that found its
+// window full never enters this set later (the window stays full): with a
+// second set taking the keys that return 2, each key is in exactly one set.
+__device__ __forceinline__ uint32_t set_insert_bounded(unsigned long long *set, uint32_t log2cap, uint64_t key,
+                                                       uint32_t max_probes) {
+    const uint32_t mask = (1u << log2cap) - 1;
+    uint32_t h = key_slot(key, log2cap);
+    for (uint32_t p = 0; p < max_probes; ++p) {
+        const unsigned long long cur = set[h];
+        if (cur == key) return 0;
+        if (cur == kEmptyKey) {
+            const unsigned long long prev = atomicCAS(&set[h], kEmptyKey, (unsigned long long)key);
+            if (prev == kEmptyKey) return 1;
+            if (prev == key) return 0;
+        }
+        h = (h + 1) & mask;
+    }
+    return 2;
 }
 
 // Returns true if key was not in the set (this call inserted it).
@@ -420,8 +450,9 @@ struct HistOut {
     uint64_t *stats;  // stats[kStOverflow] is raised when big_keys is full
 };
 
+template <uint32_t SMALL = kHistSmall>
 __device__ __forceinline__ void spec_hist_add(const HistOut &a, uint32_t *s_hist, uint32_t v) {
-    if (v < kHistSmall) {
+    if (v < SMALL) {
         atomicAdd(&s_hist[v], 1u);
     } else if (v < a.dense_n) {
         atomicAdd(&a.dense[v], 1ull);
@@ -1415,7 +1446,7 @@ __device__ __forceinline__ uint32_t sat_merge_word(uint32_t old, const uint32_t 
 // Work items hold at most this many records (their order lives in LDS as u16).
 constexpr uint32_t kMaxItemRecords = 4096;
 // Workgroup sizes of the bin kernels (their launch bounds).
-constexpr int kInsertThreads = 1024, kQueryThreads = 512, kSpectrumThreads = 1024;
+constexpr int kInsertThreads = 1024, kQueryThreads = 512, kSpectrumThreads = 512;
 
 // Orders the records [rb, rb + n) of a work item by k-mer count, longest
 // first, into s_perm (indices relative to rb). A wave's s-mer loop runs as
@@ -1977,7 +2008,7 @@ __global__ __launch_bounds__(1024) void k_spectrum_plan(const uint32_t *bin_coun
     __syncthreads();
     unsigned long long loc = 0;
     for (uint64_t b = threadIdx.x; b < nbins; b += blockDim.x)
-        if (bin_count[b] > per_item || bin_kmers[b] > kSetFill) loc += bin_kmers[b];
+        if (bin_count[b] > per_item || bin_kmers[b] > kSetDirect) loc += bin_kmers[b];
     atomicAdd(&s_sum, loc);
     __syncthreads();
     if (threadIdx.x == 0) stats[kStGlobalKmers] = s_sum;
@@ -2036,11 +2067,11 @@ __global__ __launch_bounds__(kSpectrumThreads) void k_bin_spectrum(SpecArgs sa, 
     const uint32_t nwords = tab_bytes / 4;
     uint32_t *s_tab = reinterpret_cast<uint32_t *>(s_raw);
     unsigned long long *s_set = reinterpret_cast<unsigned long long *>(s_raw + tab_bytes);
-    uint32_t *s_hist = reinterpret_cast<uint32_t *>(s_raw + tab_bytes + kSetCap * 8);
+    uint32_t *s_hist = reinterpret_cast<uint32_t *>(s_raw + tab_bytes + kSetCap * 8);  // kSpecHist counts
     const T *lt = reinterpret_cast<const T *>(s_raw);
     const uint64_t nwork = a.stats[kStNWork];
     unsigned long long *head = (unsigned long long *)&a.stats[kStWorkHead];
-    for (uint32_t i = tid; i < kHistSmall; i += nthr) s_hist[i] = 0;
+    for (uint32_t i = tid; i < kSpecHist; i += nthr) s_hist[i] = 0;
     uint64_t queried = 0, distinct = 0;
     for (;;) {
         if (tid == 0) s_item = atomicAdd(head, 1ull);
@@ -2050,7 +2081,7 @@ __global__ __launch_bounds__(kSpectrumThreads) void k_bin_spectrum(SpecArgs sa, 
         const uint4 wk = a.work[item];
         const uint64_t bin = wk.x;
         const uint32_t rb = wk.y, re = wk.z;
-        const bool global = (wk.w >> 31) || (wk.w & 0x7FFFFFFFu) > kSetFill;
+        const bool global = (wk.w >> 31) || (wk.w & 0x7FFFFFFFu) > kSetDirect;
         const uint32_t *tw = reinterpret_cast<const uint32_t *>(a.table) + bin * uint64_t(bin_slots * (BITS / 8) / 4);
         constexpr int R = kMaxItemRecords / kSpectrumThreads;
         uint32_t hi[R];
@@ -2088,12 +2119,15 @@ __global__ __launch_bounds__(kSpectrumThreads) void k_bin_spectrum(SpecArgs sa, 
                         }
                         uint32_t ab = wm.push(est, g.win);
                         if (j + 1 >= g.win) {
-                            bool fresh = global ? set_insert(sa.gset, sa.glog2, key) : set_insert(s_set, 13, key);
+                            uint32_t r = global ? 2u : set_insert_bounded(s_set, kSetLog, key, kSetProbes);
+                            if (r == 2) r = set_insert_bounded(sa.gset, sa.glog2, key, kGlobalProbes);
+                            if (r == 2)  // the global set is too small: the call runs again with a larger one
+                                atomicOr((unsigned long long *)&a.stats[kStSetFull], 1ull);
                             ++queried;
-                            if (fresh) {
+                            if (r == 1) {
                                 ++distinct;
                                 if (sa.emit_keys) emit_pair(sa, key, ab);
-                                else spec_hist_add(sa.h, s_hist, ab);
+                                else spec_hist_add<kSpecHist>(sa.h, s_hist, ab);
                             }
                         }
                     },
@@ -2102,7 +2136,7 @@ __global__ __launch_bounds__(kSpectrumThreads) void k_bin_spectrum(SpecArgs sa, 
         });
         __syncthreads();
     }
-    for (uint32_t i = tid; i < kHistSmall; i += nthr)
+    for (uint32_t i = tid; i < kSpecHist; i += nthr)
         if (s_hist[i]) atomicAdd(&sa.h.dense[i], (unsigned long long)s_hist[i]);
     uint64_t vals[2] = {queried, distinct};
 #pragma unroll
@@ -3584,11 +3618,11 @@ Status Engine::spectrum_pass(Workspace &ws, const DeviceBatch &b, const void *hi
     hipLaunchKernelGGL(k_spectrum_plan, dim3(1), dim3(1024), 0, st, ws.bin_count.p, ws.spec_bin_kmers.p, nbins,
                        per_item, ws.stats.p);
     if ((s = finish(ws)).code) return s;
-    const uint64_t gk = ws.h_stats[kStGlobalKmers];
-    uint32_t glog2 = 10;
-    while ((uint64_t(1) << glog2) < 2 * gk + 2) ++glog2;
-    SCB_HIP(ws.spec_gset.reserve(uint64_t(1) << glog2));
-    fill(reinterpret_cast<uint32_t *>(ws.spec_gset.p), (uint64_t(1) << glog2) * 2, 0xFFFFFFFFu, st);
+    // Global set: a slot per k-mer of the bins that use it directly, twice
+    // over, plus room for the keys of LDS-set bins whose probe windows fill
+    // (a 64th of the batch's k-mers). A pass whose global set overflows runs
+    // again with a slot per k-mer of the batch, four times over.
+    const uint64_t gk = ws.h_stats[kStGlobalKmers], all = ws.h_stats[kStValid];
     SpecArgs sa{};
     sa.b.sorted = ws.final_sorted;
     sa.b.bin_off = ws.bin_off.p;
@@ -3600,8 +3634,6 @@ Status Engine::spectrum_pass(Workspace &ws, const DeviceBatch &b, const void *hi
     sa.b.bin_log = bin_log;
     sa.b.per_item = per_item;
     sa.bin_kmers = ws.spec_bin_kmers.p;
-    sa.gset = ws.spec_gset.p;
-    sa.glog2 = glog2;
     if (hist) sa.h = *static_cast<const HistOut *>(hist);
     sa.h.stats = ws.stats.p;
     sa.stage = stage;
@@ -3610,9 +3642,24 @@ Status Engine::spectrum_pass(Workspace &ws, const DeviceBatch &b, const void *hi
         sa.emit_vals = ws.emit_vals.p;
         sa.emit_cap = b.n_bases + 1;
     }
-    size_t lds = (stage ? (size_t(1) << bin_log) * geo_.bytes : 0) + kSetCap * 8 + kHistSmall * 4;
+    size_t lds = (stage ? (size_t(1) << bin_log) * geo_.bytes : 0) + kSetCap * 8 + kSpecHist * 4;
     const int thr = kSpectrumThreads;
     const bool wide = geo_.win > 8;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        const uint64_t want = attempt == 0 ? 2 * (gk + all / 64) + 2 : 4 * all + 2;
+        uint32_t glog2 = 10;
+        while ((uint64_t(1) << glog2) < want) ++glog2;
+        SCB_HIP(ws.spec_gset.reserve(uint64_t(1) << glog2));
+        fill(reinterpret_cast<uint32_t *>(ws.spec_gset.p), (uint64_t(1) << glog2) * 2, 0xFFFFFFFFu, st);
+        if (attempt > 0) {
+            // start over: the kernel's counts, the work queue and the histogram
+            for (int slot : {int(kStWorkHead), int(kStQueried), int(kStDistinct), int(kStEmitted), int(kStOverflow),
+                             int(kStSetFull)})
+                fill(reinterpret_cast<uint32_t *>(ws.stats.p + slot), 2, 0, st);
+            if (hist) ws.hist.clear(geo_, st);
+        }
+        sa.gset = ws.spec_gset.p;
+        sa.glog2 = glog2;
 #define SCB_SPEC(BITS, WC)                                                                      \
     do {                                                                                        \
         s = set_lds(k_bin_spectrum<BITS, WC>, lds);                                             \
@@ -3620,17 +3667,20 @@ Status Engine::spectrum_pass(Workspace &ws, const DeviceBatch &b, const void *hi
         int grid = persistent_grid(k_bin_spectrum<BITS, WC>, thr, lds, device_);                \
         hipLaunchKernelGGL((k_bin_spectrum<BITS, WC>), dim3(grid), dim3(thr), lds, st, sa, L.dg); \
     } while (0)
-    switch (geo_.bits) {
-        case 8: if (wide) SCB_SPEC(8, 32); else SCB_SPEC(8, 8); break;
-        case 16: if (wide) SCB_SPEC(16, 32); else SCB_SPEC(16, 8); break;
-        default: if (wide) SCB_SPEC(32, 32); else SCB_SPEC(32, 8); break;
-    }
+        switch (geo_.bits) {
+            case 8: if (wide) SCB_SPEC(8, 32); else SCB_SPEC(8, 8); break;
+            case 16: if (wide) SCB_SPEC(16, 32); else SCB_SPEC(16, 8); break;
+            default: if (wide) SCB_SPEC(32, 32); else SCB_SPEC(32, 8); break;
+        }
 #undef SCB_SPEC
-    if (!s.good()) return s;
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return Status{8, std::string("device error: k_bin_spectrum: ") + hipGetErrorString(e)};
-    mark(ws, 3);
-    if ((s = finish(ws)).code) return s;
+        if (!s.good()) return s;
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return Status{8, std::string("device error: k_bin_spectrum: ") + hipGetErrorString(e)};
+        mark(ws, 3);
+        if ((s = finish(ws)).code) return s;
+        if (!ws.h_stats[kStSetFull]) break;
+        if (attempt == 1) return Status{8, "device error: the spectrum's global key set overflowed"};
+    }
     collect_times(ws, true);  // the bin stage includes the plan's host round trip
     if (ws.h_stats[kStOverflow]) return Status{8, "device error: spectrum has more than 2^20 distinct large estimates"};
     if (emit && ws.h_stats[kStEmitted] > b.n_bases + 1)

@@ -167,6 +167,42 @@ def test_spectrum_parity(gpu, oracle, geo, method):
     assert sum(c for _, c in rows) == distinct   # fastx.rs:161
 
 
+@pytest.mark.parametrize("bits", [16, 32])
+def test_spectrum_key_set_overflow_parity(gpu, oracle, bits):
+    """Four 2^14-slot spectrum bins, each with ~8,000 distinct keys: far more than
+    the 4,096-key LDS set of a work item, so most keys find their probe window
+    full and go to the global set, which then overflows its first size and the
+    pass runs again with a larger one. Rows, queried and distinct keys must still
+    equal the oracle's."""
+    scb = gpu
+    k, m, s, h, cse, bse = 31, 21, 25, 4, 16, 9
+    print(f"spectrum overflow case u{bits}", flush=True)
+    g = synth.genome(40_000, 31)
+    ib, io = synth.fixed_reads(g, 600, 150, 32, sub_rate=0.01)
+    qb, qo = synth.random_records(250, 150, 151, 33)  # random sequence: every k-mer distinct
+    cfg = scb.SuperCountingBloomConfig(k=k, m=m, s=s, n_hashes=h, counter_bits=bits, counter_slots_exponent=cse,
+                                       block_slots_exponent=bse, threads=1, queue=4)
+    b = scb.SuperCountingBloomBuilder(cfg)
+    b.add_sequences((ib, io))
+    idx = b.into_index()
+    idx.set_method(0)  # batched: the binned key sets
+    chunks = [bytes(qb[int(qo[i]):int(qo[i + 1])]) for i in range(qo.size - 1)]
+    chunks = chunks + chunks[:100]  # repeated keys as well
+    rows, queried, distinct = idx.spectrum(chunks)
+    of = oracle.Filter(oracle.make_cfg(k, m, s, h, bits, cse, bse))
+    oracle.insert_records(of, ib, io)
+    sp = oracle.Spectrum()
+    oq = sum(sp.add_chunk(of, c) for c in chunks)
+    assert queried == oq and distinct == sp.distinct()
+    assert distinct > 4 * 6000  # more keys than four LDS sets hold
+    assert rows == sp.rows()
+    # the accumulator's emitting pass takes the same overflow path
+    acc = idx.spectrum_accumulator()
+    acc.add(chunks)
+    acc.add(chunks[:50])
+    assert acc.rows() == (rows, queried + sum(max(0, len(c) - k + 1) for c in chunks[:50]), distinct)
+
+
 SPEC_DUP_GEOS = SPEC_GEOS + [(9, 5, 7, 2, 32, 1, 0), (9, 5, 7, 3, 8, 4, 2)]  # bins of 8 and 4 bytes
 
 
