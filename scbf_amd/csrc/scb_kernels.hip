@@ -3035,6 +3035,13 @@ Status launch_scan(Launch &L, const DeviceBatch &b, uint64_t seg_cap, uint32_t *
     return Status::ok();
 }
 
+// Bits of the bin index sorted by the fine pass when 2^lb bins need two passes
+// (lb > 11): the bits are split evenly (at least 7 fine), so the coarse pass
+// ranks each 8192-record tile over at most 2^((lb + 1) / 2) buckets and its
+// scattered runs stay long (with 2^11 coarse buckets a tile held ~4 records
+// per bucket and wrote each 128-byte line several times).
+uint32_t sort_fine_bits(uint32_t lb) { return std::max<uint32_t>(7, (lb + 1) / 2); }
+
 void launch_plan(Workspace &ws, uint64_t nbins, uint32_t per_item, hipStream_t st, const uint32_t *bin_kmers) {
     const unsigned nblk = unsigned((nbins + kPlanThreads - 1) / kPlanThreads);
     const uint32_t per_shift = (per_item & (per_item - 1)) == 0 ? uint32_t(__builtin_ctz(per_item)) : 32u;
@@ -3070,7 +3077,7 @@ Status batched_front(Launch &L, const DeviceBatch &b, uint32_t bin_log, uint32_t
     // exceed one tile's key range
     const bool two_pass = nbins > kPartRange;
     const uint32_t lb = uint32_t(__builtin_ctzll(nbins));
-    const uint32_t fine_bits = std::max<uint32_t>(7, lb > 11 ? lb - 11 : 0);
+    const uint32_t fine_bits = sort_fine_bits(lb);
     const uint32_t ncoarse = two_pass ? uint32_t(nbins >> fine_bits) : 0;
     constexpr uint32_t kCopies = 32;  // coarse counters are hot: spread each over 32 copies
     if (two_pass) {
@@ -3506,7 +3513,7 @@ uint64_t carve_query(uint8_t *base, const Geometry &g, uint64_t n_bases, uint32_
         take(ws.plan_blk, 2 * ((nbins + kPlanThreads - 1) / kPlanThreads));
         if (nbins > kPartRange) {
             const uint32_t lb = uint32_t(__builtin_ctzll(nbins));
-            const uint32_t fine_bits = std::max<uint32_t>(7, lb > 11 ? lb - 11 : 0);
+            const uint32_t fine_bits = sort_fine_bits(lb);
             const uint64_t ncoarse = nbins >> fine_bits;
             take(ws.coarse_hist, ncoarse * 32);
             take(ws.ccursor, ncoarse * 32);
