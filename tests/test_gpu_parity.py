@@ -298,6 +298,31 @@ def test_work_items_counted_in_sub_batches(gpu, oracle, bits, reads, cse):
     assert np.array_equal(b.counters().astype(np.uint32), of.counters())
 
 
+@pytest.mark.parametrize("bits", [16, 32])
+def test_insert_bins_sized_per_batch(gpu, oracle, bits):
+    """A batch whose 2^15-slot insert bins would hold ~2,100 records each (more
+    than one 2048-record work item) is inserted with 2^14-slot bins instead; a
+    small batch into the same table keeps 2^15-slot bins. Counters after both
+    equal the oracle's."""
+    scb = gpu
+    k, m, s, h, cse, bse = 21, 15, 17, 3, 20, 10
+    g = synth.genome(200_000, 91)
+    b1, o1 = synth.fixed_reads(g, 2000, 150, 92, sub_rate=0.005)
+    b2, o2 = synth.fixed_reads(g, 300, 150, 93, sub_rate=0.005)
+    of = oracle.Filter(oracle.make_cfg(k, m, s, h, bits, cse, bse))
+    oracle.insert_records(of, b1, o1)
+    oracle.insert_records(of, b2, o2)
+    b = scb.SuperCountingBloomBuilder(_cfg(scb, k, m, s, h, bits, cse, bse))
+    b.set_method(scb.METHOD_BATCHED)
+    r1 = b.add_sequences((b1, o1))
+    r2 = b.add_sequences((b2, o2))
+    # records (super-k-mers) = (s-mers - k-mers) / (k - s); 32 bins of 2^15 slots
+    rec1, rec2 = ((r.inserted_smers - r.inserted_kmers) // (k - s) for r in (r1, r2))
+    assert rec1 / 32 > 2048  # the large batch's 2^15-slot bins would split
+    assert rec2 / 32 < 0.9 * 2048
+    assert np.array_equal(b.counters().astype(np.uint32), of.counters())
+
+
 def _low_complexity_reads(seed):
     """Homopolymers, short tandem repeats (ACGT is its own reverse complement),
     repeats broken by N, by lowercase and by single substitutions, repeats
