@@ -762,7 +762,14 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
             // equal keys, the leftmost or the rightmost index.
             static_assert(WREG >= 9 && WREG <= 25, "register windows need 9 <= w <= 25");
             constexpr int NK = 8 + WREG - 1, NV = (NK + 7) / 8;
-            if (tid < kTile / 8) {
+            // the groups cover the tile's windows and the w after it, hj < kTile + w
+            // (threads kTile / 8 .. of the last wave, otherwise idle here): a group
+            // past kTile + w - 1 reads keys beyond the NHW computed ones, which only
+            // its windows past kTile + w - 1 use, and those are never read
+            constexpr int kGroups = (kTile + WREG + 7) / 8;
+            static_assert(kGroups <= kTileThreads, "one group per thread");
+            static_assert(8 * (kGroups - 1) + 8 * NV <= kHashN, "a group's key loads stay inside the key array");
+            if (tid < kGroups) {
                 const int h0 = tid * 8;
                 uint32_t kw[NV * 4];  // two 16-bit keys per word
                 const uint4 *src = reinterpret_cast<const uint4 *>(s_K + h0);
@@ -813,23 +820,6 @@ __global__ __launch_bounds__(kTileThreads) void k_scan(ScanArgs a, DevGeo g) {
                     const uint32_t pos = uint32_t(h0) + ((left ? pl : ~pr) & 31u);
                     s_mpos[h0 + i] = bad ? kNoWin : uint16_t(pos);
                 }
-            }
-            // the w windows after the tile, hj in [kTile, kTile + w): one lane
-            // each of the last wave, over the window's w keys
-            if (tid >= kTileThreads - 64 && tid < kTileThreads - 64 + WREG) {
-                const int hj = kTile + (tid - (kTileThreads - 64));
-                uint32_t cl = ~0u, cr = ~0u;
-#pragma unroll
-                for (int q = 0; q < WREG; ++q) {
-                    const uint32_t key = uint32_t(s_K[hj + q]) << 16;
-                    cl = min(cl, key | uint32_t(q));
-                    cr = min(cr, key | uint32_t(31 - q));
-                }
-                const int li = hj + kLeft - 1;
-                const uint32_t bad = get_bits(s_bad, li, k) | get_bits(s_brk, li + 1, k - 1);
-                const bool left = 2 * int(__popc(get_bits(s_tg, li, k))) > k;  // leftmost minimum on ties
-                const uint32_t pos = uint32_t(hj) + (left ? (cl & 0xFFFFu) : 31u - (cr & 0xFFFFu));
-                s_mpos[hj] = bad ? kNoWin : uint16_t(pos);
             }
             __syncthreads();
         } else {
