@@ -40,6 +40,7 @@
 
 #include "scb_bases.h"
 #include "scb_engine.h"
+#include "scb_hash.h"
 #include "scb_store_run.h"
 
 namespace scb {
@@ -103,32 +104,6 @@ DevGeo dev_geo(const Geometry &g) {
     d.shard_mask = g.shard_mask, d.shard_shift = g.shard_shift, d.sub_log = g.sub_log;
     d.block_mask = g.block_mask;
     return d;
-}
-
-// filter.rs:629-635 (SplitMix64 output function)
-__device__ __forceinline__ uint64_t xs64(uint64_t x) {
-    x += 0x9E3779B97F4A7C15ull;
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-    return x ^ (x >> 31);
-}
-
-// Low 15 bits of xs64(x), enough for a slot address inside a block of at most
-// 2^15 slots. The second multiply only has to be exact in bits 0..45 of its
-// product (bit b < 15 of x ^ (x >> 31) reads bits b and b + 31): the low word
-// comes from one 32x32->64 multiply and bits 32..45 from two 14-bit partial
-// products, which the full-rate 24-bit multiplier computes exactly.
-__device__ __forceinline__ uint32_t xs64_low15(uint64_t x) {
-    constexpr uint64_t kC2 = 0x94D049BB133111EBull;
-    x += 0x9E3779B97F4A7C15ull;
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-    x ^= x >> 27;
-    const uint32_t yl = uint32_t(x), yh = uint32_t(x >> 32);
-    const uint64_t p = uint64_t(yl) * uint32_t(kC2);
-    const uint32_t hi = uint32_t(p >> 32) + __umul24(yh & 0x3FFFu, uint32_t(kC2) & 0x3FFFu) +
-                        __umul24(yl & 0x3FFFu, uint32_t(kC2 >> 32) & 0x3FFFu);
-    // low word of z ^ (z >> 31) for z = hi:lo; (z >> 31) is one funnel shift
-    return uint32_t(p) ^ __builtin_amdgcn_alignbit(hi, uint32_t(p), 31);
 }
 
 // Calls f(address) for the n_hashes chain values xs64(x), xs64(xs64(x)), ...

@@ -4,7 +4,8 @@ store_run (the query kernel's packed estimate stores) byte by byte for every
 alignment and length, pack4_bases (the scan's four-at-a-time base encoding)
 against the per-base rule for every byte value at every position, and the
 rolling canonical s-mer / k-mer streams of the bin kernels against a direct
-evaluation for every k <= 32 and s <= k."""
+evaluation for every k <= 32 and s <= k, and the short form of the counter
+address hash against the full SplitMix64 output function."""
 import os
 import shutil
 import subprocess
@@ -37,3 +38,8 @@ def test_pack4_bases_matches_the_per_base_rule(tmp_path):
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
 def test_smer_and_kmer_streams_match_direct_evaluation(tmp_path):
     _run_check(tmp_path, "smer_stream_check")
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_address_hash_short_form_matches_splitmix64(tmp_path):
+    _run_check(tmp_path, "hash_check")
