@@ -1525,15 +1525,13 @@ __device__ __forceinline__ void align_words(const uint32_t (&W)[5], uint64_t b, 
 // its packed words are in flight while the current one is processed: a record
 // and its bases are two dependent global loads, which would otherwise stall
 // every round. Indices past the end are clamped, so every load is in bounds.
-// With IDX, f also receives the record's index in the item (its s_perm entry).
-template <bool IDX = false, class F>
+template <class F>
 __device__ __forceinline__ void walk_records(const uint64_t *sorted, const uint32_t *pk, uint32_t k, uint32_t rb, uint32_t n,
                                              const uint16_t *s_perm, uint32_t tid, uint32_t nthr, F &&f) {
     if (tid >= n) return;
     const uint32_t last = n - 1;
-    uint32_t pc = s_perm[tid], pn = s_perm[min(tid + nthr, last)];
-    uint64_t rc = sorted[rb + pc];
-    uint64_t rn = sorted[rb + pn];
+    uint64_t rc = sorted[rb + s_perm[tid]];
+    uint64_t rn = sorted[rb + s_perm[min(tid + nthr, last)]];
     uint32_t W[5];
     load_words5(pk, rc, k, W);
     for (uint32_t i = tid; i < n; i += nthr) {
@@ -1541,14 +1539,47 @@ __device__ __forceinline__ void walk_records(const uint64_t *sorted, const uint3
         load_words5(pk, rn, k, V);
         const uint32_t pnn = s_perm[min(i + 2 * nthr, last)];
         const uint64_t rnn = sorted[rb + pnn];
-        if constexpr (IDX) f(rc, W, pc);
-        else f(rc, W);
+        f(rc, W);
         rc = rn;
+        rn = rnn;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) W[q] = V[q];
+    }
+}
+
+// walk_records for the insert: f(record high word, bases from the first in
+// A0..A3, index in the item). With use_first, this thread's first record
+// (s_perm[tid]) is passed in registers (its bases first, its high word
+// first_hi) and neither it nor its bases are read from HBM.
+template <class F>
+__device__ __forceinline__ void walk_records_aligned(const uint64_t *sorted, const uint32_t *pk, uint32_t k, uint32_t rb,
+                                                     uint32_t n, const uint16_t *s_perm, uint32_t tid, uint32_t nthr,
+                                                     F &&f, bool use_first, uint4 first, uint32_t first_hi) {
+    if (tid >= n) return;
+    const uint32_t last = n - 1;
+    uint32_t pc = s_perm[tid], pn = s_perm[min(tid + nthr, last)];
+    uint64_t rn = sorted[rb + pn];
+    uint32_t hc, A0, A1, A2, A3;
+    if (use_first) {
+        hc = first_hi, A0 = first.x, A1 = first.y, A2 = first.z, A3 = first.w;
+    } else {
+        const uint64_t rc = sorted[rb + pc];
+        uint32_t W[5];
+        load_words5(pk, rc, k, W);
+        align_words(W, rc & 0xFFFFFFFFull, A0, A1, A2, A3);
+        hc = uint32_t(rc >> 32);
+    }
+    for (uint32_t i = tid; i < n; i += nthr) {
+        uint32_t V[5];
+        load_words5(pk, rn, k, V);
+        const uint32_t pnn = s_perm[min(i + 2 * nthr, last)];
+        const uint64_t rnn = sorted[rb + pnn];
+        f(hc, A0, A1, A2, A3, pc);
+        align_words(V, rn & 0xFFFFFFFFull, A0, A1, A2, A3);
+        hc = uint32_t(rn >> 32);
         rn = rnn;
         pc = pn;
         pn = pnn;
-#pragma unroll
-        for (int q = 0; q < 5; ++q) W[q] = V[q];
     }
 }
 
@@ -1599,7 +1630,15 @@ static_assert(kInsertItemRecords <= 4096, "table entries hold 12-bit record indi
 // (16-bit halves of words, record l in half l & 1 of word l >> 1). Identical
 // keys walk the same probe sequence and a slot is never released, so each key
 // has exactly one leader; at most n of the 2n slots are taken, so every probe
-// ends. s_ws (kRecDedupWords words) is free again after the closing barrier.
+// ends. Record i's key (its bases, aligned and masked to its length) stays at
+// dedup_key(s_ws, i) and its high word at dedup_hi(s_ws, i) until s_ws is
+// written again.
+__device__ __forceinline__ const uint4 &dedup_key(const uint32_t *s_ws, uint32_t i) {
+    return reinterpret_cast<const uint4 *>(s_ws)[i];
+}
+__device__ __forceinline__ uint32_t dedup_hi(const uint32_t *s_ws, uint32_t i) {
+    return s_ws[4 * kInsertItemRecords + i];
+}
 template <int R>
 __device__ __forceinline__ void find_duplicates(const uint64_t *sorted, const uint32_t *pk, uint32_t rb, uint32_t n,
                                                 uint32_t k, uint32_t *s_ws, uint16_t *s_lead, uint32_t *s_mult,
@@ -1714,13 +1753,26 @@ __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(
                            (uint64_t(wk.w & 0x7FFFFFFFu) + uint64_t(g.k - g.s) * n) * g.nh <= 65535u;
         uint32_t nwalk = n;
         bool one = true;
+        // with duplicates found, the leader this thread walks first (s_perm
+        // entry tid): its bases and high word
+        uint4 lkey = make_uint4(0, 0, 0, 0);
+        uint32_t lhi = 0;
         if (dedup) {
             constexpr int R = kInsertItemRecords / kInsertThreads;
             uint32_t hi[R];
             find_duplicates<R>(a.sorted, a.pk, rb, n, g.k, s_cnt, s_lead, s_mult, tid, nthr, hi);  // (synchronizes)
-            zero_lds(s_cnt, bin_slots / 2, tid, nthr);
             order_by_hi<R>(hi, n, s_perm, s_cls, tid, nthr, nullptr, 0, s_lead, &s_nwalk);  // (synchronizes)
             nwalk = s_nwalk;
+            // the keys share the LDS of the counts: the first round of the walk
+            // takes its leaders' bases from them into registers (no second read
+            // from HBM) before the counts are cleared
+            if (tid < nwalk) {
+                const uint32_t li = s_perm[tid];
+                lkey = dedup_key(s_cnt, li), lhi = dedup_hi(s_cnt, li);
+            }
+            __syncthreads();
+            zero_lds(s_cnt, bin_slots / 2, tid, nthr);
+            __syncthreads();
         } else {
             zero_lds(s_cnt, bin_slots / 2, tid, nthr);
             order_by_length<kInsertItemRecords / kInsertThreads>(a.sorted, rb, n, s_perm, s_cls, tid, nthr,
@@ -1729,39 +1781,45 @@ __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(
         }
         const uint32_t step = one ? nwalk : sub_recs;
         uint32_t *tw = reinterpret_cast<uint32_t *>(a.table) + bin * uint64_t(nwords);
-        for (uint32_t r0 = 0; r0 < nwalk; r0 += step) {
-            const uint32_t m = min(step, nwalk - r0);
-            walk_records<true>(a.sorted, a.pk, g.k, rb, m, s_perm + r0, tid, nthr, [&](uint64_t rec, const uint32_t (&W)[5], uint32_t ri) {
-                const uint64_t j = rec & 0xFFFFFFFFull;
-                const uint32_t nk = uint32_t((rec >> 32) & 31) + 1;
-                const uint32_t lbase = uint32_t((rec >> 37) & ((1u << lshift) - 1)) << g.bse;
-                // increments per s-mer address: the number of identical records
-                const uint32_t mult = dedup ? (s_mult[ri >> 1] >> (16 * (ri & 1))) & 0xFFFFu : 1u;
-                uint32_t A0, A1, A2, A3;
-                align_words(W, j, A0, A1, A2, A3);
-                with_nh(g.nh, [&](auto nht) {
-                    constexpr int NH = decltype(nht)::value;
-                    if (lshift > 0) {  // the record's block lies in one half of the bin
-                        const uint32_t wbase = lbase & (half_words - 1), inc = mult << (16 * (lbase >> half_log));
-                        for_each_smer(
-                            A0, A1, A2, A3, nk + g.k - 1, g.s,
-                            [&](uint32_t, uint64_t smer) {
-                                for_each_address_n<NH>(smer, g.nh, amask,
-                                                       [&](uint32_t ad) { atomicAdd(&s_cnt[wbase + ad], inc); });
-                            },
-                            one_word);
-                    } else {  // one block per bin: its slots span both halves
-                        for_each_smer(
-                            A0, A1, A2, A3, nk + g.k - 1, g.s,
-                            [&](uint32_t, uint64_t smer) {
-                                for_each_address_n<NH>(smer, g.nh, amask, [&](uint32_t ad) {
-                                    atomicAdd(&s_cnt[ad & (half_words - 1)], mult << (16 * (ad >> half_log)));
-                                });
-                            },
-                            one_word);
-                    }
-                });
+        // the n_hashes increments of every s-mer of one record, mult each (the
+        // number of identical records); A0..A3 hold its bases from the first,
+        // hi is the record's high word
+        auto count_record = [&](uint32_t A0, uint32_t A1, uint32_t A2, uint32_t A3, uint32_t hi, uint32_t mult) {
+            const uint32_t nk = (hi & 31) + 1;
+            const uint32_t lbase = ((hi >> 5) & ((1u << lshift) - 1)) << g.bse;
+            with_nh(g.nh, [&](auto nht) {
+                constexpr int NH = decltype(nht)::value;
+                if (lshift > 0) {  // the record's block lies in one half of the bin
+                    const uint32_t wbase = lbase & (half_words - 1), inc = mult << (16 * (lbase >> half_log));
+                    for_each_smer(
+                        A0, A1, A2, A3, nk + g.k - 1, g.s,
+                        [&](uint32_t, uint64_t smer) {
+                            for_each_address_n<NH>(smer, g.nh, amask,
+                                                   [&](uint32_t ad) { atomicAdd(&s_cnt[wbase + ad], inc); });
+                        },
+                        one_word);
+                } else {  // one block per bin: its slots span both halves
+                    for_each_smer(
+                        A0, A1, A2, A3, nk + g.k - 1, g.s,
+                        [&](uint32_t, uint64_t smer) {
+                            for_each_address_n<NH>(smer, g.nh, amask, [&](uint32_t ad) {
+                                atomicAdd(&s_cnt[ad & (half_words - 1)], mult << (16 * (ad >> half_log)));
+                            });
+                        },
+                        one_word);
+                }
             });
+        };
+        for (uint32_t r0 = 0; r0 < nwalk; r0 += step) {
+            // with duplicates (one pass, step = nwalk), the first round's leaders
+            // come from registers and later rounds (rare: an item has at most
+            // 2 nthr records) from HBM
+            walk_records_aligned(
+                a.sorted, a.pk, g.k, rb, min(step, nwalk - r0), s_perm + r0, tid, nthr,
+                [&](uint32_t hi, uint32_t A0, uint32_t A1, uint32_t A2, uint32_t A3, uint32_t ri) {
+                    count_record(A0, A1, A2, A3, hi, dedup ? (s_mult[ri >> 1] >> (16 * (ri & 1))) & 0xFFFFu : 1u);
+                },
+                dedup, lkey, lhi);
             __syncthreads();
             // merge this batch's counts: 32-bit table words of the bin
             const bool write_all = a.fresh && r0 == 0;  // the table holds no counts yet
