@@ -323,6 +323,28 @@ def test_insert_bins_sized_per_batch(gpu, oracle, bits):
     assert np.array_equal(b.counters().astype(np.uint32), of.counters())
 
 
+@pytest.mark.parametrize("bits", [8, 16])
+def test_distinct_records_past_first_walk_round(gpu, oracle, bits):
+    """Reads at 0.1x coverage of a 2 Mbp genome, one hash and k - s = 1: each
+    2^15-slot insert bin holds ~1,350 records, nearly all distinct, in one work
+    item counted by distinct record. The 1,024 threads walk the first 1,024
+    leaders with bases kept from the duplicate search and the rest with bases
+    read again from HBM; counters equal the oracle's."""
+    scb = gpu
+    k, m, s, h, cse, bse = 21, 15, 20, 1, 20, 10
+    g = synth.genome(2_000_000, 94)
+    bases, offsets = synth.fixed_reads(g, 1333, 150, 95, sub_rate=0.01)
+    of = oracle.Filter(oracle.make_cfg(k, m, s, h, bits, cse, bse))
+    ok_kmers, ok_smers = oracle.insert_records(of, bases, offsets)
+    b = scb.SuperCountingBloomBuilder(_cfg(scb, k, m, s, h, bits, cse, bse))
+    b.set_method(scb.METHOD_BATCHED)
+    r = b.add_sequences((bases, offsets))
+    assert (r.inserted_kmers, r.inserted_smers) == (ok_kmers, ok_smers)
+    records = (r.inserted_smers - r.inserted_kmers) // (k - s)
+    assert 1100 * 32 < records < 1800 * 32  # per 2^15-slot bin: past one round, within one item
+    assert np.array_equal(b.counters().astype(np.uint32), of.counters())
+
+
 def _low_complexity_reads(seed):
     """Homopolymers, short tandem repeats (ACGT is its own reverse complement),
     repeats broken by N, by lowercase and by single substitutions, repeats
