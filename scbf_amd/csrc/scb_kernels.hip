@@ -2080,6 +2080,7 @@ __global__ __launch_bounds__(kSpectrumThreads) void k_bin_spectrum(SpecArgs sa, 
     __shared__ uint64_t s_red[3][kSpectrumThreads / 64];
     __shared__ uint16_t s_perm[kMaxItemRecords];
     __shared__ uint32_t s_cls[32];
+    __shared__ uint32_t s_nwalk;
     using T = typename Ctr<BITS>::T;
     const BinArgs &a = sa.b;
     const uint32_t tid = threadIdx.x, nthr = blockDim.x, lane = tid & 63, wid = tid >> 6;
@@ -2096,6 +2097,13 @@ __global__ __launch_bounds__(kSpectrumThreads) void k_bin_spectrum(SpecArgs sa, 
     unsigned long long *head = (unsigned long long *)&a.stats[kStWorkHead];
     for (uint32_t i = tid; i < kSpecHist; i += nthr) s_hist[i] = 0;
     uint64_t queried = 0, distinct = 0;
+    // the distinct-record search of the insert, in the LDS of the staged table
+    // and the key set (which are filled after it): its table and keys, then the
+    // multiplicities and the leaders
+    uint32_t *s_ws = reinterpret_cast<uint32_t *>(s_raw);
+    uint32_t *s_mult = s_ws + kRecDedupWords;
+    uint16_t *s_lead = reinterpret_cast<uint16_t *>(s_mult + kInsertItemRecords / 2);
+    const bool dedup_fits = tab_bytes + kSetCap * 8 >= 4 * (kRecDedupWords + kInsertItemRecords);
     for (;;) {
         if (tid == 0) s_item = atomicAdd(head, 1ull);
         __syncthreads();
@@ -2106,20 +2114,49 @@ __global__ __launch_bounds__(kSpectrumThreads) void k_bin_spectrum(SpecArgs sa, 
         const uint32_t rb = wk.y, re = wk.z;
         const bool global = (wk.w >> 31) || (wk.w & 0x7FFFFFFFu) > kSetDirect;
         const uint32_t *tw = reinterpret_cast<const uint32_t *>(a.table) + bin * uint64_t(bin_slots * (BITS / 8) / 4);
-        constexpr int R = kMaxItemRecords / kSpectrumThreads;
-        uint32_t hi[R];
-        load_hi<R>(a.sorted, rb, re - rb, hi, tid, nthr);
-        stage_to_lds<8>(s_tab, tw, nwords, tid, nthr);
-        if (!global)
-            for (uint32_t i = tid; i < kSetCap; i += nthr) s_set[i] = kEmptyKey;
-        __syncthreads();
+        const uint32_t n = re - rb;
+        // Identical records (same bases, k-mer count and block) have the same
+        // keys with the same estimates: only the first of them (the leader) is
+        // walked; the others add their k-mers to the queried count.
+        const bool dedup = dedup_fits && n > 1 && n <= kInsertItemRecords;
+        uint32_t nwalk = n;
+        uint4 lkey = make_uint4(0, 0, 0, 0);  // this thread's first leader: bases and high word
+        uint32_t lhi = 0;
+        if (dedup) {
+            constexpr int R = kInsertItemRecords / kSpectrumThreads;
+            uint32_t hi[R];
+            find_duplicates<R>(a.sorted, a.pk, rb, n, g.k, s_ws, s_lead, s_mult, tid, nthr, hi);  // (synchronizes)
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const uint32_t i = r * nthr + tid;
+                if (i < n && s_lead[i] != i) queried += (hi[r] & 31) + 1;
+            }
+            order_by_hi<R>(hi, n, s_perm, s_cls, tid, nthr, nullptr, 0, s_lead, &s_nwalk);  // (synchronizes)
+            nwalk = s_nwalk;
+            if (tid < nwalk) {
+                const uint32_t li = s_perm[tid];
+                lkey = dedup_key(s_ws, li), lhi = dedup_hi(s_ws, li);
+            }
+            __syncthreads();
+            stage_to_lds<8>(s_tab, tw, nwords, tid, nthr);
+            if (!global)
+                for (uint32_t i = tid; i < kSetCap; i += nthr) s_set[i] = kEmptyKey;
+            __syncthreads();
+        } else {
+            constexpr int R = kMaxItemRecords / kSpectrumThreads;
+            uint32_t hi[R];
+            load_hi<R>(a.sorted, rb, n, hi, tid, nthr);
+            stage_to_lds<8>(s_tab, tw, nwords, tid, nthr);
+            if (!global)
+                for (uint32_t i = tid; i < kSetCap; i += nthr) s_set[i] = kEmptyKey;
+            __syncthreads();
+            order_by_hi<R>(hi, n, s_perm, s_cls, tid, nthr);  // (synchronizes)
+        }
         const T *tbase = sa.stage ? lt : reinterpret_cast<const T *>(a.table) + (bin << a.bin_log);
-        order_by_hi<R>(hi, re - rb, s_perm, s_cls, tid, nthr);  // (synchronizes)
-        walk_records(a.sorted, a.pk, g.k, rb, re - rb, s_perm, tid, nthr, [&](uint64_t rec, const uint32_t (&W)[5]) {
-            const uint32_t nk = uint32_t((rec >> 32) & 31) + 1;
-            const uint32_t lbase = uint32_t((rec >> 37) & ((1u << lshift) - 1)) << g.bse;
-            uint32_t A0, A1, A2, A3;
-            align_words(W, rec & 0xFFFFFFFFull, A0, A1, A2, A3);
+        walk_records_aligned(a.sorted, a.pk, g.k, rb, nwalk, s_perm, tid, nthr, [&](uint32_t hi, uint32_t A0, uint32_t A1,
+                                                                                   uint32_t A2, uint32_t A3, uint32_t) {
+            const uint32_t nk = (hi & 31) + 1;
+            const uint32_t lbase = ((hi >> 5) & ((1u << lshift) - 1)) << g.bse;
             WinMin<WINC> wm;
             // blocks of at most 2^15 slots use the unrolled chains; larger blocks
             // (addresses above 15 bits) the run-time loop below
@@ -2156,7 +2193,7 @@ __global__ __launch_bounds__(kSpectrumThreads) void k_bin_spectrum(SpecArgs sa, 
                     },
                     smer_tail_one_word(g));
             });
-        });
+        }, dedup, lkey, lhi);
         __syncthreads();
     }
     for (uint32_t i = tid; i < kSpecHist; i += nthr)
