@@ -326,10 +326,12 @@ def test_insert_bins_sized_per_batch(gpu, oracle, bits):
 @pytest.mark.parametrize("bits", [8, 16])
 def test_distinct_records_past_first_walk_round(gpu, oracle, bits):
     """Reads at 0.1x coverage of a 2 Mbp genome, one hash and k - s = 1: each
-    2^15-slot insert bin holds ~1,350 records, nearly all distinct, in one work
+    2^15-slot insert bin holds ~1,390 records, nearly all distinct, in one work
     item counted by distinct record. The 1,024 threads walk the first 1,024
     leaders with bases kept from the duplicate search and the rest with bases
-    read again from HBM; counters equal the oracle's."""
+    read again from HBM; counters equal the oracle's. Abundances and reports of
+    the query equal the oracle's when its bins hold more distinct records than
+    two walk rounds and when they hold two rounds of leaders with copies."""
     scb = gpu
     k, m, s, h, cse, bse = 21, 15, 20, 1, 20, 10
     g = synth.genome(2_000_000, 94)
@@ -343,6 +345,23 @@ def test_distinct_records_past_first_walk_round(gpu, oracle, bits):
     records = (r.inserted_smers - r.inserted_kmers) // (k - s)
     assert 1100 * 32 < records < 1800 * 32  # per 2^15-slot bin: past one round, within one item
     assert np.array_equal(b.counters().astype(np.uint32), of.counters())
+    # queries: the same reads (more distinct records per bin than two rounds of
+    # the query's 512 threads: every record is walked) and 40% of them three
+    # times (~560 distinct records per bin in two rounds, their copies written
+    # from the leaders' estimates)
+    idx = b.into_index()
+    n40 = (offsets.size - 1) * 2 // 5
+    cut = int(offsets[n40])
+    tb = np.concatenate([bases[:cut]] * 3)
+    to = np.concatenate([offsets[:1]] + [offsets[1:n40 + 1] + c * cut for c in range(3)]).astype(np.uint64)
+    for qb, qo in ((bases, offsets), (tb, to)):
+        vals, valid, rep = idx.estimate_sequences(qb, qo)
+        wv, wok = oracle.estimate_records(of, qb, qo, k)
+        assert np.array_equal(valid, wok)
+        assert np.array_equal(vals[wok], wv[wok])
+        assert rep.valid_kmers == int(wok.sum())
+        assert rep.positive_estimates == int((wv[wok] > 0).sum())
+        assert rep.estimate_checksum == int(wv[wok].sum()) & 0xFFFFFFFFFFFFFFFF
 
 
 def _low_complexity_reads(seed):
