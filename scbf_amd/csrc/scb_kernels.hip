@@ -1633,6 +1633,15 @@ static_assert(kInsertItemRecords <= 4096, "table entries hold 12-bit record indi
 // ends. Record i's key (its bases, aligned and masked to its length) stays at
 // dedup_key(s_ws, i) and its high word at dedup_hi(s_ws, i) until s_ws is
 // written again.
+// The insert's search pays while distinct records are a minority (50-60x
+// coverage of short reads: ~40% of a bin's records); long noisy reads make
+// most records distinct, and then it costs more than it saves. A workgroup
+// whose item had more than kDedupMaxLeaders / 8 of its records distinct skips
+// the search for its next items, 8 after the first such item and twice as many
+// after each further one (at most 256), until an item with fewer distinct
+// records resets it (decided per workgroup, from its own items).
+constexpr uint32_t kDedupMaxLeaders = 5;  // eighths
+
 __device__ __forceinline__ const uint4 &dedup_key(const uint32_t *s_ws, uint32_t i) {
     return reinterpret_cast<const uint4 *>(s_ws)[i];
 }
@@ -1726,6 +1735,7 @@ __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(
     const uint32_t amask = (1u << g.bse) - 1;
     const uint64_t nwork = a.stats[kStNWork];
     unsigned long long *head = (unsigned long long *)&a.stats[kStWorkHead];
+    uint32_t skip = 0, backoff = 4;  // items left that skip the distinct-record search (uniform)
     // records per sub-batch: a record has at most w k-mers, so w + k - s s-mers
     const uint32_t rec_incr = (g.w + g.k - g.s) * g.nh;
     const bool one_word = smer_tail_one_word(g);
@@ -1749,8 +1759,9 @@ __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(
         // k-mers plus k - s per record, times n_hashes) is counted by distinct
         // record: each leader adds its multiplicity c at every increment, which
         // equals c single increments (and stays within one 16-bit pass).
-        const bool dedup = !multi && n > 1 && a.bin_kmers &&
+        const bool dedup = skip == 0 && !multi && n > 1 && a.bin_kmers &&
                            (uint64_t(wk.w & 0x7FFFFFFFu) + uint64_t(g.k - g.s) * n) * g.nh <= 65535u;
+        skip -= skip > 0;
         uint32_t nwalk = n;
         bool one = true;
         // with duplicates found, the leader this thread walks first (s_perm
@@ -1763,6 +1774,8 @@ __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(
             find_duplicates<R>(a.sorted, a.pk, rb, n, g.k, s_cnt, s_lead, s_mult, tid, nthr, hi);  // (synchronizes)
             order_by_hi<R>(hi, n, s_perm, s_cls, tid, nthr, nullptr, 0, s_lead, &s_nwalk);  // (synchronizes)
             nwalk = s_nwalk;
+            if (8 * nwalk > kDedupMaxLeaders * n) backoff = min(2 * backoff, 256u), skip = backoff;
+            else backoff = 4;
             // the keys share the LDS of the counts: the first round of the walk
             // takes its leaders' bases from them into registers (no second read
             // from HBM) before the counts are cleared
