@@ -1839,7 +1839,7 @@ __global__ __launch_bounds__(kInsertThreads) __attribute__((amdgpu_waves_per_eu(
             if (!multi && (nwords & 7) == 0) {
                 // 16-byte read-modify-write with kU loads in flight per thread
                 // (at least two vectors per bin, so none straddles the halves)
-                constexpr int kU = BITS == 8 ? 2 : 4;  // a 2^15-slot u8 bin is two vectors per thread
+                constexpr int kU = BITS == 16 ? 4 : 2;  // fewer registers for the u8 and u32 merges
                 uint4 *tv = reinterpret_cast<uint4 *>(tw);
                 const uint32_t nvec = nwords >> 2;
                 for (uint32_t v0 = tid; v0 < nvec; v0 += nthr * kU) {
